@@ -1,0 +1,241 @@
+/*
+ * prt.h — C ABI of the MI355X (gfx950) path-tracing hot path.
+ *
+ * This is the drop-in boundary for prender's render path
+ *   main() -> Render::Run() -> radiance() -> intersect_scene() -> QBVH / Moller-Trumbore
+ * (proj/main.cpp:199, proj/render.h:1125-1455, proj/radiance.h:25-556,
+ *  proj/scene.cpp:83-214, proj/QBVH.cpp:17-159, proj/polygon.h:78-129).
+ *
+ * The reference has no FFI; its seams are C++ functions that read a global
+ * SceneEnv.  Here the scene is an explicit handle built from plain structs whose
+ * fields carry the SceneEnv / Material / Entity keywords of the in-scope subset
+ * (proj/scene_env.h:262-350 defaults, 1392-1520 MATERIAL, 1544-1850 OBJECT).
+ *
+ * Conventions (all entry points):
+ *   - every call returns int status: PRT_OK (0) or a negative PRT_E_* code,
+ *     except prt_scene_create (NULL on failure) and prt_scene_destroy (void);
+ *   - details of the last failure on the calling thread: prt_last_error();
+ *   - the caller keeps ownership of every array it passes; arrays are copied
+ *     to device memory inside the call;
+ *   - a scene handle is not thread-safe and owns one HIP stream; different
+ *     handles may be used from different threads (no library globals besides
+ *     the thread-local error string).
+ */
+#ifndef PRT_H
+#define PRT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PRT_ABI_VERSION 1
+
+/* status codes */
+#define PRT_OK              0
+#define PRT_E_INVALID      -1   /* bad argument / malformed scene description */
+#define PRT_E_HIP          -2   /* HIP runtime error (message in prt_last_error) */
+#define PRT_E_NOMEM        -3   /* device or host allocation failed */
+#define PRT_E_NODEVICE     -4   /* no gfx950 device visible */
+#define PRT_E_UNSUPPORTED  -5   /* feature outside the in-scope subset */
+
+/* Material::reflection_type subset (proj/material.h:19-29). */
+#define PRT_REFLECTION_DIFFUSE     0
+#define PRT_REFLECTION_SPECULAR    1
+#define PRT_REFLECTION_REFRACTION  2
+
+/* EntityType numbering kept from proj/entity.h:17-24. */
+#define PRT_ENTITY_SPHERE   1
+#define PRT_ENTITY_UVPLANE  3
+#define PRT_ENTITY_POLYGON  5
+
+/* Material (proj/material.h:54-240, in-scope fields only).
+ * Keyword semantics (proj/scene_env.h:1437-1491): "reflection diffuse" sets
+ * roughness 1, "reflection specular" roughness 0, "reflection reflection" is
+ * glass with IOR default 1.5.  A DIFFUSE material with roughness 0 is a mirror
+ * (proj/radiance.h:514-529). */
+typedef struct prt_material {
+    double color[3];          /* Material::color (reflectance) */
+    double emission[3];       /* Material::emission */
+    double roughness;         /* Material::roughness */
+    double refractive_index;  /* Material::refractive_index (REFRACTIVE_FORMULA0) */
+    int32_t reflection_type;  /* PRT_REFLECTION_* */
+    int32_t pad_;
+} prt_material;
+
+/* Entity (proj/entity.h:29-70 plus the per-type members).
+ *   SPHERE  (proj/sphere.h:9-21):   p0 = position, radius
+ *   UVPLANE (proj/plane.h:129-166): p0 = org, p1 = U, p2 = V (unnormalised; their
+ *                                   lengths are u_length, v_length)
+ *   POLYGON (proj/polygon.h:708-969 over ObjLoader's Obj): float32 xyz
+ *           vertices, int32 0-based triangle indices, optional per-face RGB
+ *           albedo (float32, n_faces*3) that replaces the material colour the
+ *           way per-vertex colours do in proj/polygon.h:214-225.
+ * normal_vector_inverse is Entity::normal_vector_inverse (1 or -1). */
+typedef struct prt_entity {
+    int32_t type;                  /* PRT_ENTITY_* */
+    int32_t material;              /* index into prt_scene_desc.materials */
+    int32_t normal_vector_inverse; /* 1 or -1 */
+    int32_t n_vertices;
+    double p0[3];
+    double p1[3];
+    double p2[3];
+    double radius;
+    const float* vertices;         /* n_vertices * 3 */
+    const int32_t* indices;        /* n_faces * 3 */
+    const float* face_albedo;      /* n_faces * 3 or NULL */
+    int64_t n_faces;
+} prt_entity;
+
+/* Camera: SceneEnv camera fields (proj/scene_env.h:326-343) consumed by
+ * CameraScreenPram (proj/render.h:70-158).  When camera_angle > 0 it is the
+ * CAMERA_ANGLE keyword in degrees and overrides world_screen_width/height as
+ * in proj/scene_env.h:1881-1886. */
+typedef struct prt_camera {
+    double camera_position[3];
+    double camera_dir[3];          /* normalised by the library, as SceneEnv::Load does */
+    double camera_up[3];
+    double world_screen_width;
+    double world_screen_height;
+    double world_screen_dist;
+    double camera_angle;           /* degrees; <= 0 means "not given" */
+    int32_t image_width;
+    int32_t image_height;
+} prt_camera;
+
+/* Sampling keywords (proj/scene_env.h:221, 329-349). */
+typedef struct prt_sampling {
+    int32_t samples;               /* SAMPLING */
+    int32_t supersamples;          /* SUPERSAMPLING (fixed SSxSS sub-pixel grid) */
+    int32_t depth;                 /* DEPTH: RR starts above it */
+    int32_t depth_limit;           /* DEPTH_LIMIT: RR probability halves per level above it */
+    int32_t next_event_estimation; /* NEXTEVENTESTIMATION */
+    int32_t pad_;
+    double luminance_cutoff;       /* LUMINANCE_CUTOFF (<= 0 disables the clamp) */
+    double sensor_response;
+    double background_color[3];    /* BackgroundColor */
+    double environment_light[3];   /* Environment_Light */
+} prt_sampling;
+
+typedef struct prt_scene_desc {
+    prt_camera camera;
+    prt_sampling sampling;
+    const prt_material* materials;
+    int32_t n_materials;
+    int32_t n_entities;
+    const prt_entity* entities;
+} prt_scene_desc;
+
+/* Ray (proj/ray.h:9-14): origin and direction in double.  Directions are used
+ * as given (the reference normalises before tracing). */
+typedef struct prt_ray {
+    double org[3];
+    double dir[3];
+} prt_ray;
+
+/* Closest hit as intersect_scene() reports it (proj/scene.cpp:83-214):
+ * t = IntersectionPos::distance measured from the nudged origin
+ * org + 1e-6*dir; entity = Intersection::object_id (entity index, -1 = miss);
+ * prim = triangle face index inside a POLYGON entity (-1 otherwise);
+ * u, v = Moller-Trumbore barycentrics for triangles. */
+typedef struct prt_hit {
+    double t;
+    double u;
+    double v;
+    int32_t entity;
+    int32_t prim;
+} prt_hit;
+
+/* Counters of one render / trace call. */
+typedef struct prt_stats {
+    uint64_t camera_samples;   /* W*H*spp */
+    uint64_t primary_rays;     /* camera rays traversed (unique sub-pixel rays) */
+    uint64_t extension_rays;   /* bounce rays traversed */
+    uint64_t shadow_rays;      /* NEE shadow rays traversed */
+    uint64_t pickup_hits;      /* specular-pickup contributions (no extra ray) */
+    uint64_t node_visits;      /* BVH nodes fetched over all traversals */
+    uint64_t tri_tests;        /* triangle tests over all traversals */
+    uint64_t truncated_paths;  /* paths stopped by the safety depth cap */
+    uint64_t bytes_algorithmic;/* 128*nodes + 48*tris + 48*rays */
+    double ms_total;           /* device time of the whole call */
+    double ms_extend;          /* closest-hit kernels */
+    double ms_shade;           /* shading kernels */
+    double ms_shadow;          /* shadow kernels */
+    double ms_other;           /* generate / resolve / clears */
+    int64_t launches_extend;
+} prt_stats;
+
+typedef struct prt_render_opts {
+    uint64_t seed;             /* RNG key */
+    int32_t samples_override;  /* > 0 replaces prt_sampling.samples */
+    int32_t batch_pixels;      /* pixels per wavefront batch (0 = auto) */
+    int32_t max_depth;         /* safety cap on path vertices (0 = 128) */
+    int32_t use_graph;         /* 1 = replay the depth loop as a hipGraph */
+    int32_t timing;            /* 1 = per-kernel device timing into prt_stats */
+    int32_t pad_;
+} prt_render_opts;
+
+/* ---- version / device ---------------------------------------------------- */
+int prt_abi_version(void);
+const char* prt_last_error(void);
+/* Fills name (>= 64 bytes) and the CU count of the current device. */
+int prt_device_info(char* name, int name_len, int* cu_count, int* arch_major, int* arch_minor);
+
+/* ---- scene (replaces SceneEnv::Load + CreateEntList: scene_env.h:357,
+ *      scene.cpp:38; QBVH::Construct QBVH.cpp:161) ------------------------- */
+typedef struct prt_scene prt_scene;
+prt_scene* prt_scene_create(const prt_scene_desc* desc);
+void prt_scene_destroy(prt_scene* scene);
+/* Number of BVH nodes / leaf triangles of the flattened device BVH. */
+int prt_scene_info(const prt_scene* scene, int64_t* n_nodes, int64_t* n_tris,
+                   int64_t* n_lights, int64_t* bvh_bytes);
+
+/* ---- closest-hit query (replaces intersect_scene, scene.cpp:448) --------- */
+/* Host arrays. */
+int prt_trace_closest(prt_scene* scene, const prt_ray* rays, int64_t n,
+                      prt_hit* hits, prt_stats* stats);
+/* Device arrays (pointers from hipMalloc / torch); runs on the scene's stream
+ * and returns after the kernel completes. */
+int prt_trace_closest_device(prt_scene* scene, const prt_ray* d_rays, int64_t n,
+                             prt_hit* d_hits, prt_stats* stats);
+
+/* ---- render (replaces Render::Run, render.h:1125-1455) ------------------- */
+/* out_rgb: W*H*3 float, reference row order (row 0 = top, render.h:1234).
+ * out_var: W*H*3 float or NULL — unbiased variance of each pixel's mean. */
+int prt_render(prt_scene* scene, const prt_render_opts* opts,
+               float* out_rgb, float* out_var, prt_stats* stats);
+/* Same, writing device buffers. */
+int prt_render_device(prt_scene* scene, const prt_render_opts* opts,
+                      float* d_rgb, float* d_var, prt_stats* stats);
+
+/* ---- batched persistent rendering of many scenes ------------------------- */
+typedef struct prt_batch_opts {
+    uint64_t seed;
+    int32_t tile_pixels;     /* pixels per work item (0 = autotuned) */
+    int32_t waves_per_cu;    /* persistent grid size (0 = autotuned) */
+    int32_t queue_rays;      /* in-flight ray capacity (0 = autotuned) */
+    int32_t use_graph;
+} prt_batch_opts;
+/* outs[i]: W_i*H_i*3 float device buffers (one per scene). */
+int prt_render_batch_device(prt_scene* const* scenes, int n_scenes,
+                            const prt_batch_opts* opts, float* const* d_outs,
+                            prt_stats* stats);
+
+/* ---- differentiable rendering (per-face albedo + light emission) --------- */
+/* Replace the per-face albedo of POLYGON entity `entity` (float n_faces*3,
+ * device pointer) and scale factor of the emission of light entity `light`. */
+int prt_set_face_albedo_device(prt_scene* scene, int entity, const float* d_albedo);
+int prt_set_emission(prt_scene* scene, int entity, const double emission[3]);
+/* Path-replay adjoint: given dL/dimage (W*H*3 device floats), accumulate
+ * dL/dalbedo into d_albedo_grad (n_faces*3 device floats of `entity`) and
+ * dL/demission (3 doubles, host) for light `light`. */
+int prt_adjoint_device(prt_scene* scene, const prt_render_opts* opts,
+                       const float* d_dimage, int entity, float* d_albedo_grad,
+                       int light, double* d_emission_grad, prt_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRT_H */

@@ -1,0 +1,132 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/prt_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+product package (prender_amd) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional, Tuple
+
+import numpy as np
+
+from prender_amd.scene import (CDesc, HIT_DTYPE, PrtHit, PrtRay, PrtSceneDesc, RAY_DTYPE,
+                               SceneDesc)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++, no GPU needed)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", HERE] + (["-B"] if force else []))
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_scene_create.restype = C.c_void_p
+        L.orc_scene_create.argtypes = [C.POINTER(PrtSceneDesc)]
+        L.orc_scene_destroy.argtypes = [C.c_void_p]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_trace_closest.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int,
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int]
+        L.orc_render.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                 C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
+        L.orc_radiance_sample.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(C.c_double)]
+        L.orc_camera_rays.argtypes = [C.c_void_p, C.c_void_p]
+        L.orc_qbvh_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+        L.orc_light_order.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int]
+        L.orc_set_max_vertices.argtypes = [C.c_void_p, C.c_int]
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.c_uint32)]
+        _lib = L
+    return _lib
+
+
+def philox4x32_10(ctr, key):
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return [int(x) for x in o]
+
+
+class OracleScene:
+    def __init__(self, desc: SceneDesc):
+        self.desc = desc
+        self._c = CDesc(desc)
+        h = lib().orc_scene_create(self._c.ptr())
+        if not h:
+            raise ValueError("oracle scene create failed: " + lib().orc_last_error().decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_max_vertices(self, cap: int):
+        assert lib().orc_set_max_vertices(self.h, cap) == 0
+
+    def trace(self, rays: np.ndarray, mode: str = "qbvh", threads: int = 0):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(rays.shape[0], dtype=HIT_DTYPE)
+        nodes, tris = C.c_uint64(0), C.c_uint64(0)
+        rc = lib().orc_trace_closest(self.h, rays.ctypes.data, rays.shape[0], hits.ctypes.data,
+                                     0 if mode == "qbvh" else 1, C.byref(nodes), C.byref(tris), threads)
+        assert rc == 0
+        return hits, int(nodes.value), int(tris.value)
+
+    def render(self, seed: int, samples: Optional[int] = None, rows: Optional[Tuple[int, int]] = None,
+               threads: int = 0):
+        W, H = self.desc.camera.width, self.desc.camera.height
+        rgb = np.zeros((H, W, 3), dtype=np.float64)
+        var = np.zeros((H, W, 3), dtype=np.float64)
+        stats = (C.c_uint64 * 7)()
+        r0, r1 = rows if rows is not None else (0, H)
+        rc = lib().orc_render(self.h, seed, samples or 0, r0, r1, rgb.ctypes.data, var.ctypes.data,
+                              stats, threads)
+        assert rc == 0
+        keys = ["primary", "extension", "shadow", "pickup", "nodes", "tris", "truncated"]
+        return rgb, var, dict(zip(keys, [int(x) for x in stats]))
+
+    def radiance_sample(self, seed: int, x: int, y: int, sample: int):
+        out = (C.c_double * 3)()
+        lib().orc_radiance_sample(self.h, seed, x, y, sample, out)
+        return np.array(out[:])
+
+    def camera_rays(self) -> np.ndarray:
+        W, H = self.desc.camera.width, self.desc.camera.height
+        ss = self.desc.sampling.supersamples
+        out = np.zeros(W * H * ss * ss, dtype=RAY_DTYPE)
+        lib().orc_camera_rays(self.h, out.ctypes.data)
+        return out
+
+    def qbvh_info(self, entity: int):
+        n, r, ok = C.c_int64(), C.c_int64(), C.c_int()
+        assert lib().orc_qbvh_info(self.h, entity, C.byref(n), C.byref(r), C.byref(ok)) == 0
+        return int(n.value), int(r.value), bool(ok.value)
+
+    def light_order(self):
+        buf = (C.c_int32 * 256)()
+        n = lib().orc_light_order(self.h, buf, 256)
+        return [int(buf[i]) for i in range(n)]

@@ -1,0 +1,270 @@
+"""Pins the FP64 CPU oracle (oracle/prt_oracle.cpp) before it is trusted.
+
+Each check is chosen so that a plausible restatement mistake fails it: a
+dropped term or wrong sign in Moller-Trumbore, the sphere root choice, the
+UVPlane bound quirk, the camera basis, the QBVH (closest hit == brute force,
+the reference's own invariant, scene.cpp:122-141), Russian roulette and the
+recursion (closed-form furnaces), NEE (closed-form direct light of a sphere
+light) and the counter-based RNG (Random123 known-answer vectors).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleScene, philox4x32_10
+from prender_amd import generators as G
+from prender_amd.scene import (Camera, Material, Polygon, SceneDesc, Sampling, Sphere, UVPlane,
+                               rays_to_array)
+
+
+# ---------------------------------------------------------------- RNG KATs
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32 R=10
+    assert philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert philox4x32_10([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert philox4x32_10([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+# ---------------------------------------------------------------- helpers
+def tri_scene(verts, faces, extra=()):
+    mats = [Material(color=(0.5, 0.5, 0.5))]
+    ents = [Polygon(np.asarray(verts, np.float32), np.asarray(faces, np.int32), 0)] + list(extra)
+    return SceneDesc(camera=Camera(width=4, height=4), sampling=Sampling(samples=1), materials=mats,
+                     entities=ents)
+
+
+def trace_one(o, org, d, mode="qbvh"):
+    h, _, _ = o.trace(rays_to_array(np.array([org]), np.array([d])), mode)
+    return h[0]
+
+
+# ---------------------------------------------------------------- Moller-Trumbore
+def test_mt_barycentric_closed_form():
+    # triangle in z=0; a ray along -z through barycentric (u,v) hits at
+    # P = v0 + u e1 + v e2 with t = z0 - 1e-6 (the origin is nudged by 1e-6*dir)
+    v = [[0, 0, 0], [2, 0, 0], [0, 4, 0]]
+    o = OracleScene(tri_scene(v, [[0, 1, 2]]))
+    for (u, w) in [(0.25, 0.5), (0.1, 0.2), (0.5, 0.25)]:
+        P = np.array([2 * u, 4 * w, 0.0])
+        h = trace_one(o, P + [0, 0, 3.0], [0, 0, -1.0])
+        assert h["entity"] == 0 and h["prim"] == 0
+        assert h["t"] == pytest.approx(3.0 - 1e-6, abs=1e-12)
+        assert h["u"] == pytest.approx(u, abs=1e-12)
+        assert h["v"] == pytest.approx(w, abs=1e-12)
+
+
+def test_mt_two_sided_and_edges():
+    v = [[0, 0, 0], [1, 0, 0], [0, 1, 0]]
+    o = OracleScene(tri_scene(v, [[0, 1, 2]]))
+    # back face (det < 0 branch) hits too
+    h = trace_one(o, [0.2, 0.2, -1.0], [0, 0, 1.0])
+    assert h["entity"] == 0 and h["t"] == pytest.approx(1.0 - 1e-6, abs=1e-12)
+    # outside across the hypotenuse u + v > 1 misses; just inside hits
+    assert trace_one(o, [0.5001, 0.5001, 1.0], [0, 0, -1.0])["entity"] == -1
+    assert trace_one(o, [0.4999, 0.4999, 1.0], [0, 0, -1.0])["entity"] == 0
+    # u < 0 side misses
+    assert trace_one(o, [-1e-4, 0.3, 1.0], [0, 0, -1.0])["entity"] == -1
+    # parallel ray: det = 0 -> miss
+    assert trace_one(o, [-1.0, 0.2, 0.0], [1.0, 0, 0])["entity"] == -1
+    # behind the origin: t < 0 -> miss
+    assert trace_one(o, [0.2, 0.2, -1.0], [0, 0, -1.0])["entity"] == -1
+
+
+def test_mt_det_epsilon():
+    # |det| = |e1 . (d x e2)|; for a ray along -z and a triangle in z=0 with
+    # legs a, b: det = a*b.  The reference rejects |det| <= 1e-7 (PS_EPS*0.1).
+    a = 1e-4
+    for b, expect in [(0.5e-3, -1), (2e-3, 0)]:  # det = 5e-8 (reject), 2e-7 (accept)
+        o = OracleScene(tri_scene([[0, 0, 0], [a, 0, 0], [0, b, 0]], [[0, 1, 2]]))
+        h = trace_one(o, [a / 4, b / 4, 1.0], [0, 0, -1.0])
+        assert h["entity"] == expect
+
+
+def test_t_epsilon():
+    # hits are accepted only for t > 1e-6 from the nudged origin
+    v = [[-1, -1, 0], [1, -1, 0], [0, 1, 0]]
+    o = OracleScene(tri_scene(v, [[0, 1, 2]]))
+    assert trace_one(o, [0, 0, 1.5e-6], [0, 0, -1.0])["entity"] == -1     # t = 0.5e-6
+    assert trace_one(o, [0, 0, 3e-6], [0, 0, -1.0])["entity"] == 0         # t = 2e-6
+
+
+# ---------------------------------------------------------------- sphere / uvplane
+def test_sphere_roots():
+    d = SceneDesc(camera=Camera(width=4, height=4), sampling=Sampling(samples=1),
+                  materials=[Material(color=(0.5,) * 3)], entities=[Sphere((0, 0, 10), 2.0, 0)])
+    o = OracleScene(d)
+    h = trace_one(o, [0, 0, 0], [0, 0, 1.0])
+    assert h["entity"] == 0 and h["t"] == pytest.approx(8.0 - 1e-6, abs=1e-9)
+    # from inside: far root
+    h = trace_one(o, [0, 0, 10], [0, 0, 1.0])
+    assert h["t"] == pytest.approx(2.0 - 1e-6, abs=1e-9)
+    # big wall sphere (r = 1e5) is computed in double as the reference does
+    d2 = SceneDesc(camera=Camera(width=4, height=4), sampling=Sampling(samples=1),
+                   materials=[Material(color=(0.5,) * 3)], entities=[Sphere((1e5 + 1, 40.8, 81.6), 1e5, 0)])
+    h = trace_one(OracleScene(d2), [50, 40.8, 81.6], [-1.0, 0, 0])
+    assert h["t"] == pytest.approx(49.0, abs=1e-5)
+
+
+def test_uvplane_bound_quirk():
+    # UVPlane::intersect divides the projection by u_length and then bounds it
+    # by u_length (plane.h:232-238): with |U| = 0.5 the hittable extent is 0.25.
+    d = SceneDesc(camera=Camera(width=4, height=4), sampling=Sampling(samples=1),
+                  materials=[Material(color=(0.5,) * 3)],
+                  entities=[UVPlane((0, 0, 0), (0.5, 0, 0), (0, 0.5, 0), 0)])
+    o = OracleScene(d)
+    assert trace_one(o, [0.2, 0.2, 1], [0, 0, -1.0])["entity"] == 0
+    assert trace_one(o, [0.3, 0.2, 1], [0, 0, -1.0])["entity"] == -1
+    assert trace_one(o, [0.2, 0.3, 1], [0, 0, -1.0])["entity"] == -1
+    # |U| = 2: extent grows to 4
+    d.entities = [UVPlane((0, 0, 0), (2.0, 0, 0), (0, 2.0, 0), 0)]
+    o = OracleScene(d)
+    assert trace_one(o, [3.9, 3.9, 1], [0, 0, -1.0])["entity"] == 0
+    assert trace_one(o, [4.1, 1.0, 1], [0, 0, -1.0])["entity"] == -1
+
+
+def test_nearest_entity_wins():
+    d = SceneDesc(camera=Camera(width=4, height=4), sampling=Sampling(samples=1),
+                  materials=[Material(color=(0.5,) * 3)],
+                  entities=[Sphere((0, 0, 10), 1.0, 0), Sphere((0, 0, 5), 1.0, 0)])
+    h = trace_one(OracleScene(d), [0, 0, 0], [0, 0, 1.0])
+    assert h["entity"] == 1
+
+
+# ---------------------------------------------------------------- QBVH
+@pytest.fixture(scope="module")
+def cfg2_small():
+    return G.config2(width=64, height=64, samples=1, level=5)
+
+
+def test_qbvh_equals_brute_force(cfg2_small):
+    o = OracleScene(cfg2_small)
+    rays = G.hit_parity_rays(cfg2_small, n=6000)
+    hq, nq, tq = o.trace(rays, "qbvh")
+    hb, _, _ = o.trace(rays, "brute")
+    assert (hq["entity"] == hb["entity"]).all()
+    same = hq["prim"] == hb["prim"]
+    # closest-hit ties are allowed to differ only where t agrees exactly
+    assert (same | (hq["t"] == hb["t"])).all()
+    assert (hq["t"] == hb["t"]).all()
+    assert (hq["entity"] >= 0).mean() > 0.9
+    assert tq < 0.05 * hb.shape[0] * cfg2_small.n_triangles  # the QBVH prunes
+
+
+def test_qbvh_structure(cfg2_small):
+    o = OracleScene(cfg2_small)
+    n_nodes, refs, once = o.qbvh_info(1)
+    nf = cfg2_small.entities[1].indices.shape[0]
+    assert once and refs == nf
+    assert 0 < n_nodes < nf
+
+
+# ---------------------------------------------------------------- camera
+def test_camera_rays_closed_form():
+    d = G.config1(width=16, height=12)
+    o = OracleScene(d)
+    rays = o.camera_rays()
+    pos, center, sx, sy = G.camera_basis(d)
+    W, H, ss = 16, 12, 2
+    k = 0
+    for y in range(H):
+        for x in range(W):
+            for sy_ in range(ss):
+                for sx_ in range(ss):
+                    r1, r2 = (sx_ + 0.5) / ss, (sy_ + 0.5) / ss
+                    sp = center + sx * ((r1 + x) / W - 0.5) + sy * ((r2 + y) / H - 0.5)
+                    dd = (sp - pos) / np.linalg.norm(sp - pos)
+                    np.testing.assert_allclose(rays[k]["dir"], dd, atol=1e-14)
+                    k += 1
+    # vertical FOV 30 deg: the top-centre edge direction makes 15 deg with the axis
+    c = np.asarray(d.camera.dir) / np.linalg.norm(d.camera.dir)
+    top = center + sy * 0.5 - pos
+    assert math.degrees(math.acos(top @ c / np.linalg.norm(top))) == pytest.approx(15.0, abs=1e-9)
+
+
+def test_light_order_by_power():
+    d = G.config3(width=8, height=8, samples=1, scale=0.02)
+    o = OracleScene(d)
+    order = o.light_order()
+    def power(i):
+        e = d.entities[i]
+        m = d.materials[e.material]
+        ul, vl = np.linalg.norm(e.U), np.linalg.norm(e.V)
+        return ul * vl * np.linalg.norm(m.emission)
+    p = [power(i) for i in order]
+    assert len(order) == 9 and p == sorted(p, reverse=True)
+
+
+# ---------------------------------------------------------------- transport closed forms
+def enclosure(mat: Material, nee=0, samples=8):
+    """Camera inside a radius-10 sphere of material ``mat``."""
+    ents = [Sphere((0, 0, 0), 10.0, 0)]
+    cam = Camera(position=(0, 0, 0), dir=(0, 0, -1.0), up=(0, 1, 0), camera_angle=60.0, width=8, height=8,
+                 world_screen_dist=1.0)
+    smp = Sampling(samples=samples, supersamples=2, depth=5, depth_limit=8, next_event_estimation=nee,
+                   luminance_cutoff=0.0)
+    return SceneDesc(camera=cam, sampling=smp, materials=[mat], entities=ents)
+
+
+def test_diffuse_furnace():
+    # L = E + rho * L  =>  E / (1 - rho); Russian roulette returns the emission
+    # on termination (radiance.h:106-116), which keeps the estimator unbiased.
+    E, rho = 0.1, 0.5
+    o = OracleScene(enclosure(Material(color=(rho,) * 3, emission=(E,) * 3), samples=64))
+    rgb, var, st = o.render(seed=7)
+    m = rgb.mean()
+    se = math.sqrt(var.mean() / rgb[..., 0].size)
+    assert abs(m - E / (1 - rho)) < 5 * se + 1e-9
+    assert st["truncated"] == 0
+
+
+def test_specular_furnace_exact():
+    # mirror enclosure: L = E + c L  (deterministic directions); with c = 0.5
+    # the series is E/(1-c) up to Russian roulette noise only
+    E, c = 0.1, 0.5
+    o = OracleScene(enclosure(Material(color=(c,) * 3, emission=(E,) * 3, reflection="specular"), samples=64))
+    rgb, var, _ = o.render(seed=3)
+    se = math.sqrt(var.mean() / rgb[..., 0].size)
+    assert abs(rgb.mean() - E / (1 - c)) < 5 * se + 1e-9
+
+
+def test_nee_sphere_light_closed_form():
+    # Diffuse floor lit by a sphere light straight above, seen at depth 0:
+    # E[direct] = rho * Le * (r/D)^2 * cos(theta)  (projected solid angle of a
+    # sphere); bounce rays can only reach the light at depth > 0, which NEE
+    # zeroes (brdf.h:443-461), so the pixel value is the direct term only.
+    rho, Le, r, D = 0.5, 20.0, 1.0, 5.0
+    mats = [Material(color=(rho,) * 3), Material(color=(0, 0, 0), emission=(Le,) * 3)]
+    floor = UVPlane((-30, 0, -30), (0, 0, math.sqrt(60.0)), (math.sqrt(60.0), 0, 0), 0)  # normal +y, 60x60
+    ents = [floor, Sphere((0, D, 0), r, 1)]
+    cam = Camera(position=(0, 2.0, 0.0), dir=(0, -1.0, 1e-3), up=(0, 0, -1.0), camera_angle=2.0, width=4, height=4,
+                 world_screen_dist=1.0)
+    smp = Sampling(samples=256, supersamples=1, depth=5, depth_limit=8, next_event_estimation=1,
+                   luminance_cutoff=0.0)
+    o = OracleScene(SceneDesc(camera=cam, sampling=smp, materials=mats, entities=ents))
+    rgb, var, st = o.render(seed=11)
+    expect = rho * Le * (r / D) ** 2 * 1.0
+    se = math.sqrt(var.mean() / rgb[..., 0].size)
+    assert st["shadow"] > 0
+    assert abs(rgb.mean() - expect) < 5 * se + 1e-3 * expect
+
+
+def test_render_deterministic_and_thread_independent():
+    d = G.config1(width=24, height=18, samples=2)
+    o = OracleScene(d)
+    a, _, _ = o.render(seed=5, threads=1)
+    b, _, _ = o.render(seed=5, threads=4)
+    c, _, _ = o.render(seed=6, threads=4)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, c)
+
+
+def test_render_row_band_matches_full():
+    d = G.config1(width=24, height=18, samples=2)
+    o = OracleScene(d)
+    full, _, _ = o.render(seed=5)
+    band, _, _ = o.render(seed=5, rows=(4, 9))
+    H = 18
+    for y in range(4, 9):
+        assert np.array_equal(full[H - 1 - y], band[H - 1 - y])
