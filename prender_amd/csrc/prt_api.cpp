@@ -1,0 +1,735 @@
+// prt_api.cpp — the C ABI of include/prt.h: scene upload, closest-hit
+// queries and the wavefront render loop on one HIP stream.
+//
+// Scene semantics follow SceneEnv::Load (proj/scene_env.h:357-1990): the
+// light list holds the light-material spheres and uvplanes in entity order,
+// sorted by power = area * |emission| descending (light.h:95-99,
+// scene_env.h:1567-1719); the camera is CameraScreenPram (render.h:100-146)
+// after SceneEnv's CAMERA_ANGLE handling and normalisation
+// (scene_env.h:1881-1886, 1920).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/prt.h"
+#include "bvh.h"
+#include "prt_device.h"
+#include "prt_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(PRT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+// Host Vector3d with the reference's operation order (vector3d.h).
+struct V3 {
+  double x, y, z;
+};
+V3 mk(double a, double b, double c) { return V3{a, b, c}; }
+V3 operator+(V3 a, V3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+V3 operator-(V3 a, V3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+V3 operator*(V3 a, double s) { return mk(a.x * s, a.y * s, a.z * s); }
+double dot(V3 a, V3 b) { return (a.x * b.x + a.z * b.z) + a.y * b.y; }
+double length(V3 a) { return std::sqrt((a.x * a.x + a.z * a.z) + a.y * a.y); }
+V3 normalize(V3 a) { return a * (1.0 / length(a)); }
+V3 cross(V3 a, V3 b) { return mk((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x)); }
+V3 from(const double* p) { return mk(p[0], p[1], p[2]); }
+void put(double* d, V3 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; }
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+int dev_alloc(DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return PRT_OK;
+  b.release();
+  if (bytes == 0) return PRT_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(PRT_E_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return PRT_OK;
+}
+
+template <class T>
+int dev_upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  int rc = dev_alloc(b, std::max<size_t>(v.size() * sizeof(T), 256));
+  if (rc) return rc;
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return PRT_OK;
+}
+
+// Render workspace sized for a batch of samples.
+struct Workspace {
+  int64_t batch_samples = 0;  // camera samples per batch
+  int64_t queue_cap = 0;      // rays per queue
+  int64_t shadow_cap = 0;
+  DevBuf q[2][14];            // two ray queues (SoA)
+  DevBuf hit_t, hit_id;
+  DevBuf sq[11];
+  DevBuf accum;
+  DevBuf counts;              // uint32 [4]: q0 count, q1 count, shadow count, spare
+  DevBuf fetch;               // uint32 [512] work-fetch counters, one per launch
+  DevBuf ctr;                 // uint64 [CTR_COUNT]
+  uint32_t* h_counts = nullptr;
+  void release() {
+    for (auto& a : q)
+      for (auto& b : a) b.release();
+    hit_t.release();
+    hit_id.release();
+    for (auto& b : sq) b.release();
+    accum.release();
+    counts.release();
+    fetch.release();
+    ctr.release();
+    if (h_counts) (void)hipHostFree(h_counts);
+    h_counts = nullptr;
+    batch_samples = queue_cap = shadow_cap = 0;
+  }
+};
+
+RayQ ray_queue(DevBuf* b) {
+  RayQ q;
+  q.ox = (double*)b[0].p; q.oy = (double*)b[1].p; q.oz = (double*)b[2].p;
+  q.dx = (double*)b[3].p; q.dy = (double*)b[4].p; q.dz = (double*)b[5].p;
+  q.tr = (float*)b[6].p; q.tg = (float*)b[7].p; q.tb = (float*)b[8].p;
+  q.pr = (float*)b[9].p; q.pg = (float*)b[10].p; q.pb = (float*)b[11].p;
+  q.slot = (uint32_t*)b[12].p;
+  q.meta = (uint32_t*)b[13].p;
+  return q;
+}
+ShadowQ shadow_queue(DevBuf* b) {
+  ShadowQ q;
+  q.ox = (double*)b[0].p; q.oy = (double*)b[1].p; q.oz = (double*)b[2].p;
+  q.dx = (double*)b[3].p; q.dy = (double*)b[4].p; q.dz = (double*)b[5].p;
+  q.tmax = (double*)b[6].p;
+  q.cr = (float*)b[7].p; q.cg = (float*)b[8].p; q.cb = (float*)b[9].p;
+  q.slot = (uint32_t*)b[10].p;
+  return q;
+}
+
+}  // namespace
+
+struct prt_scene {
+  int device = 0;
+  int cu_count = 0;
+  hipStream_t stream = nullptr;
+  Params P{};  // scene + sampling part; per-call fields are patched in
+  DevBuf nodes, tris, tri_nrm, tri_alb, ents, mats, analytic, lights, ovf;
+  std::vector<DevEntity> h_ents;
+  std::vector<DevMaterial> h_mats;
+  std::vector<int32_t> h_lights;
+  std::vector<int64_t> face_offset;  // per entity: first slot index is not contiguous; kept for albedo updates
+  std::vector<uint32_t> slot_of_tri; // global input triangle -> leaf slot
+  std::vector<int64_t> ent_first_tri;// per entity: first global input triangle
+  std::vector<int64_t> ent_n_tris;
+  int64_t n_tris = 0;
+  int bvh_depth = 0;
+  double bvh_ms = 0;
+  bool has_glass = false;
+  int max_trav_blocks = 0;
+  Workspace ws;
+  DevBuf out_rgb, out_var;  // staging for host-output renders
+};
+
+extern "C" {
+
+int prt_abi_version(void) { return PRT_ABI_VERSION; }
+const char* prt_last_error(void) { return g_err.c_str(); }
+
+int prt_device_info(char* name, int name_len, int* cu_count, int* arch_major, int* arch_minor) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(PRT_E_NODEVICE, hipGetErrorString(e));
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, dev));
+  if (name && name_len > 0) {
+    std::snprintf(name, name_len, "%s", p.gcnArchName);
+  }
+  if (cu_count) *cu_count = p.multiProcessorCount;
+  if (arch_major) *arch_major = p.major;
+  if (arch_minor) *arch_minor = p.minor;
+  return PRT_OK;
+}
+
+void prt_scene_destroy(prt_scene* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  s->nodes.release(); s->tris.release(); s->tri_nrm.release(); s->tri_alb.release();
+  s->ents.release(); s->mats.release(); s->analytic.release(); s->lights.release(); s->ovf.release();
+  s->ws.release();
+  s->out_rgb.release();
+  s->out_var.release();
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+static int scene_build(prt_scene* s, const prt_scene_desc* d) {
+  if (!d) return fail(PRT_E_INVALID, "null scene description");
+  const prt_camera& cam = d->camera;
+  const prt_sampling& smp = d->sampling;
+  if (cam.image_width <= 0 || cam.image_height <= 0) return fail(PRT_E_INVALID, "image size must be positive");
+  if (smp.samples <= 0 || smp.supersamples <= 0) return fail(PRT_E_INVALID, "samples and supersamples must be positive");
+  if ((int64_t)smp.samples * smp.supersamples * smp.supersamples > (1 << 20))
+    return fail(PRT_E_INVALID, "samples per pixel above 2^20");
+  if (d->n_materials <= 0 || !d->materials) return fail(PRT_E_INVALID, "no materials");
+  if (d->n_entities < 0 || (d->n_entities > 0 && !d->entities)) return fail(PRT_E_INVALID, "bad entity list");
+
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(PRT_E_NODEVICE, hipGetErrorString(e));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(PRT_E_NODEVICE, std::string("libprt is built for gfx950, device is ") + prop.gcnArchName);
+  s->device = dev;
+  s->cu_count = prop.multiProcessorCount;
+  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+
+  // materials (Material::isLight, material.h:195-210)
+  s->h_mats.resize(d->n_materials);
+  for (int i = 0; i < d->n_materials; ++i) {
+    const prt_material& pm = d->materials[i];
+    DevMaterial& m = s->h_mats[i];
+    std::memcpy(m.color, pm.color, sizeof(m.color));
+    std::memcpy(m.emission, pm.emission, sizeof(m.emission));
+    m.roughness = pm.roughness;
+    m.ior = pm.refractive_index;
+    m.type = pm.reflection_type;
+    if (m.type < PRT_REFLECTION_DIFFUSE || m.type > PRT_REFLECTION_REFRACTION)
+      return fail(PRT_E_UNSUPPORTED, "reflection_type outside diffuse/specular/refraction");
+    m.is_light = (m.type == PRT_REFLECTION_DIFFUSE && length(from(pm.color)) < 0.001 &&
+                  length(from(pm.emission)) > 0.001) ? 1 : 0;
+    if (m.type == PRT_REFLECTION_REFRACTION) s->has_glass = true;
+  }
+
+  // entities, triangles
+  std::vector<int32_t> analytic;
+  std::vector<float> v9;         // per input triangle: v0 v1 v2
+  std::vector<int32_t> t_ent, t_face;
+  std::vector<float> t_nrm, t_alb;
+  bool any_albedo = false;
+  s->h_ents.resize(d->n_entities);
+  s->ent_first_tri.assign(d->n_entities, -1);
+  s->ent_n_tris.assign(d->n_entities, 0);
+  for (int i = 0; i < d->n_entities; ++i) {
+    const prt_entity& pe = d->entities[i];
+    DevEntity& de = s->h_ents[i];
+    std::memset(&de, 0, sizeof(de));
+    de.type = pe.type;
+    de.material = pe.material;
+    de.ninv = pe.normal_vector_inverse < 0 ? -1 : 1;
+    if (pe.material < 0 || pe.material >= d->n_materials) return fail(PRT_E_INVALID, "entity material out of range");
+    de.light = s->h_mats[pe.material].is_light;
+    if (pe.type == PRT_ENTITY_SPHERE) {
+      if (!(pe.radius > 0)) return fail(PRT_E_INVALID, "sphere radius must be positive");
+      std::memcpy(de.p0, pe.p0, sizeof(de.p0));
+      de.radius = pe.radius;
+      de.area = (12.5663706143591729538505735331180115 * pe.radius * pe.radius);  // sphere.h:23-27
+      analytic.push_back(i);
+    } else if (pe.type == PRT_ENTITY_UVPLANE) {
+      // UVPlane ctor + CalcArea, plane.h:143-166
+      const V3 U = from(pe.p1), V = from(pe.p2);
+      de.u_len = length(U);
+      de.v_len = length(V);
+      if (!(de.u_len > 0) || !(de.v_len > 0)) return fail(PRT_E_INVALID, "uvplane axes must be non-zero");
+      const V3 ua = normalize(U), va = normalize(V);
+      put(de.u_axis, ua);
+      put(de.v_axis, va);
+      put(de.normal, normalize(cross(ua, va)));
+      std::memcpy(de.p0, pe.p0, sizeof(de.p0));
+      de.area = length(ua * de.u_len) * length(va * de.v_len) * (1.0 - std::pow(dot(ua, va), 2.0));
+      analytic.push_back(i);
+    } else if (pe.type == PRT_ENTITY_POLYGON) {
+      if (pe.n_faces <= 0 || pe.n_vertices <= 0 || !pe.vertices || !pe.indices)
+        return fail(PRT_E_INVALID, "polygon entity without vertices or faces");
+      if (de.light && smp.next_event_estimation)
+        return fail(PRT_E_UNSUPPORTED, "POLYGON lights cannot be sampled by NEE (directLight_radiance.h:329)");
+      de.has_albedo = pe.face_albedo ? 1 : 0;
+      any_albedo |= (pe.face_albedo != nullptr);
+      s->ent_first_tri[i] = (int64_t)t_ent.size();
+      s->ent_n_tris[i] = pe.n_faces;
+      for (int64_t f = 0; f < pe.n_faces; ++f) {
+        const int32_t* ix = pe.indices + 3 * f;
+        for (int k = 0; k < 3; ++k)
+          if (ix[k] < 0 || ix[k] >= pe.n_vertices) return fail(PRT_E_INVALID, "triangle index out of range");
+        const float* a = pe.vertices + 3 * (size_t)ix[0];
+        const float* b = pe.vertices + 3 * (size_t)ix[1];
+        const float* c = pe.vertices + 3 * (size_t)ix[2];
+        v9.insert(v9.end(), {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2]});
+        t_ent.push_back(i);
+        t_face.push_back((int32_t)f);
+        // f32 face normal, ObjLoader.cpp:577-602 + 688-698
+        const float dx1 = b[0] - a[0], dy1 = b[1] - a[1], dz1 = b[2] - a[2];
+        const float dx2 = c[0] - a[0], dy2 = c[1] - a[1], dz2 = c[2] - a[2];
+        float nx = dy1 * dz2 - dz1 * dy2;
+        float ny = dz1 * dx2 - dx1 * dz2;
+        float nz = dx1 * dy2 - dy1 * dx2;
+        const float len = std::sqrt(nx * nx + ny * ny + nz * nz);
+        if (len != 0.0f) { nx /= len; ny /= len; nz /= len; }
+        t_nrm.insert(t_nrm.end(), {nx, ny, nz});
+        if (pe.face_albedo) {
+          const float* al = pe.face_albedo + 3 * f;
+          t_alb.insert(t_alb.end(), {al[0], al[1], al[2]});
+        } else {
+          t_alb.insert(t_alb.end(), {0.0f, 0.0f, 0.0f});
+        }
+      }
+    } else {
+      return fail(PRT_E_UNSUPPORTED, "entity type outside sphere/uvplane/polygon");
+    }
+  }
+  s->n_tris = (int64_t)t_ent.size();
+
+  // lights: light-material spheres and uvplanes, power-descending (stable)
+  struct L { int32_t ent; double power; };
+  std::vector<L> ls;
+  for (int i = 0; i < d->n_entities; ++i) {
+    const DevEntity& de = s->h_ents[i];
+    if (!de.light || de.type == PRT_ENTITY_POLYGON) continue;
+    ls.push_back(L{i, de.area * length(from(s->h_mats[de.material].emission))});
+  }
+  std::stable_sort(ls.begin(), ls.end(), [](const L& a, const L& b) { return a.power > b.power; });
+  s->h_lights.clear();
+  for (const L& l : ls) s->h_lights.push_back(l.ent);
+
+  // BVH over all triangles, leaf-ordered triangle records
+  BvhOutput bo;
+  if (s->n_tris > 0) {
+    BvhInput bi{v9.data(), s->n_tris, 0};
+    const char* err = nullptr;
+    if (!prt_build_bvh(bi, bo, &err)) return fail(PRT_E_UNSUPPORTED, err ? err : "bvh build failed");
+  }
+  s->bvh_depth = bo.depth;
+  s->bvh_ms = bo.build_ms;
+  std::vector<float> tri4((size_t)s->n_tris * 12);
+  std::vector<float> nrm((size_t)s->n_tris * 3), alb(any_albedo ? (size_t)s->n_tris * 3 : 0);
+  s->slot_of_tri.assign(s->n_tris, 0);
+  for (int64_t slot = 0; slot < s->n_tris; ++slot) {
+    const uint32_t t = bo.order[slot];
+    s->slot_of_tri[t] = (uint32_t)slot;
+    float* r = &tri4[12 * slot];
+    const float* v = &v9[9 * (size_t)t];
+    int32_t ent = t_ent[t], face = t_face[t], zero = 0;
+    r[0] = v[0]; r[1] = v[1]; r[2] = v[2]; std::memcpy(&r[3], &ent, 4);
+    r[4] = v[3]; r[5] = v[4]; r[6] = v[5]; std::memcpy(&r[7], &face, 4);
+    r[8] = v[6]; r[9] = v[7]; r[10] = v[8]; std::memcpy(&r[11], &zero, 4);
+    for (int k = 0; k < 3; ++k) nrm[3 * slot + k] = t_nrm[3 * (size_t)t + k];
+    if (any_albedo)
+      for (int k = 0; k < 3; ++k) alb[3 * slot + k] = t_alb[3 * (size_t)t + k];
+  }
+
+  hipStream_t st = s->stream;
+  int rc;
+  if ((rc = dev_upload(s->nodes, bo.nodes, st))) return rc;
+  if ((rc = dev_upload(s->tris, tri4, st))) return rc;
+  if ((rc = dev_upload(s->tri_nrm, nrm, st))) return rc;
+  if (any_albedo && (rc = dev_upload(s->tri_alb, alb, st))) return rc;
+  if ((rc = dev_upload(s->ents, s->h_ents, st))) return rc;
+  if ((rc = dev_upload(s->mats, s->h_mats, st))) return rc;
+  if ((rc = dev_upload(s->analytic, analytic, st))) return rc;
+  if ((rc = dev_upload(s->lights, s->h_lights, st))) return rc;
+  // traversal stack overflow: one area per thread of the largest grid
+  s->max_trav_blocks = std::max(1, s->cu_count) * 8;
+  const int64_t ovf_threads = (int64_t)s->max_trav_blocks * 256;
+  if ((rc = dev_alloc(s->ovf, (size_t)ovf_threads * 2 * (PRT_STACK_TOTAL - PRT_STACK_LDS) * sizeof(uint32_t))))
+    return rc;
+  HIPCHK(hipStreamSynchronize(st));
+
+  // Params: camera (CameraScreenPram, render.h:100-146; scene_env.h:1881-1886, 1920)
+  Params& P = s->P;
+  std::memset(&P, 0, sizeof(P));
+  V3 cpos = from(cam.camera_position), cdir = normalize(from(cam.camera_dir)), cup = from(cam.camera_up);
+  double wsw = cam.world_screen_width, wsh = cam.world_screen_height;
+  if (cam.camera_angle > 0.0) {
+    wsw = 2.0 * cam.world_screen_dist * std::tan(cam.camera_angle * (3.14159265358979323846264338327950288 / 180.0) / 2.0);
+    wsh = wsw;
+  }
+  const double PI = 3.14159265358979323846264338327950288;
+  double cam_phi = std::atan2(cdir.y, cdir.x);
+  if (cam_phi < 0.0) cam_phi += 2.0 * PI;
+  if (std::fabs(cam_phi - PI) < 1.0e-6 || cam_phi < 1.0e-6 || std::fabs(cam_phi - 2.0 * PI) < 1.0e-6) {
+    cdir.y += 1.0e-6;
+    const double len = std::sqrt(cdir.x * cdir.x + cdir.y * cdir.y + cdir.z * cdir.z);
+    cdir.x /= len; cdir.y /= len; cdir.z /= len;
+  }
+  const double screen_width = wsw * cam.image_width / cam.image_height;
+  const double screen_height = wsh;
+  const V3 sx = normalize(cross(cdir, cup)) * screen_width;
+  const V3 sy = normalize(cross(sx, cdir)) * screen_height;
+  put(P.cam_pos, cpos);
+  put(P.screen_x, sx);
+  put(P.screen_y, sy);
+  put(P.center, cpos + cdir * cam.world_screen_dist);
+  P.iw = 1.0 / cam.image_width;
+  P.ih = 1.0 / cam.image_height;
+  P.W = cam.image_width;
+  P.H = cam.image_height;
+  P.SS = smp.supersamples;
+  P.samples = smp.samples;
+  P.spp = smp.samples * smp.supersamples * smp.supersamples;
+  P.depth = smp.depth;
+  P.depth_limit = smp.depth_limit;
+  P.nee = smp.next_event_estimation ? 1 : 0;
+  P.max_vertices = 128;
+  P.n_lights = (int32_t)s->h_lights.size();
+  P.n_analytic = (int32_t)analytic.size();
+  P.n_nodes = (int32_t)bo.nodes.size();
+  P.smp = 1.0 / smp.samples / (smp.supersamples * smp.supersamples);
+  P.sensor_response = smp.sensor_response;
+  P.cutoff = (float)smp.luminance_cutoff;  // a float in SceneEnv (scene_env.h:221)
+  std::memcpy(P.background, smp.background_color, sizeof(P.background));
+  std::memcpy(P.env_light, smp.environment_light, sizeof(P.env_light));
+  P.nodes = (const DevNode*)s->nodes.p;
+  P.tris = (const float*)s->tris.p;
+  P.tri_nrm = (const float*)s->tri_nrm.p;
+  P.tri_alb = any_albedo ? (const float*)s->tri_alb.p : nullptr;
+  P.ents = (const DevEntity*)s->ents.p;
+  P.mats = (const DevMaterial*)s->mats.p;
+  P.analytic = (const int32_t*)s->analytic.p;
+  P.lights = (const int32_t*)s->lights.p;
+  P.stack_ovf = (uint32_t*)s->ovf.p;
+  P.ovf_threads = (int32_t)ovf_threads;
+  P.n_tris = (int32_t)s->n_tris;
+  return PRT_OK;
+}
+
+prt_scene* prt_scene_create(const prt_scene_desc* desc) {
+  prt_scene* s = new (std::nothrow) prt_scene();
+  if (!s) { g_err = "out of host memory"; return nullptr; }
+  int rc;
+  try {
+    rc = scene_build(s, desc);
+  } catch (const std::bad_alloc&) {
+    rc = fail(PRT_E_NOMEM, "out of host memory while building the scene");
+  }
+  if (rc != PRT_OK) {
+    std::string keep = g_err;
+    prt_scene_destroy(s);
+    g_err = keep;
+    return nullptr;
+  }
+  return s;
+}
+
+int prt_scene_info(const prt_scene* s, int64_t* n_nodes, int64_t* n_tris, int64_t* n_lights, int64_t* bvh_bytes) {
+  if (!s) return fail(PRT_E_INVALID, "null scene");
+  if (n_nodes) *n_nodes = s->P.n_nodes;
+  if (n_tris) *n_tris = s->n_tris;
+  if (n_lights) *n_lights = (int64_t)s->h_lights.size();
+  if (bvh_bytes) *bvh_bytes = (int64_t)s->P.n_nodes * (int64_t)sizeof(DevNode) + s->n_tris * 48;
+  return PRT_OK;
+}
+
+static void fill_stats(prt_stats* st, const uint64_t* c) {
+  st->node_visits += c[CTR_NODES];
+  st->tri_tests += c[CTR_TRIS];
+  st->pickup_hits += c[CTR_PICKUP];
+  st->truncated_paths += c[CTR_TRUNC];
+}
+
+static int trace_device(prt_scene* s, const prt_ray* d_rays, int64_t n, prt_hit* d_hits, prt_stats* stats) {
+  HIPCHK(hipSetDevice(s->device));
+  int rc;
+  if ((rc = dev_alloc(s->ws.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
+  HIPCHK(hipMemsetAsync(s->ws.ctr.p, 0, CTR_COUNT * sizeof(uint64_t), s->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s->stream));
+  if (n > 0)
+    HIPCHK(launch_trace(s->P, (const prt_ray_dev*)d_rays, n, (prt_hit_dev*)d_hits, (uint64_t*)s->ws.ctr.p,
+                        s->max_trav_blocks, s->stream));
+  HIPCHK(hipEventRecord(e1, s->stream));
+  uint64_t c[CTR_COUNT];
+  HIPCHK(hipMemcpyAsync(c, s->ws.ctr.p, sizeof(c), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    fill_stats(stats, c);
+    stats->extension_rays = (uint64_t)n;
+    stats->ms_total = ms;
+    stats->ms_extend = ms;
+    stats->launches_extend = 1;
+    stats->bytes_algorithmic = 128 * stats->node_visits + 48 * stats->tri_tests + 48 * (uint64_t)n + 32 * (uint64_t)n;
+  }
+  return PRT_OK;
+}
+
+int prt_trace_closest_device(prt_scene* s, const prt_ray* d_rays, int64_t n, prt_hit* d_hits, prt_stats* stats) {
+  if (!s || n < 0 || (n > 0 && (!d_rays || !d_hits))) return fail(PRT_E_INVALID, "bad arguments");
+  return trace_device(s, d_rays, n, d_hits, stats);
+}
+
+int prt_trace_closest(prt_scene* s, const prt_ray* rays, int64_t n, prt_hit* hits, prt_stats* stats) {
+  if (!s || n < 0 || (n > 0 && (!rays || !hits))) return fail(PRT_E_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(s->device));
+  void *dr = nullptr, *dh = nullptr;
+  if (n > 0) {
+    HIPCHK(hipMalloc(&dr, (size_t)n * sizeof(prt_ray)));
+    hipError_t e = hipMalloc(&dh, (size_t)n * sizeof(prt_hit));
+    if (e != hipSuccess) { (void)hipFree(dr); return fail(PRT_E_NOMEM, hipGetErrorString(e)); }
+    e = hipMemcpyAsync(dr, rays, (size_t)n * sizeof(prt_ray), hipMemcpyHostToDevice, s->stream);
+    if (e != hipSuccess) { (void)hipFree(dr); (void)hipFree(dh); return fail(PRT_E_HIP, hipGetErrorString(e)); }
+  }
+  int rc = trace_device(s, (const prt_ray*)dr, n, (prt_hit*)dh, stats);
+  if (rc == PRT_OK && n > 0) {
+    hipError_t e = hipMemcpyAsync(hits, dh, (size_t)n * sizeof(prt_hit), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = fail(PRT_E_HIP, hipGetErrorString(e));
+  }
+  if (dr) (void)hipFree(dr);
+  if (dh) (void)hipFree(dh);
+  return rc;
+}
+
+// ---------------------------------------------------------------- render
+static int ensure_workspace(prt_scene* s, int64_t batch_samples, int64_t n_pixels) {
+  Workspace& w = s->ws;
+  const int64_t fan = s->has_glass ? 8 : 1;
+  const int64_t qcap = batch_samples * fan;
+  const int64_t scap = 3 * qcap;
+  if (w.batch_samples >= batch_samples && w.queue_cap >= qcap && w.h_counts) return PRT_OK;
+  w.release();
+  int rc;
+  const size_t dsz[14] = {8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4};
+  for (int qi = 0; qi < 2; ++qi)
+    for (int k = 0; k < 14; ++k)
+      if ((rc = dev_alloc(w.q[qi][k], (size_t)qcap * dsz[k]))) return rc;
+  if ((rc = dev_alloc(w.hit_t, (size_t)qcap * 8))) return rc;
+  if ((rc = dev_alloc(w.hit_id, (size_t)qcap * 4))) return rc;
+  const size_t ssz[11] = {8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 4};
+  for (int k = 0; k < 11; ++k)
+    if ((rc = dev_alloc(w.sq[k], (size_t)scap * ssz[k]))) return rc;
+  if ((rc = dev_alloc(w.accum, (size_t)batch_samples * 3 * sizeof(float)))) return rc;
+  if ((rc = dev_alloc(w.counts, 16 * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(w.fetch, 1024 * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(w.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
+  HIPCHK(hipHostMalloc((void**)&w.h_counts, 16 * sizeof(uint32_t), hipHostMallocDefault));
+  w.batch_samples = batch_samples;
+  w.queue_cap = qcap;
+  w.shadow_cap = scap;
+  (void)n_pixels;
+  return PRT_OK;
+}
+
+struct Timer {
+  bool on;
+  hipStream_t s;
+  std::vector<std::pair<hipEvent_t, int>> marks;  // event + category of the span that ends here
+  hipEvent_t start{};
+  int begin(bool enable, hipStream_t st) {
+    on = enable;
+    s = st;
+    if (!on) return PRT_OK;
+    HIPCHK(hipEventCreate(&start));
+    HIPCHK(hipEventRecord(start, s));
+    return PRT_OK;
+  }
+  int mark(int cat) {
+    if (!on) return PRT_OK;
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(e, s));
+    marks.push_back({e, cat});
+    return PRT_OK;
+  }
+  void collect(double* cat_ms) {
+    if (!on) return;
+    hipEvent_t prev = start;
+    for (auto& m : marks) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, prev, m.first) == hipSuccess) cat_ms[m.second] += ms;
+      prev = m.first;
+    }
+    for (auto& m : marks) (void)hipEventDestroy(m.first);
+    (void)hipEventDestroy(start);
+    marks.clear();
+  }
+};
+enum { T_EXTEND = 0, T_SHADE = 1, T_SHADOW = 2, T_OTHER = 3 };
+
+static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, float* d_var, prt_stats* stats) {
+  HIPCHK(hipSetDevice(s->device));
+  prt_render_opts opts{};
+  if (o) opts = *o;
+  Params P = s->P;
+  if (opts.samples_override > 0) {
+    P.samples = opts.samples_override;
+    P.spp = P.samples * P.SS * P.SS;
+    P.smp = 1.0 / P.samples / (P.SS * P.SS);
+  }
+  if (P.spp > (1 << 20)) return fail(PRT_E_INVALID, "samples per pixel above 2^20");
+  P.max_vertices = opts.max_depth > 0 ? std::min(opts.max_depth, 255) : 128;
+  P.key0 = (uint32_t)opts.seed;
+  P.key1 = (uint32_t)(opts.seed >> 32);
+  const int64_t n_pix = (int64_t)P.W * P.H;
+  int64_t bp = opts.batch_pixels > 0 ? opts.batch_pixels : std::max<int64_t>(1, ((int64_t)1 << 21) / P.spp);
+  bp = std::min(bp, n_pix);
+  // queues hold up to 8x (glass splits) and 24x (shadow rays) the batch
+  while (bp > 1 && bp * P.spp * 24 >= ((int64_t)1 << 32)) bp /= 2;
+  int rc;
+  if ((rc = ensure_workspace(s, bp * P.spp, bp))) return rc;
+  Workspace& w = s->ws;
+  hipStream_t st = s->stream;
+  uint32_t* counts = (uint32_t*)w.counts.p;
+  uint32_t* fetch = (uint32_t*)w.fetch.p;
+  uint64_t* ctr = (uint64_t*)w.ctr.p;
+  RayQ q[2] = {ray_queue(w.q[0]), ray_queue(w.q[1])};
+  ShadowQ sq = shadow_queue(w.sq);
+  q[0].cap = q[1].cap = (uint32_t)w.queue_cap;
+  sq.cap = (uint32_t)w.shadow_cap;
+  HIPCHK(launch_fill_u32((uint32_t*)ctr, 0u, 2 * CTR_COUNT, st));
+  Timer tm;
+  if ((rc = tm.begin(opts.timing != 0, st))) return rc;
+  uint64_t primary_rays = 0, ext_launch_rays = 0;
+  int64_t launches = 0;
+  const int tb = s->max_trav_blocks;
+  for (int64_t pb = 0; pb < n_pix; pb += bp) {
+    Batch B{pb, std::min(bp, n_pix - pb)};
+    const int64_t ns = B.n_pixels * P.spp;
+    const int64_t np = B.n_pixels * P.SS * P.SS;
+    HIPCHK(launch_fill_f32((float*)w.accum.p, 0.0f, 3 * ns, st));
+    HIPCHK(launch_fill_u32(fetch, 0u, 1024, st));
+    HIPCHK(launch_fill_u32(counts, 0u, 16, st));
+    // counts[3] = number of primary rays (consumed by the extend kernel)
+    HIPCHK(launch_fill_u32(counts + 3, (uint32_t)np, 1, st));
+    HIPCHK(launch_gen_primary(P, B, q[0], st));
+    if ((rc = tm.mark(T_OTHER))) return rc;
+    int fi = 0;
+    HIPCHK(launch_extend(P, q[0], counts + 3, np, fetch + fi++, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb, st));
+    if ((rc = tm.mark(T_EXTEND))) return rc;
+    primary_rays += (uint64_t)np;
+    launches++;
+    // depth 0: every camera sample against its sub-pixel's primary hit
+    HIPCHK(launch_shade(P, B, 1, q[0], counts + 3, ns, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1],
+                        counts + 1, sq, counts + 2, (float*)w.accum.p, ctr, st));
+    if ((rc = tm.mark(T_SHADE))) return rc;
+    HIPCHK(launch_shadow(P, sq, counts + 2, w.shadow_cap, fetch + fi++, (float*)w.accum.p, ctr, tb, st));
+    if ((rc = tm.mark(T_SHADOW))) return rc;
+    int cur = 1;
+    while (true) {
+      HIPCHK(hipMemcpyAsync(w.h_counts, counts, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      const uint32_t n = w.h_counts[cur];
+      if (n == 0) break;
+      if ((int64_t)n > w.queue_cap || (int64_t)w.h_counts[2] > w.shadow_cap)
+        return fail(PRT_E_INVALID, "ray queue overflow");
+      if (fi + 2 >= 1024) return fail(PRT_E_INVALID, "too many wavefront launches in one batch");
+      const int nxt = cur ^ 1;
+      // reset the output queue and the shadow queue for this depth
+      HIPCHK(launch_fill_u32(counts + nxt, 0u, 1, st));
+      HIPCHK(launch_fill_u32(counts + 2, 0u, 1, st));
+      if ((rc = tm.mark(T_OTHER))) return rc;
+      HIPCHK(launch_extend(P, q[cur], counts + cur, n, fetch + fi++, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr,
+                           tb, st));
+      if ((rc = tm.mark(T_EXTEND))) return rc;
+      ext_launch_rays += n;
+      launches++;
+      HIPCHK(launch_shade(P, B, 0, q[cur], counts + cur, n, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p,
+                          q[nxt], counts + nxt, sq, counts + 2, (float*)w.accum.p, ctr, st));
+      if ((rc = tm.mark(T_SHADE))) return rc;
+      HIPCHK(launch_shadow(P, sq, counts + 2, w.shadow_cap, fetch + fi++, (float*)w.accum.p, ctr, tb, st));
+      if ((rc = tm.mark(T_SHADOW))) return rc;
+      cur = nxt;
+    }
+    HIPCHK(launch_resolve(P, B, (const float*)w.accum.p, d_rgb, d_var, st));
+    if ((rc = tm.mark(T_OTHER))) return rc;
+  }
+  uint64_t c[CTR_COUNT];
+  HIPCHK(hipMemcpyAsync(c, ctr, sizeof(c), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (c[CTR_DROP] != 0) return fail(PRT_E_INVALID, "ray queue overflow (" + std::to_string(c[CTR_DROP]) + " rays)");
+  double cat[4] = {0, 0, 0, 0};
+  tm.collect(cat);
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    fill_stats(stats, c);
+    stats->camera_samples = (uint64_t)n_pix * P.spp;
+    stats->primary_rays = primary_rays;
+    stats->extension_rays = ext_launch_rays;
+    stats->shadow_rays = c[CTR_SHADOW];
+    stats->ms_extend = cat[T_EXTEND];
+    stats->ms_shade = cat[T_SHADE];
+    stats->ms_shadow = cat[T_SHADOW];
+    stats->ms_other = cat[T_OTHER];
+    stats->ms_total = cat[0] + cat[1] + cat[2] + cat[3];
+    stats->launches_extend = launches;
+    const uint64_t rays = primary_rays + ext_launch_rays + c[CTR_SHADOW];
+    stats->bytes_algorithmic = 128 * stats->node_visits + 48 * stats->tri_tests + 48 * rays;
+  }
+  return PRT_OK;
+}
+
+int prt_render_device(prt_scene* s, const prt_render_opts* opts, float* d_rgb, float* d_var, prt_stats* stats) {
+  if (!s || !d_rgb) return fail(PRT_E_INVALID, "bad arguments");
+  return render_device(s, opts, d_rgb, d_var, stats);
+}
+
+int prt_render(prt_scene* s, const prt_render_opts* opts, float* out_rgb, float* out_var, prt_stats* stats) {
+  if (!s || !out_rgb) return fail(PRT_E_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(s->device));
+  const size_t bytes = (size_t)s->P.W * s->P.H * 3 * sizeof(float);
+  int rc;
+  if ((rc = dev_alloc(s->out_rgb, bytes))) return rc;
+  if (out_var && (rc = dev_alloc(s->out_var, bytes))) return rc;
+  rc = render_device(s, opts, (float*)s->out_rgb.p, out_var ? (float*)s->out_var.p : nullptr, stats);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(out_rgb, s->out_rgb.p, bytes, hipMemcpyDeviceToHost, s->stream));
+  if (out_var) HIPCHK(hipMemcpyAsync(out_var, s->out_var.p, bytes, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return PRT_OK;
+}
+
+int prt_render_batch_device(prt_scene* const* scenes, int n_scenes, const prt_batch_opts* opts, float* const* d_outs,
+                            prt_stats* stats) {
+  (void)scenes; (void)n_scenes; (void)opts; (void)d_outs; (void)stats;
+  return fail(PRT_E_UNSUPPORTED, "batched rendering is not built yet");
+}
+
+int prt_set_face_albedo_device(prt_scene* s, int entity, const float* d_albedo) {
+  (void)s; (void)entity; (void)d_albedo;
+  return fail(PRT_E_UNSUPPORTED, "not built yet");
+}
+int prt_set_emission(prt_scene* s, int entity, const double emission[3]) {
+  (void)s; (void)entity; (void)emission;
+  return fail(PRT_E_UNSUPPORTED, "not built yet");
+}
+int prt_adjoint_device(prt_scene* s, const prt_render_opts* opts, const float* d_dimage, int entity,
+                       float* d_albedo_grad, int light, double* d_emission_grad, prt_stats* stats) {
+  (void)s; (void)opts; (void)d_dimage; (void)entity; (void)d_albedo_grad; (void)light; (void)d_emission_grad;
+  (void)stats;
+  return fail(PRT_E_UNSUPPORTED, "not built yet");
+}
+
+}  // extern "C"

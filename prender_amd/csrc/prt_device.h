@@ -1,0 +1,81 @@
+// prt_device.h — device-side data layout shared by the host builder and the
+// HIP kernels of libprt.
+//
+// Layout in HBM (all arrays 256-byte aligned):
+//   nodes   : DevNode[n_nodes], 128 B each — a 4-wide BVH node with the child
+//             boxes as six float4 (lo x/y/z, hi x/y/z over the 4 children) so
+//             one node is eight 16-byte loads;
+//   tris    : float4[3*n_tris], 48 B per triangle in leaf order:
+//             (v0.xyz, entity), (v1.xyz, face), (v2.xyz, unused) — the float32
+//             vertices of the reference's Obj storage (ObjLoader.h:39-75);
+//   tri_nrm : float[3*n_tris] f32 face normal (ObjLoader.cpp:577-700);
+//   tri_alb : float[3*n_tris] per-face albedo or null;
+//   entities, materials, lights: small tables read through the scalar cache.
+#pragma once
+#include <stdint.h>
+
+#define PRT_MAX_LEAF 8
+#define PRT_STACK_LDS 16
+#define PRT_STACK_TOTAL 128
+
+struct __attribute__((aligned(16))) DevNode {
+  float lo_x[4], lo_y[4], lo_z[4];
+  float hi_x[4], hi_y[4], hi_z[4];
+  // >= 0: internal node index; < 0: leaf, ~child = first | (count-1) << 27
+  int32_t child[4];
+  int32_t pad[4];
+};
+static_assert(sizeof(DevNode) == 128, "node record must be 128 bytes");
+
+static inline int32_t prt_leaf_code(uint32_t first, uint32_t count) {
+  return (int32_t)~(first | ((count - 1u) << 27));
+}
+
+struct DevEntity {
+  int32_t type;      // PRT_ENTITY_*
+  int32_t material;
+  int32_t ninv;      // normal_vector_inverse
+  int32_t light;     // material isLight()
+  int32_t has_albedo;// POLYGON with per-face albedo in tri_alb
+  int32_t pad_;
+  double p0[3];      // sphere position / uvplane org
+  double radius;
+  double u_axis[3], v_axis[3], normal[3];  // uvplane (normalised, normal = u x v)
+  double u_len, v_len, area;
+};
+
+struct DevMaterial {
+  double color[3];
+  double emission[3];
+  double roughness;
+  double ior;
+  int32_t type;
+  int32_t is_light;
+};
+
+// Everything a kernel needs about the scene and the render settings.
+struct Params {
+  // camera (CameraScreenPram, render.h:100-146)
+  double cam_pos[3], screen_x[3], screen_y[3], center[3];
+  double iw, ih;
+  int32_t W, H, SS, samples;
+  int32_t spp, depth, depth_limit, nee;
+  int32_t max_vertices, n_lights, n_analytic, n_nodes;
+  uint32_t key0, key1;
+  double smp, sensor_response;
+  float cutoff;
+  int32_t pad_;
+  double background[3], env_light[3];
+  // scene arrays
+  const DevNode* nodes;
+  const float* tris;         // read as float4 on the device
+  const float* tri_nrm;
+  const float* tri_alb;
+  const DevEntity* ents;
+  const DevMaterial* mats;
+  const int32_t* analytic;   // entity indices of spheres / uvplanes
+  const int32_t* lights;     // light entity indices, power-descending
+  uint32_t* stack_ovf;       // per-thread traversal stack overflow
+  int32_t ovf_threads;
+  int32_t n_tris;
+};

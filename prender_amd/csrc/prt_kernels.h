@@ -1,0 +1,69 @@
+// prt_kernels.h — queue layouts and kernel launchers of libprt.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "prt_device.h"
+
+// counters (uint64) accumulated by the kernels
+enum {
+  CTR_NODES = 0,   // BVH nodes fetched
+  CTR_TRIS = 1,    // triangle tests
+  CTR_EXT = 2,     // extension rays queued
+  CTR_SHADOW = 3,  // shadow rays queued
+  CTR_PICKUP = 4,  // specular pickups taken from a child hit
+  CTR_TRUNC = 5,   // children dropped by the vertex cap
+  CTR_DROP = 6,    // appends beyond a queue's capacity (must stay 0)
+  CTR_COUNT = 8
+};
+
+struct prt_ray_dev {
+  double org[3];
+  double dir[3];
+};
+struct prt_hit_dev {
+  double t, u, v;
+  int32_t entity, prim;
+};
+
+// Structure-of-arrays ray queue (one entry = one path segment to trace).
+struct RayQ {
+  double *ox, *oy, *oz, *dx, *dy, *dz;
+  float *tr, *tg, *tb;   // path throughput
+  float *pr, *pg, *pb;   // pickup weight (mirror / glass parent with NEE)
+  uint32_t* slot;        // batch-local camera sample
+  uint32_t* meta;        // depth | branch << 8
+  uint32_t cap;          // capacity (entries)
+};
+
+// NEE shadow rays: nudged origin, direction, the light's own hit distance,
+// and the contribution added when nothing nearer blocks it.
+struct ShadowQ {
+  double *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
+  float *cr, *cg, *cb;
+  uint32_t* slot;
+  uint32_t cap;
+};
+
+// A batch: pixels [pixel_base, pixel_base + n_pixels) of the y*W + x order,
+// all spp samples of each.
+struct Batch {
+  int64_t pixel_base;
+  int64_t n_pixels;
+};
+
+hipError_t launch_trace(const Params& P, const prt_ray_dev* rays, int64_t n, prt_hit_dev* hits, uint64_t* ctr,
+                        int max_blocks, hipStream_t s);
+hipError_t launch_gen_primary(const Params& P, const Batch& B, const RayQ& q, hipStream_t s);
+hipError_t launch_extend(const Params& P, const RayQ& q, const uint32_t* count, int64_t n_host, uint32_t* fetch,
+                         double* hit_t, int32_t* hit_id, uint64_t* ctr, int max_blocks, hipStream_t s);
+hipError_t launch_shadow(const Params& P, const ShadowQ& q, const uint32_t* count, int64_t n_max, uint32_t* fetch,
+                         float* accum, uint64_t* ctr, int max_blocks, hipStream_t s);
+hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const uint32_t* count_in,
+                        int64_t n_host, const double* hit_t, const int32_t* hit_id, const RayQ& qout,
+                        uint32_t* count_out, const ShadowQ& sq, uint32_t* count_sq, float* accum, uint64_t* ctr,
+                        hipStream_t s);
+hipError_t launch_resolve(const Params& P, const Batch& B, const float* accum, float* rgb, float* var,
+                          hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);

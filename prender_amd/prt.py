@@ -1,0 +1,160 @@
+"""ctypes shim over libprt.so (include/prt.h).
+
+This is the Python face of the render entry point: ``Scene(desc)`` is what
+``SceneEnv::Load`` + ``CreateEntList`` set up (scene_env.h:357,
+scene.cpp:38), ``Scene.render`` is ``Render::Run`` (render.h:1125-1455) and
+``Scene.trace`` is ``intersect_scene`` over a ray batch (scene.cpp:448).
+Everything runs in the HIP library; there is no CPU fallback — a missing or
+unloadable library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .scene import (CDesc, HIT_DTYPE, PrtBatchOpts, PrtRenderOpts, PrtSceneDesc, PrtStats, RAY_DTYPE,
+                    SceneDesc)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libprt.so")
+
+_lib = None
+
+
+class PrtError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libprt.so (built by ``python -m prender_amd.build``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PrtError(f"{LIB_PATH} is missing: build it with `python -m prender_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    L.prt_abi_version.restype = C.c_int
+    L.prt_last_error.restype = C.c_char_p
+    L.prt_device_info.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.prt_scene_create.restype = C.c_void_p
+    L.prt_scene_create.argtypes = [C.POINTER(PrtSceneDesc)]
+    L.prt_scene_destroy.argtypes = [C.c_void_p]
+    L.prt_scene_info.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 4
+    L.prt_trace_closest.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(PrtStats)]
+    L.prt_trace_closest_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(PrtStats)]
+    L.prt_render.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_void_p, C.POINTER(PrtStats)]
+    L.prt_render_device.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_void_p,
+                                    C.POINTER(PrtStats)]
+    L.prt_render_batch_device.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(PrtBatchOpts),
+                                          C.POINTER(C.c_void_p), C.POINTER(PrtStats)]
+    L.prt_set_face_albedo_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.prt_set_emission.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+    L.prt_adjoint_device.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_int, C.c_void_p,
+                                     C.c_int, C.POINTER(C.c_double), C.POINTER(PrtStats)]
+    _lib = L
+    return L
+
+
+EXPORTED = ["prt_abi_version", "prt_last_error", "prt_device_info", "prt_scene_create", "prt_scene_destroy",
+            "prt_scene_info", "prt_trace_closest", "prt_trace_closest_device", "prt_render", "prt_render_device",
+            "prt_render_batch_device", "prt_set_face_albedo_device", "prt_set_emission", "prt_adjoint_device"]
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        raise PrtError(f"{what} failed ({rc}): {lib().prt_last_error().decode()}")
+
+
+def device_info():
+    name = C.create_string_buffer(64)
+    cu, ma, mi = C.c_int(), C.c_int(), C.c_int()
+    check(lib().prt_device_info(name, 64, C.byref(cu), C.byref(ma), C.byref(mi)), "prt_device_info")
+    return {"arch": name.value.decode(), "cu_count": cu.value, "major": ma.value, "minor": mi.value}
+
+
+def render_opts(seed: int = 0, samples: Optional[int] = None, batch_pixels: int = 0, max_depth: int = 0,
+                use_graph: bool = False, timing: bool = False) -> PrtRenderOpts:
+    o = PrtRenderOpts()
+    o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    o.samples_override = int(samples or 0)
+    o.batch_pixels = int(batch_pixels)
+    o.max_depth = int(max_depth)
+    o.use_graph = 1 if use_graph else 0
+    o.timing = 1 if timing else 0
+    return o
+
+
+class Scene:
+    """A scene uploaded to the current HIP device."""
+
+    def __init__(self, desc: SceneDesc):
+        self.desc = desc
+        self._c = CDesc(desc)
+        h = lib().prt_scene_create(self._c.ptr())
+        if not h:
+            raise PrtError("prt_scene_create failed: " + lib().prt_last_error().decode())
+        self.h = C.c_void_p(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().prt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def width(self):
+        return self.desc.camera.width
+
+    @property
+    def height(self):
+        return self.desc.camera.height
+
+    def info(self):
+        v = [C.c_int64() for _ in range(4)]
+        check(lib().prt_scene_info(self.h, *[C.byref(x) for x in v]), "prt_scene_info")
+        return {"nodes": v[0].value, "tris": v[1].value, "lights": v[2].value, "bvh_bytes": v[3].value}
+
+    # ---- closest hit
+    def trace(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(rays.shape[0], dtype=HIT_DTYPE)
+        st = PrtStats()
+        check(lib().prt_trace_closest(self.h, rays.ctypes.data, rays.shape[0], hits.ctypes.data, C.byref(st)),
+              "prt_trace_closest")
+        return hits, st.as_dict()
+
+    def trace_device(self, rays_ptr: int, n: int, hits_ptr: int):
+        st = PrtStats()
+        check(lib().prt_trace_closest_device(self.h, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr), C.byref(st)),
+              "prt_trace_closest_device")
+        return st.as_dict()
+
+    # ---- render
+    def render(self, seed: int = 0, samples: Optional[int] = None, variance: bool = True, **kw):
+        """Returns (rgb[H,W,3], var[H,W,3] or None, stats); rows in the
+        reference's image order (row 0 = top, render.h:1234)."""
+        H, W = self.height, self.width
+        rgb = np.zeros((H, W, 3), dtype=np.float32)
+        var = np.zeros((H, W, 3), dtype=np.float32) if variance else None
+        st = PrtStats()
+        o = render_opts(seed, samples, **kw)
+        check(lib().prt_render(self.h, C.byref(o), rgb.ctypes.data, var.ctypes.data if variance else None,
+                               C.byref(st)), "prt_render")
+        return rgb, var, st.as_dict()
+
+    def render_device(self, rgb_ptr: int, var_ptr: Optional[int] = None, seed: int = 0,
+                      samples: Optional[int] = None, **kw):
+        st = PrtStats()
+        o = render_opts(seed, samples, **kw)
+        check(lib().prt_render_device(self.h, C.byref(o), C.c_void_p(rgb_ptr),
+                                      C.c_void_p(var_ptr) if var_ptr else None, C.byref(st)),
+              "prt_render_device")
+        return st.as_dict()
