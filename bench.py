@@ -1,0 +1,166 @@
+"""Benchmark of the path-tracing hot path on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-rows R] [--no-cpu]
+
+A step is one full render (Render::Run, render.h:1125-1455) of the workload:
+the quad Cornell box with the level-6 displaced icosphere (81,944 triangles),
+a 0.25x0.25 area light, 512x512, 64 spp (SAMPLING 16 x SUPERSAMPLING 2),
+DEPTH 5 / DEPTH_LIMIT 8, next-event estimation.  Inputs (scene, BVH) are
+resident in HBM before the timed region; the image stays on the device.
+
+Prints one JSON line: samples/s over the timed steps, Mrays/s, the roofline
+of the dominant traversal kernel (algorithmic bytes per launch / its average
+duration from HIP events on the library's stream) and the CPU baseline (the
+FP64 oracle on a band of rows of the same workload, all host threads).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mrays/s + samples/s at matched RMSE vs host CPU; adjoint grad evals/s, 1 GPU"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-rows", type=int, default=48, help="rows of the workload rendered by the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rmse", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device copy first (reference dispatch for PMC byte counters)")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one MI355X (asked for --gpus {args.gpus}); "
+              "the multi-GPU split of camera samples is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from prender_amd import generators as G
+    from prender_amd import prt
+
+    dev = torch.device("cuda:0")
+    info = prt.device_info()
+    if args.calibrate_copy:
+        # 1 GiB read + 1 GiB write of known size: the FETCH_SIZE / WRITE_SIZE
+        # counters of this dispatch scale those of the traversal kernels
+        a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    desc = G.config2()
+    W, H, spp = desc.camera.width, desc.camera.height, desc.spp
+    t0 = time.time()
+    scene = prt.Scene(desc)
+    build_s = time.time() - t0
+    sinfo = scene.info()
+    rgb = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    var = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    for i in range(args.warmup):
+        scene.render_device(rgb.data_ptr(), var.data_ptr(), seed=1000 + i, timing=True)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        stats.append(scene.render_device(rgb.data_ptr(), var.data_ptr(), seed=args.seed + i, timing=True))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    samples = W * H * spp * args.steps
+    sum_ = lambda k: sum(s[k] for s in stats)
+    rays = sum_("primary_rays") + sum_("extension_rays") + sum_("shadow_rays")
+    ms_ext, ms_sh, ms_shade = sum_("ms_extend"), sum_("ms_shadow"), sum_("ms_shade")
+    launches = sum_("launches_extend")
+    kernels = {
+        "k_extend": (sum_("bytes_extend"), ms_ext, launches),
+        "k_shadow": (sum_("bytes_shadow"), ms_sh, launches),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    b, ms, nl = kernels[dom]
+    achieved = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": samples / elapsed,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "cfg2: quad Cornell box (24 tris) + L6 displaced icosphere (81,920 tris), "
+                               "0.25^2 area light, 512x512, 64 spp (16x2x2), DEPTH 5 / DEPTH_LIMIT 8, NEE",
+                   "width": W, "height": H, "spp": spp, "triangles": sinfo["tris"], "bvh_nodes": sinfo["nodes"],
+                   "parallelism": "1 GPU"},
+        "mrays_per_s": rays / elapsed / 1e6,
+        "rays": {"primary": sum_("primary_rays"), "extension": sum_("extension_rays"),
+                 "shadow": sum_("shadow_rays"), "pickup_hits_no_ray": sum_("pickup_hits")},
+        "kernel_ms_per_step": {"extend": ms_ext / args.steps, "shade": ms_shade / args.steps,
+                               "shadow": ms_sh / args.steps, "other": sum_("ms_other") / args.steps},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": b / max(1, nl), "avg_launch_ms": ms / max(1, nl),
+                     "note": "algorithmic bytes = 128 B per BVH node + 48 B per triangle + 48 B per ray; "
+                             "node and triangle reuse in L2 / MALL lets this exceed the HBM figure"},
+        "device": info["arch"],
+        "scene_build_s": build_s,
+    }
+    if not args.no_rmse:
+        # RMSE of the measured image against a 16x-spp render of the same scene
+        ref = torch.empty_like(rgb)
+        scene.render_device(ref.data_ptr(), None, seed=999_999, samples=desc.sampling.samples * 16)
+        torch.cuda.synchronize()
+        out["rmse_vs_16x_spp"] = float(torch.sqrt(torch.mean((rgb - ref) ** 2)).item())
+        ref_np = ref.cpu().numpy()
+    else:
+        ref_np = None
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(desc, args, elapsed / args.steps, ref_np)
+    print(json.dumps(out))
+
+
+def cpu_baseline(desc, args, gpu_step_s, ref_np):
+    """The FP64 oracle (a restatement of the reference's Render::Run) on a
+    band of rows of the same workload, timed on this host's cores."""
+    from oracle.oracle import OracleScene
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    o = OracleScene(desc)
+    H, W = desc.camera.height, desc.camera.width
+    r0 = H // 2 - args.cpu_rows // 2
+    r1 = r0 + args.cpu_rows
+    t0 = time.perf_counter()
+    rgb, var, st = o.render(seed=args.seed, rows=(r0, r1), threads=threads)
+    dt = time.perf_counter() - t0
+    n = W * (r1 - r0) * desc.spp
+    res = {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+           "sample": f"rows {r0}..{r1 - 1} of the 512x512x64spp workload ({n} camera samples, {dt:.1f} s)",
+           "mrays_per_s": (st["primary"] + st["extension"] + st["shadow"] + st["pickup"]) / dt / 1e6}
+    if ref_np is not None:
+        rows = slice(H - r1, H - r0)  # image rows are stored top-down (render.h:1234)
+        res["rmse_vs_16x_spp_same_rows"] = float(np.sqrt(np.mean((rgb[rows] - ref_np[rows]) ** 2)))
+    res["gpu_over_cpu"] = (desc.spp * W * H / gpu_step_s) / res["value"]
+    return res
+
+
+if __name__ == "__main__":
+    main()

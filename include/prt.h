@@ -164,6 +164,10 @@ typedef struct prt_stats {
     double ms_shadow;          /* shadow kernels */
     double ms_other;           /* generate / resolve / clears */
     int64_t launches_extend;
+    uint64_t shadow_node_visits;  /* part of node_visits spent on shadow rays */
+    uint64_t shadow_tri_tests;    /* part of tri_tests spent on shadow rays */
+    uint64_t bytes_extend;        /* algorithmic bytes of the closest-hit kernels */
+    uint64_t bytes_shadow;        /* algorithmic bytes of the shadow kernels */
 } prt_stats;
 
 typedef struct prt_render_opts {

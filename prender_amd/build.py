@@ -18,6 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libprt.so")
 SOURCES = ["prt_kernels.hip", "prt_api.cpp", "bvh.cpp"]
+HEADERS = ["prt_device.h", "prt_kernels.h", "prt_math.h", "prt_trace.h", "bvh.h"]
 ARCH = "gfx950"
 
 

@@ -89,7 +89,7 @@ struct Workspace {
   int64_t batch_samples = 0;  // camera samples per batch
   int64_t queue_cap = 0;      // rays per queue
   int64_t shadow_cap = 0;
-  DevBuf q[2][14];            // two ray queues (SoA)
+  DevBuf q[2][17];            // two ray queues (SoA)
   DevBuf hit_t, hit_id;
   DevBuf sq[11];
   DevBuf accum;
@@ -121,6 +121,7 @@ RayQ ray_queue(DevBuf* b) {
   q.pr = (float*)b[9].p; q.pg = (float*)b[10].p; q.pb = (float*)b[11].p;
   q.slot = (uint32_t*)b[12].p;
   q.meta = (uint32_t*)b[13].p;
+  q.nx = (float*)b[14].p; q.ny = (float*)b[15].p; q.nz = (float*)b[16].p;
   return q;
 }
 ShadowQ shadow_queue(DevBuf* b) {
@@ -448,8 +449,10 @@ int prt_scene_info(const prt_scene* s, int64_t* n_nodes, int64_t* n_tris, int64_
 }
 
 static void fill_stats(prt_stats* st, const uint64_t* c) {
-  st->node_visits += c[CTR_NODES];
-  st->tri_tests += c[CTR_TRIS];
+  st->node_visits += c[CTR_NODES] + c[CTR_SNODES];
+  st->tri_tests += c[CTR_TRIS] + c[CTR_STRIS];
+  st->shadow_node_visits += c[CTR_SNODES];
+  st->shadow_tri_tests += c[CTR_STRIS];
   st->pickup_hits += c[CTR_PICKUP];
   st->truncated_paths += c[CTR_TRUNC];
 }
@@ -481,7 +484,8 @@ static int trace_device(prt_scene* s, const prt_ray* d_rays, int64_t n, prt_hit*
     stats->ms_total = ms;
     stats->ms_extend = ms;
     stats->launches_extend = 1;
-    stats->bytes_algorithmic = 128 * stats->node_visits + 48 * stats->tri_tests + 48 * (uint64_t)n + 32 * (uint64_t)n;
+    stats->bytes_extend = 128 * stats->node_visits + 48 * stats->tri_tests + 48 * (uint64_t)n;
+    stats->bytes_algorithmic = stats->bytes_extend;
   }
   return PRT_OK;
 }
@@ -522,9 +526,9 @@ static int ensure_workspace(prt_scene* s, int64_t batch_samples, int64_t n_pixel
   if (w.batch_samples >= batch_samples && w.queue_cap >= qcap && w.h_counts) return PRT_OK;
   w.release();
   int rc;
-  const size_t dsz[14] = {8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4};
+  const size_t dsz[17] = {8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
   for (int qi = 0; qi < 2; ++qi)
-    for (int k = 0; k < 14; ++k)
+    for (int k = 0; k < 17; ++k)
       if ((rc = dev_alloc(w.q[qi][k], (size_t)qcap * dsz[k]))) return rc;
   if ((rc = dev_alloc(w.hit_t, (size_t)qcap * 8))) return rc;
   if ((rc = dev_alloc(w.hit_id, (size_t)qcap * 4))) return rc;
@@ -609,7 +613,7 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   ShadowQ sq = shadow_queue(w.sq);
   q[0].cap = q[1].cap = (uint32_t)w.queue_cap;
   sq.cap = (uint32_t)w.shadow_cap;
-  HIPCHK(launch_fill_u32((uint32_t*)ctr, 0u, 2 * CTR_COUNT, st));
+  HIPCHK(launch_fill_u32((uint32_t*)ctr, 0u, 2 * CTR_COUNT, st));  // uint64 counters as u32 pairs
   Timer tm;
   if ((rc = tm.begin(opts.timing != 0, st))) return rc;
   uint64_t primary_rays = 0, ext_launch_rays = 0;
@@ -633,7 +637,9 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
     launches++;
     // depth 0: every camera sample against its sub-pixel's primary hit
     HIPCHK(launch_shade(P, B, 1, q[0], counts + 3, ns, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1],
-                        counts + 1, sq, counts + 2, (float*)w.accum.p, ctr, st));
+                        counts + 1, (float*)w.accum.p, ctr, st));
+    if (P.nee && P.n_lights > 0)
+      HIPCHK(launch_nee(P, B, q[1], counts + 1, w.queue_cap, sq, counts + 2, ctr, st));
     if ((rc = tm.mark(T_SHADE))) return rc;
     HIPCHK(launch_shadow(P, sq, counts + 2, w.shadow_cap, fetch + fi++, (float*)w.accum.p, ctr, tb, st));
     if ((rc = tm.mark(T_SHADOW))) return rc;
@@ -657,7 +663,9 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
       ext_launch_rays += n;
       launches++;
       HIPCHK(launch_shade(P, B, 0, q[cur], counts + cur, n, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p,
-                          q[nxt], counts + nxt, sq, counts + 2, (float*)w.accum.p, ctr, st));
+                          q[nxt], counts + nxt, (float*)w.accum.p, ctr, st));
+      if (P.nee && P.n_lights > 0)
+        HIPCHK(launch_nee(P, B, q[nxt], counts + nxt, w.queue_cap, sq, counts + 2, ctr, st));
       if ((rc = tm.mark(T_SHADE))) return rc;
       HIPCHK(launch_shadow(P, sq, counts + 2, w.shadow_cap, fetch + fi++, (float*)w.accum.p, ctr, tb, st));
       if ((rc = tm.mark(T_SHADOW))) return rc;
@@ -685,8 +693,11 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
     stats->ms_other = cat[T_OTHER];
     stats->ms_total = cat[0] + cat[1] + cat[2] + cat[3];
     stats->launches_extend = launches;
-    const uint64_t rays = primary_rays + ext_launch_rays + c[CTR_SHADOW];
-    stats->bytes_algorithmic = 128 * stats->node_visits + 48 * stats->tri_tests + 48 * rays;
+    // per ray query: 128 B per node record, 48 B per triangle record, 48 B
+    // of ray in + hit out
+    stats->bytes_extend = 128 * c[CTR_NODES] + 48 * c[CTR_TRIS] + 48 * (primary_rays + ext_launch_rays);
+    stats->bytes_shadow = 128 * c[CTR_SNODES] + 48 * c[CTR_STRIS] + 48 * c[CTR_SHADOW];
+    stats->bytes_algorithmic = stats->bytes_extend + stats->bytes_shadow;
   }
   return PRT_OK;
 }

@@ -14,7 +14,9 @@ enum {
   CTR_PICKUP = 4,  // specular pickups taken from a child hit
   CTR_TRUNC = 5,   // children dropped by the vertex cap
   CTR_DROP = 6,    // appends beyond a queue's capacity (must stay 0)
-  CTR_COUNT = 8
+  CTR_SNODES = 8,  // BVH nodes fetched by shadow rays
+  CTR_STRIS = 9,   // triangle tests of shadow rays
+  CTR_COUNT = 12
 };
 
 struct prt_ray_dev {
@@ -32,9 +34,15 @@ struct RayQ {
   float *tr, *tg, *tb;   // path throughput
   float *pr, *pg, *pb;   // pickup weight (mirror / glass parent with NEE)
   uint32_t* slot;        // batch-local camera sample
-  uint32_t* meta;        // depth | branch << 8
+  uint32_t* meta;        // depth | branch << 8 | META_NEE
+  float *nx, *ny, *nz;   // orienting normal of a diffuse parent (NEE input)
   uint32_t cap;          // capacity (entries)
 };
+
+// meta bit: the ray leaves a diffuse vertex whose next-event estimation is
+// still to be done (by k_nee, from the ray's origin, throughput and nx/ny/nz)
+#define META_NEE 0x80000000u
+#define META_BRANCH(m) (((m) >> 8) & 0xFFFFu)
 
 // NEE shadow rays: nudged origin, direction, the light's own hit distance,
 // and the contribution added when nothing nearer blocks it.
@@ -57,12 +65,13 @@ hipError_t launch_trace(const Params& P, const prt_ray_dev* rays, int64_t n, prt
 hipError_t launch_gen_primary(const Params& P, const Batch& B, const RayQ& q, hipStream_t s);
 hipError_t launch_extend(const Params& P, const RayQ& q, const uint32_t* count, int64_t n_host, uint32_t* fetch,
                          double* hit_t, int32_t* hit_id, uint64_t* ctr, int max_blocks, hipStream_t s);
+hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const uint32_t* count, int64_t n_max,
+                      const ShadowQ& sq, uint32_t* count_sq, uint64_t* ctr, hipStream_t s);
 hipError_t launch_shadow(const Params& P, const ShadowQ& q, const uint32_t* count, int64_t n_max, uint32_t* fetch,
                          float* accum, uint64_t* ctr, int max_blocks, hipStream_t s);
 hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const uint32_t* count_in,
                         int64_t n_host, const double* hit_t, const int32_t* hit_id, const RayQ& qout,
-                        uint32_t* count_out, const ShadowQ& sq, uint32_t* count_sq, float* accum, uint64_t* ctr,
-                        hipStream_t s);
+                        uint32_t* count_out, float* accum, uint64_t* ctr, hipStream_t s);
 hipError_t launch_resolve(const Params& P, const Batch& B, const float* accum, float* rgb, float* var,
                           hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);

@@ -83,7 +83,9 @@ class PrtStats(C.Structure):
                 ("bytes_algorithmic", C.c_uint64), ("ms_total", C.c_double),
                 ("ms_extend", C.c_double), ("ms_shade", C.c_double),
                 ("ms_shadow", C.c_double), ("ms_other", C.c_double),
-                ("launches_extend", C.c_int64)]
+                ("launches_extend", C.c_int64), ("shadow_node_visits", C.c_uint64),
+                ("shadow_tri_tests", C.c_uint64), ("bytes_extend", C.c_uint64),
+                ("bytes_shadow", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
