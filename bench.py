@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3"],
+                    help="cfg2 (default) is the metric's configuration; cfg1/cfg3 are extra measurements")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference dispatch for PMC byte counters)")
     return ap.parse_args()
@@ -66,7 +68,7 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    desc = G.config2()
+    desc, wl_text = workload(G, args.workload)
     W, H, spp = desc.camera.width, desc.camera.height, desc.spp
     t0 = time.time()
     scene = prt.Scene(desc)
@@ -108,15 +110,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded procedural scene)",
-        "config": {"workload": "cfg2: quad Cornell box (24 tris) + L6 displaced icosphere (81,920 tris), "
-                               "0.25^2 area light, 512x512, 64 spp (16x2x2), DEPTH 5 / DEPTH_LIMIT 8, NEE",
+        "config": {"workload": wl_text,
                    "width": W, "height": H, "spp": spp, "triangles": sinfo["tris"], "bvh_nodes": sinfo["nodes"],
                    "parallelism": "1 GPU"},
         "mrays_per_s": rays / elapsed / 1e6,
         "rays": {"primary": sum_("primary_rays"), "extension": sum_("extension_rays"),
                  "shadow": sum_("shadow_rays"), "pickup_hits_no_ray": sum_("pickup_hits")},
         "kernel_ms_per_step": {"extend": ms_ext / args.steps, "shade": ms_shade / args.steps,
-                               "shadow": ms_sh / args.steps, "other": sum_("ms_other") / args.steps},
+                               "nee": sum_("ms_nee") / args.steps, "shadow": ms_sh / args.steps,
+                               "other": sum_("ms_other") / args.steps},
+        "mrays_per_s_by_class": {
+            "primary": sum_("primary_rays") / max(1e-9, sum_("ms_extend_primary")) / 1e3,
+            "secondary": sum_("extension_rays") / max(1e-9, ms_ext - sum_("ms_extend_primary")) / 1e3,
+            "shadow": sum_("shadow_rays") / max(1e-9, ms_sh) / 1e3,
+            "note": "rays / time of their traversal kernels"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": b / max(1, nl), "avg_launch_ms": ms / max(1, nl),
@@ -139,6 +146,18 @@ def main():
     print(json.dumps(out))
 
 
+def workload(G, name):
+    if name == "cfg1":
+        return G.config1(), ("cfg1: sphere Cornell box (6 r=1e5 walls, mirror, glass, r=5 light), 320x240, "
+                             "16 spp (4x2x2), DEPTH 5 / DEPTH_LIMIT 8, no NEE")
+    if name == "cfg3":
+        return G.config3(), ("cfg3: ~2.1M triangles in 400 instances (displaced icospheres L3-L6 and boxes) on a "
+                             "100x100 ground, 70/15/15 diffuse/mirror/glass, 1 large + 8 small area lights, "
+                             "1024x768, 64 spp (16x2x2), DEPTH 5 / DEPTH_LIMIT 10, NEE")
+    return G.config2(), ("cfg2: quad Cornell box (24 tris) + L6 displaced icosphere (81,920 tris), "
+                         "0.25^2 area light, 512x512, 64 spp (16x2x2), DEPTH 5 / DEPTH_LIMIT 8, NEE")
+
+
 def cpu_baseline(desc, args, gpu_step_s, ref_np):
     """The FP64 oracle (a restatement of the reference's Render::Run) on a
     band of rows of the same workload, timed on this host's cores."""
@@ -153,7 +172,7 @@ def cpu_baseline(desc, args, gpu_step_s, ref_np):
     dt = time.perf_counter() - t0
     n = W * (r1 - r0) * desc.spp
     res = {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-           "sample": f"rows {r0}..{r1 - 1} of the 512x512x64spp workload ({n} camera samples, {dt:.1f} s)",
+           "sample": f"rows {r0}..{r1 - 1} of the {W}x{H}x{desc.spp}spp workload ({n} camera samples, {dt:.1f} s)",
            "mrays_per_s": (st["primary"] + st["extension"] + st["shadow"] + st["pickup"]) / dt / 1e6}
     if ref_np is not None:
         rows = slice(H - r1, H - r0)  # image rows are stored top-down (render.h:1234)

@@ -168,6 +168,8 @@ typedef struct prt_stats {
     uint64_t shadow_tri_tests;    /* part of tri_tests spent on shadow rays */
     uint64_t bytes_extend;        /* algorithmic bytes of the closest-hit kernels */
     uint64_t bytes_shadow;        /* algorithmic bytes of the shadow kernels */
+    double ms_extend_primary;     /* part of ms_extend spent on camera rays */
+    double ms_nee;                /* next-event-estimation kernels (light sampling) */
 } prt_stats;
 
 typedef struct prt_render_opts {

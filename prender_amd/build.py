@@ -29,23 +29,25 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: ROCm is required to build libprt")
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(LIB_DIR, exist_ok=True)
-    tmp = LIB_PATH + ".tmp"
+def build(verbose: bool = False, defines=(), out: str = LIB_PATH) -> str:
+    """Compile libprt.so; ``defines`` are extra -D options (tuning variants)."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp"
     cmd = [hipcc(), "-O3", f"--offload-arch={ARCH}", "-std=c++17", "-fPIC", "-shared",
            "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
            "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
-           "-I", CSRC] + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp, "-lpthread"]
+           "-I", CSRC] + [f"-D{d}" for d in defines] + [os.path.join(CSRC, s) for s in SOURCES] + \
+        ["-o", tmp, "-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
-    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    log = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
     if r.returncode != 0:
-        raise RuntimeError("libprt build failed:\n" + out)
-    if verbose and out:
-        print(out)
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+        raise RuntimeError("libprt build failed:\n" + log)
+    if verbose and log:
+        print(log)
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

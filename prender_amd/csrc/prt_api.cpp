@@ -581,7 +581,7 @@ struct Timer {
     marks.clear();
   }
 };
-enum { T_EXTEND = 0, T_SHADE = 1, T_SHADOW = 2, T_OTHER = 3 };
+enum { T_EXTEND = 0, T_SHADE = 1, T_SHADOW = 2, T_OTHER = 3, T_EXTEND0 = 4, T_NEE = 5, T_NCAT = 6 };
 
 static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, float* d_var, prt_stats* stats) {
   HIPCHK(hipSetDevice(s->device));
@@ -632,15 +632,16 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
     if ((rc = tm.mark(T_OTHER))) return rc;
     int fi = 0;
     HIPCHK(launch_extend(P, q[0], counts + 3, np, fetch + fi++, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb, st));
-    if ((rc = tm.mark(T_EXTEND))) return rc;
+    if ((rc = tm.mark(T_EXTEND0))) return rc;
     primary_rays += (uint64_t)np;
     launches++;
     // depth 0: every camera sample against its sub-pixel's primary hit
     HIPCHK(launch_shade(P, B, 1, q[0], counts + 3, ns, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1],
                         counts + 1, (float*)w.accum.p, ctr, st));
+    if ((rc = tm.mark(T_SHADE))) return rc;
     if (P.nee && P.n_lights > 0)
       HIPCHK(launch_nee(P, B, q[1], counts + 1, w.queue_cap, sq, counts + 2, ctr, st));
-    if ((rc = tm.mark(T_SHADE))) return rc;
+    if ((rc = tm.mark(T_NEE))) return rc;
     HIPCHK(launch_shadow(P, sq, counts + 2, w.shadow_cap, fetch + fi++, (float*)w.accum.p, ctr, tb, st));
     if ((rc = tm.mark(T_SHADOW))) return rc;
     int cur = 1;
@@ -664,9 +665,10 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
       launches++;
       HIPCHK(launch_shade(P, B, 0, q[cur], counts + cur, n, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p,
                           q[nxt], counts + nxt, (float*)w.accum.p, ctr, st));
+      if ((rc = tm.mark(T_SHADE))) return rc;
       if (P.nee && P.n_lights > 0)
         HIPCHK(launch_nee(P, B, q[nxt], counts + nxt, w.queue_cap, sq, counts + 2, ctr, st));
-      if ((rc = tm.mark(T_SHADE))) return rc;
+      if ((rc = tm.mark(T_NEE))) return rc;
       HIPCHK(launch_shadow(P, sq, counts + 2, w.shadow_cap, fetch + fi++, (float*)w.accum.p, ctr, tb, st));
       if ((rc = tm.mark(T_SHADOW))) return rc;
       cur = nxt;
@@ -678,7 +680,7 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   HIPCHK(hipMemcpyAsync(c, ctr, sizeof(c), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (c[CTR_DROP] != 0) return fail(PRT_E_INVALID, "ray queue overflow (" + std::to_string(c[CTR_DROP]) + " rays)");
-  double cat[4] = {0, 0, 0, 0};
+  double cat[T_NCAT] = {0, 0, 0, 0, 0, 0};
   tm.collect(cat);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
@@ -691,7 +693,10 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
     stats->ms_shade = cat[T_SHADE];
     stats->ms_shadow = cat[T_SHADOW];
     stats->ms_other = cat[T_OTHER];
-    stats->ms_total = cat[0] + cat[1] + cat[2] + cat[3];
+    stats->ms_extend_primary = cat[T_EXTEND0];
+    stats->ms_nee = cat[T_NEE];
+    stats->ms_extend += cat[T_EXTEND0];
+    stats->ms_total = cat[0] + cat[1] + cat[2] + cat[3] + cat[4] + cat[5];
     stats->launches_extend = launches;
     // per ray query: 128 B per node record, 48 B per triangle record, 48 B
     // of ray in + hit out

@@ -15,7 +15,9 @@
 #include <stdint.h>
 
 #define PRT_MAX_LEAF 8
+#ifndef PRT_STACK_LDS
 #define PRT_STACK_LDS 16
+#endif
 #define PRT_STACK_TOTAL 128
 
 struct __attribute__((aligned(16))) DevNode {

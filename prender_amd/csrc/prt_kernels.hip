@@ -23,8 +23,22 @@
 #include "prt_math.h"
 #include "prt_trace.h"
 
+// Register budgets (waves per SIMD) per kernel class; 0 leaves the choice to
+// the compiler.  Chosen by measurement (the tuning sweep builds variants
+// with -DPRT_WAVES_*).
+#ifndef PRT_WAVES_TRAVERSE
+#define PRT_WAVES_TRAVERSE 0
+#endif
+#ifndef PRT_WAVES_SHADE
+#define PRT_WAVES_SHADE 0
+#endif
+#ifndef PRT_WAVES_NEE
+#define PRT_WAVES_NEE 0
+#endif
+#define PRT_WAVES_ATTR(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1)))
+
 // ------------------------------------------------------------ trace API kernel
-__global__ __launch_bounds__(256) void k_trace(Params P, const prt_ray_dev* __restrict__ rays, int64_t n,
+__global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_trace(Params P, const prt_ray_dev* __restrict__ rays, int64_t n,
                                                prt_hit_dev* __restrict__ hits, uint64_t* ctr) {
   PRT_DECLARE_STACK(S, P);
   TravStats ts{0, 0};
@@ -78,7 +92,7 @@ __global__ __launch_bounds__(256) void k_gen_primary(Params P, Batch B, RayQ q) 
 
 // ------------------------------------------------------------ extend / shadow
 // Closest hit for every ray of a queue; each wave fetches 64 rays at a time.
-__global__ __launch_bounds__(256) void k_extend(Params P, RayQ q, const uint32_t* __restrict__ count_ptr,
+__global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_extend(Params P, RayQ q, const uint32_t* __restrict__ count_ptr,
                                                 uint32_t* fetch, double* __restrict__ hit_t,
                                                 int32_t* __restrict__ hit_id, uint64_t* ctr) {
   PRT_DECLARE_STACK(S, P);
@@ -103,7 +117,7 @@ __global__ __launch_bounds__(256) void k_extend(Params P, RayQ q, const uint32_t
 
 // Any hit for NEE shadow rays; an unoccluded ray adds its contribution to the
 // camera sample's accumulator.
-__global__ __launch_bounds__(256) void k_shadow(Params P, ShadowQ q, const uint32_t* __restrict__ count_ptr,
+__global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_shadow(Params P, ShadowQ q, const uint32_t* __restrict__ count_ptr,
                                                 uint32_t* fetch, float* __restrict__ accum, uint64_t* ctr) {
   PRT_DECLARE_STACK(S, P);
   const uint32_t n = *count_ptr;
@@ -288,7 +302,7 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
 
 // Shade one queue (or, with primary = 1, every camera sample of the batch
 // against the hit of its sub-pixel's primary ray).
-__global__ __launch_bounds__(256) void k_shade(Params P, Batch B, int primary, RayQ qin,
+__global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(Params P, Batch B, int primary, RayQ qin,
                                                const uint32_t* __restrict__ count_in, const double* __restrict__ hit_t,
                                                const int32_t* __restrict__ hit_id, RayQ qout, uint32_t* count_out,
                                                float* __restrict__ accum, uint64_t* ctr) {
@@ -416,7 +430,7 @@ __device__ __forceinline__ void put_shadow(ShadowQ& q, uint32_t j, const ShadowR
 // origin, throughput T*colour/rr and orienting normal), with the parent
 // vertex's random numbers (dimensions 4..12).  Each wave appends its shadow
 // rays with one prefix sum and one atomic.
-__global__ __launch_bounds__(256) void k_nee(Params P, Batch B, RayQ q, const uint32_t* __restrict__ count_ptr,
+__global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(Params P, Batch B, RayQ q, const uint32_t* __restrict__ count_ptr,
                                              ShadowQ sq, uint32_t* count_sq, uint64_t* ctr) {
   const uint32_t n = *count_ptr;
   const uint32_t stride = gridDim.x * blockDim.x;

@@ -19,7 +19,8 @@ from .scene import (CDesc, HIT_DTYPE, PrtBatchOpts, PrtRenderOpts, PrtSceneDesc,
                     SceneDesc)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libprt.so")
+# PRT_LIBRARY selects another build of libprt.so (e.g. a tuning variant)
+LIB_PATH = os.environ.get("PRT_LIBRARY") or os.path.join(HERE, "_lib", "libprt.so")
 
 _lib = None
 

@@ -105,3 +105,25 @@ def test_render_parity_cfg2_common_random_numbers(prt):
     assert sigma_test(rg, vg, ro, vo, 1.0) < 0.01
     assert st["shadow_rays"] > 0
     print("cfg2 CRN: max err", err.max(), "gpu", st, "cpu", so)
+
+
+def test_hit_parity_cfg3_instances(prt):
+    # cfg3 generator with a reduced instance set: many POLYGON entities in one
+    # device BVH against the oracle's per-entity QBVH loop
+    d = G.config3(width=64, height=48, samples=1, scale=0.08)
+    rays = G.hit_parity_rays(d, n=1 << 15)
+    print("hit parity cfg3 (scaled):", len(d.entities), "entities,", d.n_triangles, "tris", flush=True)
+    hg, _ = prt.Scene(d).trace(rays)
+    ho, _, _ = OracleScene(d).trace(rays, "qbvh")
+    assert_hits_match(hg, ho)
+
+
+def test_render_parity_cfg3_common_random_numbers(prt):
+    d = G.config3(width=48, height=36, samples=2, scale=0.08)
+    print("render cfg3 (scaled) 48x36x8spp seed 12", flush=True)
+    rg, vg, st = prt.Scene(d).render(seed=12)
+    ro, vo, so = OracleScene(d).render(seed=12)
+    err = np.abs(rg - ro)
+    assert np.median(err) < 1e-3 * max(1e-6, float(np.median(ro)))
+    assert sigma_test(rg, vg, ro, vo, 1.0) < 0.02
+    print("cfg3 CRN: max err", err.max(), "means", rg.mean(), ro.mean(), "gpu", st, "cpu", so)
