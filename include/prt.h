@@ -179,7 +179,7 @@ typedef struct prt_render_opts {
     int32_t max_depth;         /* safety cap on path vertices (0 = 128) */
     int32_t use_graph;         /* 1 = replay the depth loop as a hipGraph */
     int32_t timing;            /* 1 = per-kernel device timing into prt_stats */
-    int32_t pad_;
+    int32_t image_f64;         /* 1 = out_rgb / out_var are double arrays (device entry points) */
 } prt_render_opts;
 
 /* ---- version / device ---------------------------------------------------- */
@@ -233,6 +233,11 @@ int prt_render_batch_device(prt_scene* const* scenes, int n_scenes,
  * device pointer) and scale factor of the emission of light entity `light`. */
 int prt_set_face_albedo_device(prt_scene* scene, int entity, const float* d_albedo);
 int prt_set_emission(prt_scene* scene, int entity, const double emission[3]);
+/* hold = 1: Russian-roulette probabilities keep using the per-face albedo as
+ * it is now, whatever later prt_set_face_albedo_device calls set (the
+ * primal decisions, for finite-difference checks under common random
+ * numbers); hold = 0: they follow the current albedo again. */
+int prt_hold_rr_albedo(prt_scene* scene, int hold);
 /* Path-replay adjoint: given dL/dimage (W*H*3 device floats), accumulate
  * dL/dalbedo into d_albedo_grad (n_faces*3 device floats of `entity`) and
  * dL/demission (3 doubles, host) for light `light`. */

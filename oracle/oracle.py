@@ -52,6 +52,10 @@ def lib():
         L.orc_set_max_vertices.argtypes = [C.c_void_p, C.c_int]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                         C.POINTER(C.c_uint32)]
+        L.orc_adjoint.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                  C.POINTER(C.c_double), C.c_int]
+        L.orc_set_face_albedo.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        L.orc_set_emission.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -130,3 +134,23 @@ class OracleScene:
         buf = (C.c_int32 * 256)()
         n = lib().orc_light_order(self.h, buf, 256)
         return [int(buf[i]) for i in range(n)]
+
+    def set_face_albedo(self, entity: int, albedo: np.ndarray):
+        a = np.ascontiguousarray(albedo, dtype=np.float32)
+        assert lib().orc_set_face_albedo(self.h, entity, a.ctypes.data) == 0
+
+    def set_emission(self, entity: int, emission):
+        e = (C.c_double * 3)(*[float(x) for x in emission])
+        assert lib().orc_set_emission(self.h, entity, e) == 0
+
+    def adjoint(self, seed: int, dimage: np.ndarray, entity: int, light: int, samples: Optional[int] = None,
+                threads: int = 0):
+        """(d loss/d albedo [n_faces, 3], d loss/d emission [3]) for the
+        adjoint image dimage [H, W, 3]."""
+        d = np.ascontiguousarray(dimage, dtype=np.float64)
+        nf = self.desc.entities[entity].indices.shape[0] if entity >= 0 else 0
+        g = np.zeros((nf, 3), dtype=np.float64)
+        ge = (C.c_double * 3)()
+        rc = lib().orc_adjoint(self.h, seed, samples or 0, d.ctypes.data, entity, g.ctypes.data, light, ge, threads)
+        assert rc == 0, rc
+        return g, np.array(ge[:])

@@ -643,14 +643,51 @@ static bool intersect_scene(const Scene& S, const Ray& ray_in, Hit& hit, Stats& 
 }
 
 // ---------------------------------------------------------------- transport
+// Adjoint accumulators of one thread (the CPU restatement of the GPU's
+// path-replay adjoint): d(loss)/d(face albedo) of one POLYGON entity and
+// d(loss)/d(emission) of one light, with Russian-roulette probabilities held
+// fixed.  g is d(loss)/d(image) at the sample's pixel times the image's
+// derivative with respect to the sample radiance.
+struct Adj {
+  int entity = -1, light = -1;
+  V3 g;
+  std::vector<double> grad_alb;  // n_faces * 3
+  double grad_e[3] = {0, 0, 0};
+};
+
 struct Ctx {
   const Scene* S;
   uint32_t key[2];
   uint32_t pixel, sample;
   Stats st;
+  Adj* ad = nullptr;
 };
 
-static V3 radiance(Ctx& C, const Ray& ray, int depth, uint32_t branch);
+static V3 radiance(Ctx& C, const Ray& ray, int depth, uint32_t branch, const V3& T);
+
+// d(loss) += g . T . dR/dp for a term R linear in the parameter p: adds
+// g * T * R / p per channel.
+static inline void adj_emission(Ctx& C, const V3& T, const V3& term_over_e) {
+  Adj& a = *C.ad;
+  a.grad_e[0] += a.g.x * T.x * term_over_e.x;
+  a.grad_e[1] += a.g.y * T.y * term_over_e.y;
+  a.grad_e[2] += a.g.z * T.z * term_over_e.z;
+}
+static inline V3 safe_div(const V3& a, const V3& b) {
+  return V3(b.x != 0 ? a.x / b.x : 0.0, b.y != 0 ? a.y / b.y : 0.0, b.z != 0 ? a.z / b.z : 0.0);
+}
+// A vertex whose continuation and NEE scale with its colour rho: R - Env - E
+// is linear in rho.
+static inline void adj_albedo(Ctx& C, const Hit& h, const V3& T, const V3& R, const Mat& m) {
+  Adj& a = *C.ad;
+  if (h.entity != a.entity || h.prim < 0) return;
+  const V3 rest = R - C.S->env_light - m.emission;
+  const V3 d = safe_div(rest, m.color);
+  double* g = &a.grad_alb[3 * (size_t)h.prim];
+  g[0] += a.g.x * T.x * d.x;
+  g[1] += a.g.y * T.y * d.y;
+  g[2] += a.g.z * T.z * d.z;
+}
 
 static inline Mat hit_material(const Scene& S, const Hit& h) {
   const Ent& e = S.ents[h.entity];
@@ -664,11 +701,14 @@ static inline Mat hit_material(const Scene& S, const Hit& h) {
 
 // direct_light_perfect_refrection, directLight_radiance.h:516-556 (infinity,
 // parallel and spot lights are out of scope)
-static V3 pickup(Ctx& C, const V3& pos, const V3& dir) {
+static V3 pickup(Ctx& C, const V3& pos, const V3& dir, int* hit_entity = nullptr) {
   Hit h;
   C.st.rays_pickup++;
-  if (intersect_scene(*C.S, Ray{pos, dir}, h, C.st) && C.S->ents[h.entity].light)
+  if (hit_entity) *hit_entity = -1;
+  if (intersect_scene(*C.S, Ray{pos, dir}, h, C.st) && C.S->ents[h.entity].light) {
+    if (hit_entity) *hit_entity = h.entity;
     return hit_material(*C.S, h).emission;
+  }
   return V3();
 }
 
@@ -719,24 +759,24 @@ static V3 direct_sample(Ctx& C, const PathKey& pk, int i, const Light& L, const 
 
 // BRDF_Diffuse, brdf.h:408-700
 static V3 brdf_diffuse(Ctx& C, const PathKey& pk, const Ray& ray, const Hit& h, const Mat& m, const V3& n_orient,
-                       double rr, int depth, uint32_t branch) {
+                       double rr, int depth, uint32_t branch, const V3& T) {
   const Scene& S = *C.S;
   const Ent& e = S.ents[h.entity];
   if (e.light && e.type == PRT_ENTITY_UVPLANE) {
     if (dot(h.normal, ray.dir) >= 0.0) return V3();
   }
-  if (S.nee) {
-    if (e.light) return depth == 0 ? m.emission / rr / (1.0 - 0.0) : V3();
-  } else {
-    if (e.light) return m.emission / rr / (1.0 - 0.0);
+  if (e.light && (!S.nee || depth == 0)) {
+    if (C.ad && h.entity == C.ad->light) adj_emission(C, T, V3(1.0 / rr, 1.0 / rr, 1.0 / rr));
+    return m.emission / rr / (1.0 - 0.0);
   }
+  if (e.light) return V3();
   V3 w, u, v;
   orthonormal_basis(n_orient, w, u, v);
   const double r1 = PS_TWOPI * rnd(pk, DIM_DIFF_U1);
   const double r2 = rnd(pk, DIM_DIFF_U2), r2s = std::sqrt(r2);
   const V3 dir = normalize((u * std::cos(r1) * r2s + v * std::sin(r1) * r2s + w * std::sqrt(1.0 - r2)));
-  const V3 incoming = radiance(C, Ray{h.position, dir}, depth + 1, branch);
   const V3 weight = m.color / rr / (1.0 - 0.0);
+  const V3 incoming = radiance(C, Ray{h.position, dir}, depth + 1, branch, mul(T, weight));
   V3 direct;
   if (S.nee && !S.lights.empty()) {
     const int N = (int)S.lights.size();
@@ -746,27 +786,45 @@ static V3 brdf_diffuse(Ctx& C, const PathKey& pk, const Ray& ray, const Hit& h, 
       int index = (int)((next) * (double)(N - 1) + 0.5);
       if (index >= N) index = N - 1;
       if (index < 0) index = 0;
-      direct = direct + direct_sample(C, pk, i, S.lights[index], h.position, n_orient, m.color) / 1.0;
+      const V3 dl = direct_sample(C, pk, i, S.lights[index], h.position, n_orient, m.color) / 1.0;
+      direct = direct + dl;
+      if (C.ad && S.lights[index].entity == C.ad->light)
+        adj_emission(C, T, safe_div(dl / 3 / rr, S.mats[S.ents[S.lights[index].entity].mat].emission));
     }
     direct = direct / 3;
     direct = direct / rr / (1.0 - 0.0);
   }
-  return S.env_light + m.emission + direct + mul(weight, incoming);
+  const V3 R = S.env_light + m.emission + direct + mul(weight, incoming);
+  if (C.ad) {
+    adj_albedo(C, h, T, R, m);
+    if (h.entity == C.ad->light) adj_emission(C, T, V3(1, 1, 1));
+  }
+  return R;
 }
 
 // BRDF_Specular, brdf.h:702-761
-static V3 brdf_specular(Ctx& C, const Ray& ray, const Hit& h, const Mat& m, double rr, int depth, uint32_t branch) {
+static V3 brdf_specular(Ctx& C, const Ray& ray, const Hit& h, const Mat& m, double rr, int depth, uint32_t branch,
+                        const V3& T) {
   const Scene& S = *C.S;
   const V3 dir = normalize(ray.dir - h.normal * 2.0 * dot(h.normal, ray.dir));
-  V3 incoming = radiance(C, Ray{h.position, dir}, depth + 1, branch);
   const V3 weight = m.color / rr / (1.0 - 0.0);
-  if (S.nee) incoming = incoming + pickup(C, h.position, dir);
-  return S.env_light + m.emission + mul(weight, incoming);
+  V3 incoming = radiance(C, Ray{h.position, dir}, depth + 1, branch, mul(T, weight));
+  if (S.nee) {
+    int pe = -1;
+    incoming = incoming + pickup(C, h.position, dir, &pe);
+    if (C.ad && pe >= 0 && pe == C.ad->light) adj_emission(C, mul(T, weight), V3(1, 1, 1));
+  }
+  const V3 R = S.env_light + m.emission + mul(weight, incoming);
+  if (C.ad) {
+    adj_albedo(C, h, T, R, m);
+    if (h.entity == C.ad->light) adj_emission(C, T, V3(1, 1, 1));
+  }
+  return R;
 }
 
 // BRDF_Refraction, brdf.h:763-953
 static V3 brdf_refraction(Ctx& C, const PathKey& pk, const Ray& ray, const Hit& h, const Mat& m,
-                          const V3& n_orient, double rr, int depth, uint32_t branch) {
+                          const V3& n_orient, double rr, int depth, uint32_t branch, const V3& T) {
   const Scene& S = *C.S;
   const V3 refl_dir = normalize(ray.dir - h.normal * 2.0 * dot(h.normal, ray.dir));
   const bool into = dot(h.normal, n_orient) > 0.0;
@@ -779,8 +837,8 @@ static V3 brdf_refraction(Ctx& C, const PathKey& pk, const Ray& ray, const Hit& 
   const double cos2t = 1.0 - nnt * nnt * (1.0 - ddn * ddn);
   V3 incoming, weight;
   if (cos2t < 0.0) {
-    incoming = radiance(C, Ray{h.position, refl_dir}, depth + 1, branch);
     weight = m.color / rr / (1.0 - 0.0);
+    incoming = radiance(C, Ray{h.position, refl_dir}, depth + 1, branch, mul(T, weight));
     incoming = incoming + direct;
     return S.env_light + m.emission + mul(weight, incoming);
   }
@@ -795,25 +853,25 @@ static V3 brdf_refraction(Ctx& C, const PathKey& pk, const Ray& ray, const Hit& 
   if (S.nee) direct_t = pickup(C, h.position, tdir);
   if (depth > 2) {
     if (rnd(pk, DIM_GLASS) < probability) {
-      incoming = radiance(C, Ray{h.position, refl_dir}, depth + 1, branch) * Re;
       weight = m.color / (probability * rr) / (1.0 - 0.0);
+      incoming = radiance(C, Ray{h.position, refl_dir}, depth + 1, branch, mul(T, weight) * Re) * Re;
       incoming = incoming + direct;
     } else {
-      incoming = radiance(C, Ray{h.position, tdir}, depth + 1, branch) * Tr;
       weight = m.color / ((1.0 - probability) * rr) / (1.0 - 0.0);
+      incoming = radiance(C, Ray{h.position, tdir}, depth + 1, branch, mul(T, weight) * Tr) * Tr;
       incoming = incoming + direct_t;
     }
   } else {
-    incoming = radiance(C, Ray{h.position, refl_dir}, depth + 1, branch * 2) * Re +
-               radiance(C, Ray{h.position, tdir}, depth + 1, branch * 2 + 1) * Tr;
     weight = m.color / (rr) / (1.0 - 0.0);
+    incoming = radiance(C, Ray{h.position, refl_dir}, depth + 1, branch * 2, mul(T, weight) * Re) * Re +
+               radiance(C, Ray{h.position, tdir}, depth + 1, branch * 2 + 1, mul(T, weight) * Tr) * Tr;
     incoming = incoming + direct + direct_t;
   }
   return S.env_light + m.emission + mul(weight, incoming);
 }
 
 // radiance, radiance.h:25-556 (RGB build, no media, no SSS)
-static V3 radiance(Ctx& C, const Ray& ray, int depth, uint32_t branch) {
+static V3 radiance(Ctx& C, const Ray& ray, int depth, uint32_t branch, const V3& T) {
   const Scene& S = *C.S;
   if (depth >= S.max_vertices) { C.st.truncated++; return V3(); }
   if (depth == 0) C.st.rays_primary++; else C.st.rays_ext++;
@@ -825,19 +883,22 @@ static V3 radiance(Ctx& C, const Ray& ray, int depth, uint32_t branch) {
   if (depth > S.depth_limit) rr *= std::pow(0.5, depth - S.depth_limit);
   const PathKey pk = vertex_key(C.key, C.pixel, C.sample, depth, branch);
   if (depth > S.depth) {
-    if (rnd(pk, DIM_RR) >= rr) return m.emission;
+    if (rnd(pk, DIM_RR) >= rr) {
+      if (C.ad && h.entity == C.ad->light) adj_emission(C, T, V3(1, 1, 1));
+      return m.emission;
+    }
   } else {
     rr = 1.0;
   }
   switch (m.type) {
     case PRT_REFLECTION_DIFFUSE:
-      if (rnd(pk, DIM_MATERIAL) < m.roughness) return brdf_diffuse(C, pk, ray, h, m, n_orient, rr, depth, branch);
-      return brdf_specular(C, ray, h, m, rr, depth, branch);
+      if (rnd(pk, DIM_MATERIAL) < m.roughness) return brdf_diffuse(C, pk, ray, h, m, n_orient, rr, depth, branch, T);
+      return brdf_specular(C, ray, h, m, rr, depth, branch, T);
     case PRT_REFLECTION_SPECULAR:
-      if (rnd(pk, DIM_MATERIAL) < 1.0 - m.roughness) return brdf_specular(C, ray, h, m, rr, depth, branch);
-      return brdf_diffuse(C, pk, ray, h, m, n_orient, rr, depth, branch);
+      if (rnd(pk, DIM_MATERIAL) < 1.0 - m.roughness) return brdf_specular(C, ray, h, m, rr, depth, branch, T);
+      return brdf_diffuse(C, pk, ray, h, m, n_orient, rr, depth, branch, T);
     case PRT_REFLECTION_REFRACTION:
-      return brdf_refraction(C, pk, ray, h, m, n_orient, rr, depth, branch);
+      return brdf_refraction(C, pk, ray, h, m, n_orient, rr, depth, branch, T);
   }
   return V3();
 }
@@ -1073,7 +1134,7 @@ int orc_radiance_sample(const orc_scene* s, uint64_t seed, int x, int y, int sam
   C.key[0] = (uint32_t)seed; C.key[1] = (uint32_t)(seed >> 32);
   C.pixel = (uint32_t)(y * S.cam.width + x);
   C.sample = (uint32_t)sample;
-  V3 v = radiance(C, camera_ray(S.cam, x, y, sub % ss, sub / ss, ss), 0, 1);
+  V3 v = radiance(C, camera_ray(S.cam, x, y, sub % ss, sub / ss, ss), 0, 1, V3(1, 1, 1));
   out[0] = v.x; out[1] = v.y; out[2] = v.z;
   return PRT_OK;
 }
@@ -1122,7 +1183,7 @@ int orc_render(const orc_scene* s, uint64_t seed, int samples_override, int row_
               C.pixel = (uint32_t)(y * W + x);
               C.sample = (uint32_t)(smp_i * ss * ss + sy * ss + sx);
               const Ray ray = camera_ray(S.cam, x, y, sx, sy, ss);
-              V3 v = radiance(C, ray, 0, 1) * smp;
+              V3 v = radiance(C, ray, 0, 1, V3(1, 1, 1)) * smp;
               // luminance_cutoff is a float in SceneEnv (scene_env.h:221)
               const float cutoff = (float)S.luminance_cutoff;
               if (cutoff > 0.0f) {
@@ -1162,6 +1223,86 @@ int orc_render(const orc_scene* s, uint64_t seed, int samples_override, int row_
     stats[0] = total.rays_primary; stats[1] = total.rays_ext; stats[2] = total.rays_shadow;
     stats[3] = total.rays_pickup; stats[4] = total.nodes; stats[5] = total.tris; stats[6] = total.truncated;
   }
+  return PRT_OK;
+}
+
+// CPU adjoint of Render::Run's image with respect to the per-face albedo of
+// POLYGON entity `entity` and the emission of light entity `light`, for the
+// adjoint image dimage (W*H*3 doubles, image row order): the FP64 recursion
+// carries the path throughput T, and a vertex's return value is the
+// radiance gathered from it onwards (no luminance clamp, no glass).
+int orc_adjoint(const orc_scene* s, uint64_t seed, int samples_override, const double* dimage, int entity,
+                double* grad_alb, int light, double* grad_e, int nthreads) {
+  if (!s || !dimage) return PRT_E_INVALID;
+  const Scene& S = s->S;
+  for (const Mat& m : S.mats)
+    if (m.type == PRT_REFLECTION_REFRACTION) return PRT_E_UNSUPPORTED;
+  if (S.luminance_cutoff > 0.0) return PRT_E_UNSUPPORTED;
+  const int W = S.cam.width, H = S.cam.height;
+  const int samples = samples_override > 0 ? samples_override : S.samples;
+  const int ss = S.supersamples;
+  const double smp = 1.0 / samples / (ss * ss);
+  const size_t nf = (entity >= 0 && entity < (int)S.ents.size() && S.ents[entity].type == PRT_ENTITY_POLYGON)
+                        ? (size_t)S.ents[entity].mesh.nf : 0;
+  std::vector<double> total(nf * 3, 0.0);
+  double ge[3] = {0, 0, 0};
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+#endif
+  {
+    Adj ad;
+    ad.entity = nf ? entity : -1;
+    ad.light = light;
+    ad.grad_alb.assign(nf * 3, 0.0);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+    for (int y = 0; y < H; ++y) {
+      for (int x = 0; x < W; ++x) {
+        const size_t ii = (size_t)(H - y - 1) * W + x;
+        const double sc = smp * S.sensor_response;
+        ad.g = V3(dimage[3 * ii] * sc, dimage[3 * ii + 1] * sc, dimage[3 * ii + 2] * sc);
+        for (int k = 0; k < samples * ss * ss; ++k) {
+          const int sub = k % (ss * ss);
+          Ctx C;
+          C.S = &S;
+          C.key[0] = (uint32_t)seed; C.key[1] = (uint32_t)(seed >> 32);
+          C.pixel = (uint32_t)(y * W + x);
+          C.sample = (uint32_t)k;
+          C.ad = &ad;
+          radiance(C, camera_ray(S.cam, x, y, sub % ss, sub / ss, ss), 0, 1, V3(1, 1, 1));
+        }
+      }
+    }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+    {
+      for (size_t i = 0; i < total.size(); ++i) total[i] += ad.grad_alb[i];
+      for (int c = 0; c < 3; ++c) ge[c] += ad.grad_e[c];
+    }
+  }
+  if (grad_alb) std::memcpy(grad_alb, total.data(), total.size() * sizeof(double));
+  if (grad_e) std::memcpy(grad_e, ge, sizeof(ge));
+  return PRT_OK;
+}
+
+int orc_set_face_albedo(orc_scene* s, int entity, const float* albedo) {
+  if (!s || !albedo || entity < 0 || entity >= (int)s->S.ents.size()) return PRT_E_INVALID;
+  Ent& e = s->S.ents[entity];
+  if (e.type != PRT_ENTITY_POLYGON || e.mesh.albedo.empty()) return PRT_E_INVALID;
+  std::memcpy(e.mesh.albedo.data(), albedo, e.mesh.albedo.size() * sizeof(float));
+  return PRT_OK;
+}
+
+int orc_set_emission(orc_scene* s, int entity, const double emission[3]) {
+  if (!s || !emission || entity < 0 || entity >= (int)s->S.ents.size()) return PRT_E_INVALID;
+  Scene& S = s->S;
+  Mat& m = S.mats[S.ents[entity].mat];
+  m.emission = from(emission);
+  for (Light& L : S.lights) L.power = S.ents[L.entity].area * length(S.mats[S.ents[L.entity].mat].emission);
+  std::stable_sort(S.lights.begin(), S.lights.end(), [](const Light& a, const Light& b) { return a.power > b.power; });
   return PRT_OK;
 }
 

@@ -91,8 +91,8 @@ struct Workspace {
   int64_t shadow_cap = 0;
   DevBuf q[2][17];            // two ray queues (SoA)
   DevBuf hit_t, hit_id;
-  DevBuf sq[11];
-  DevBuf accum;
+  DevBuf sq[12];
+  DevBuf accum, accum2;       // fixed-point sample radiance (accum2: adjoint primal)
   DevBuf counts;              // uint32 [4]: q0 count, q1 count, shadow count, spare
   DevBuf fetch;               // uint32 [512] work-fetch counters, one per launch
   DevBuf ctr;                 // uint64 [CTR_COUNT]
@@ -104,6 +104,7 @@ struct Workspace {
     hit_id.release();
     for (auto& b : sq) b.release();
     accum.release();
+    accum2.release();
     counts.release();
     fetch.release();
     ctr.release();
@@ -131,6 +132,7 @@ ShadowQ shadow_queue(DevBuf* b) {
   q.tmax = (double*)b[6].p;
   q.cr = (float*)b[7].p; q.cg = (float*)b[8].p; q.cb = (float*)b[9].p;
   q.slot = (uint32_t*)b[10].p;
+  q.light = (int32_t*)b[11].p;
   return q;
 }
 
@@ -156,6 +158,9 @@ struct prt_scene {
   int max_trav_blocks = 0;
   Workspace ws;
   DevBuf out_rgb, out_var;  // staging for host-output renders
+  DevBuf d_slot_of_tri;     // input triangle -> leaf slot (per-face albedo updates)
+  DevBuf grad_buf;          // adjoint accumulators (double)
+  DevBuf tri_alb_rr;        // held albedo for Russian-roulette probabilities
 };
 
 extern "C" {
@@ -187,6 +192,9 @@ void prt_scene_destroy(prt_scene* s) {
   s->ws.release();
   s->out_rgb.release();
   s->out_var.release();
+  s->d_slot_of_tri.release();
+  s->grad_buf.release();
+  s->tri_alb_rr.release();
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -356,6 +364,7 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   if ((rc = dev_upload(s->mats, s->h_mats, st))) return rc;
   if ((rc = dev_upload(s->analytic, analytic, st))) return rc;
   if ((rc = dev_upload(s->lights, s->h_lights, st))) return rc;
+  if (any_albedo && (rc = dev_upload(s->d_slot_of_tri, s->slot_of_tri, st))) return rc;
   // traversal stack overflow: one area per thread of the largest grid
   s->max_trav_blocks = std::max(1, s->cu_count) * 8;
   const int64_t ovf_threads = (int64_t)s->max_trav_blocks * 256;
@@ -518,12 +527,13 @@ int prt_trace_closest(prt_scene* s, const prt_ray* rays, int64_t n, prt_hit* hit
 }
 
 // ---------------------------------------------------------------- render
-static int ensure_workspace(prt_scene* s, int64_t batch_samples, int64_t n_pixels) {
+static int ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum) {
   Workspace& w = s->ws;
   const int64_t fan = s->has_glass ? 8 : 1;
   const int64_t qcap = batch_samples * fan;
   const int64_t scap = 3 * qcap;
-  if (w.batch_samples >= batch_samples && w.queue_cap >= qcap && w.h_counts) return PRT_OK;
+  if (w.batch_samples >= batch_samples && w.queue_cap >= qcap && w.h_counts && (!two_accum || w.accum2.p))
+    return PRT_OK;
   w.release();
   int rc;
   const size_t dsz[17] = {8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
@@ -532,10 +542,11 @@ static int ensure_workspace(prt_scene* s, int64_t batch_samples, int64_t n_pixel
       if ((rc = dev_alloc(w.q[qi][k], (size_t)qcap * dsz[k]))) return rc;
   if ((rc = dev_alloc(w.hit_t, (size_t)qcap * 8))) return rc;
   if ((rc = dev_alloc(w.hit_id, (size_t)qcap * 4))) return rc;
-  const size_t ssz[11] = {8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 4};
-  for (int k = 0; k < 11; ++k)
+  const size_t ssz[12] = {8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4};
+  for (int k = 0; k < 12; ++k)
     if ((rc = dev_alloc(w.sq[k], (size_t)scap * ssz[k]))) return rc;
-  if ((rc = dev_alloc(w.accum, (size_t)batch_samples * 3 * sizeof(float)))) return rc;
+  if ((rc = dev_alloc(w.accum, (size_t)batch_samples * 3 * sizeof(uint64_t)))) return rc;
+  if (two_accum && (rc = dev_alloc(w.accum2, (size_t)batch_samples * 3 * sizeof(uint64_t)))) return rc;
   if ((rc = dev_alloc(w.counts, 16 * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(w.fetch, 1024 * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(w.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
@@ -543,13 +554,12 @@ static int ensure_workspace(prt_scene* s, int64_t batch_samples, int64_t n_pixel
   w.batch_samples = batch_samples;
   w.queue_cap = qcap;
   w.shadow_cap = scap;
-  (void)n_pixels;
   return PRT_OK;
 }
 
 struct Timer {
-  bool on;
-  hipStream_t s;
+  bool on = false;
+  hipStream_t s = nullptr;
   std::vector<std::pair<hipEvent_t, int>> marks;  // event + category of the span that ends here
   hipEvent_t start{};
   int begin(bool enable, hipStream_t st) {
@@ -583,8 +593,13 @@ struct Timer {
 };
 enum { T_EXTEND = 0, T_SHADE = 1, T_SHADOW = 2, T_OTHER = 3, T_EXTEND0 = 4, T_NEE = 5, T_NCAT = 6 };
 
-static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, float* d_var, prt_stats* stats) {
-  HIPCHK(hipSetDevice(s->device));
+// Settings of one render call derived from the scene and the options.
+struct RenderSetup {
+  Params P;
+  int64_t n_pix = 0, bp = 0;
+};
+
+static int setup_render(prt_scene* s, const prt_render_opts* o, RenderSetup& R) {
   prt_render_opts opts{};
   if (o) opts = *o;
   Params P = s->P;
@@ -597,13 +612,27 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   P.max_vertices = opts.max_depth > 0 ? std::min(opts.max_depth, 255) : 128;
   P.key0 = (uint32_t)opts.seed;
   P.key1 = (uint32_t)(opts.seed >> 32);
-  const int64_t n_pix = (int64_t)P.W * P.H;
+  R.n_pix = (int64_t)P.W * P.H;
   int64_t bp = opts.batch_pixels > 0 ? opts.batch_pixels : std::max<int64_t>(1, ((int64_t)1 << 21) / P.spp);
-  bp = std::min(bp, n_pix);
+  bp = std::min(bp, R.n_pix);
   // queues hold up to 8x (glass splits) and 24x (shadow rays) the batch
   while (bp > 1 && bp * P.spp * 24 >= ((int64_t)1 << 32)) bp /= 2;
-  int rc;
-  if ((rc = ensure_workspace(s, bp * P.spp, bp))) return rc;
+  R.bp = bp;
+  R.P = P;
+  return PRT_OK;
+}
+
+// Counters of a call, summed over its batches.
+struct CallTotals {
+  uint64_t primary_rays = 0, ext_rays = 0;
+  int64_t launches = 0;
+};
+
+// One batch through the wavefront: generate, then extend / shade / NEE /
+// shadow per path depth until no ray is left.  Sample radiance accumulates
+// into `accum` (fixed point, 3 per camera sample of the batch).
+static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* accum, const Adjoint& A, Timer& tm,
+                     CallTotals& tot) {
   Workspace& w = s->ws;
   hipStream_t st = s->stream;
   uint32_t* counts = (uint32_t*)w.counts.p;
@@ -613,71 +642,66 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   ShadowQ sq = shadow_queue(w.sq);
   q[0].cap = q[1].cap = (uint32_t)w.queue_cap;
   sq.cap = (uint32_t)w.shadow_cap;
-  HIPCHK(launch_fill_u32((uint32_t*)ctr, 0u, 2 * CTR_COUNT, st));  // uint64 counters as u32 pairs
-  Timer tm;
-  if ((rc = tm.begin(opts.timing != 0, st))) return rc;
-  uint64_t primary_rays = 0, ext_launch_rays = 0;
-  int64_t launches = 0;
   const int tb = s->max_trav_blocks;
-  for (int64_t pb = 0; pb < n_pix; pb += bp) {
-    Batch B{pb, std::min(bp, n_pix - pb)};
-    const int64_t ns = B.n_pixels * P.spp;
-    const int64_t np = B.n_pixels * P.SS * P.SS;
-    HIPCHK(launch_fill_f32((float*)w.accum.p, 0.0f, 3 * ns, st));
-    HIPCHK(launch_fill_u32(fetch, 0u, 1024, st));
-    HIPCHK(launch_fill_u32(counts, 0u, 16, st));
-    // counts[3] = number of primary rays (consumed by the extend kernel)
-    HIPCHK(launch_fill_u32(counts + 3, (uint32_t)np, 1, st));
-    HIPCHK(launch_gen_primary(P, B, q[0], st));
+  const bool nee = P.nee && P.n_lights > 0;
+  int rc;
+  const int64_t ns = B.n_pixels * P.spp;
+  const int64_t np = B.n_pixels * P.SS * P.SS;
+  HIPCHK(launch_fill_u64(accum, 0ull, 3 * ns, st));
+  HIPCHK(launch_fill_u32(fetch, 0u, 1024, st));
+  HIPCHK(launch_fill_u32(counts, 0u, 16, st));
+  // counts[3] = number of primary rays (consumed by the extend kernel)
+  HIPCHK(launch_fill_u32(counts + 3, (uint32_t)np, 1, st));
+  HIPCHK(launch_gen_primary(P, B, q[0], st));
+  if ((rc = tm.mark(T_OTHER))) return rc;
+  int fi = 0;
+  HIPCHK(launch_extend(P, q[0], counts + 3, np, fetch + fi++, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb, st));
+  if ((rc = tm.mark(T_EXTEND0))) return rc;
+  tot.primary_rays += (uint64_t)np;
+  tot.launches++;
+  // depth 0: every camera sample against its sub-pixel's primary hit
+  HIPCHK(launch_shade(P, B, 1, q[0], counts + 3, ns, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1],
+                      counts + 1, accum, A, ctr, st));
+  if ((rc = tm.mark(T_SHADE))) return rc;
+  if (nee) HIPCHK(launch_nee(P, B, q[1], counts + 1, w.queue_cap, sq, counts + 2, ctr, st));
+  if ((rc = tm.mark(T_NEE))) return rc;
+  HIPCHK(launch_shadow(P, B, sq, counts + 2, w.shadow_cap, fetch + fi++, accum, A, ctr, tb, st));
+  if ((rc = tm.mark(T_SHADOW))) return rc;
+  int cur = 1;
+  while (true) {
+    HIPCHK(hipMemcpyAsync(w.h_counts, counts, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const uint32_t n = w.h_counts[cur];
+    if (n == 0) break;
+    if ((int64_t)n > w.queue_cap || (int64_t)w.h_counts[2] > w.shadow_cap)
+      return fail(PRT_E_INVALID, "ray queue overflow");
+    if (fi + 2 >= 1024) return fail(PRT_E_INVALID, "too many wavefront launches in one batch");
+    const int nxt = cur ^ 1;
+    // reset the output queue and the shadow queue for this depth
+    HIPCHK(launch_fill_u32(counts + nxt, 0u, 1, st));
+    HIPCHK(launch_fill_u32(counts + 2, 0u, 1, st));
     if ((rc = tm.mark(T_OTHER))) return rc;
-    int fi = 0;
-    HIPCHK(launch_extend(P, q[0], counts + 3, np, fetch + fi++, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb, st));
-    if ((rc = tm.mark(T_EXTEND0))) return rc;
-    primary_rays += (uint64_t)np;
-    launches++;
-    // depth 0: every camera sample against its sub-pixel's primary hit
-    HIPCHK(launch_shade(P, B, 1, q[0], counts + 3, ns, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1],
-                        counts + 1, (float*)w.accum.p, ctr, st));
+    HIPCHK(launch_extend(P, q[cur], counts + cur, n, fetch + fi++, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb,
+                         st));
+    if ((rc = tm.mark(T_EXTEND))) return rc;
+    tot.ext_rays += n;
+    tot.launches++;
+    HIPCHK(launch_shade(P, B, 0, q[cur], counts + cur, n, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[nxt],
+                        counts + nxt, accum, A, ctr, st));
     if ((rc = tm.mark(T_SHADE))) return rc;
-    if (P.nee && P.n_lights > 0)
-      HIPCHK(launch_nee(P, B, q[1], counts + 1, w.queue_cap, sq, counts + 2, ctr, st));
+    if (nee) HIPCHK(launch_nee(P, B, q[nxt], counts + nxt, w.queue_cap, sq, counts + 2, ctr, st));
     if ((rc = tm.mark(T_NEE))) return rc;
-    HIPCHK(launch_shadow(P, sq, counts + 2, w.shadow_cap, fetch + fi++, (float*)w.accum.p, ctr, tb, st));
+    HIPCHK(launch_shadow(P, B, sq, counts + 2, w.shadow_cap, fetch + fi++, accum, A, ctr, tb, st));
     if ((rc = tm.mark(T_SHADOW))) return rc;
-    int cur = 1;
-    while (true) {
-      HIPCHK(hipMemcpyAsync(w.h_counts, counts, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      const uint32_t n = w.h_counts[cur];
-      if (n == 0) break;
-      if ((int64_t)n > w.queue_cap || (int64_t)w.h_counts[2] > w.shadow_cap)
-        return fail(PRT_E_INVALID, "ray queue overflow");
-      if (fi + 2 >= 1024) return fail(PRT_E_INVALID, "too many wavefront launches in one batch");
-      const int nxt = cur ^ 1;
-      // reset the output queue and the shadow queue for this depth
-      HIPCHK(launch_fill_u32(counts + nxt, 0u, 1, st));
-      HIPCHK(launch_fill_u32(counts + 2, 0u, 1, st));
-      if ((rc = tm.mark(T_OTHER))) return rc;
-      HIPCHK(launch_extend(P, q[cur], counts + cur, n, fetch + fi++, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr,
-                           tb, st));
-      if ((rc = tm.mark(T_EXTEND))) return rc;
-      ext_launch_rays += n;
-      launches++;
-      HIPCHK(launch_shade(P, B, 0, q[cur], counts + cur, n, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p,
-                          q[nxt], counts + nxt, (float*)w.accum.p, ctr, st));
-      if ((rc = tm.mark(T_SHADE))) return rc;
-      if (P.nee && P.n_lights > 0)
-        HIPCHK(launch_nee(P, B, q[nxt], counts + nxt, w.queue_cap, sq, counts + 2, ctr, st));
-      if ((rc = tm.mark(T_NEE))) return rc;
-      HIPCHK(launch_shadow(P, sq, counts + 2, w.shadow_cap, fetch + fi++, (float*)w.accum.p, ctr, tb, st));
-      if ((rc = tm.mark(T_SHADOW))) return rc;
-      cur = nxt;
-    }
-    HIPCHK(launch_resolve(P, B, (const float*)w.accum.p, d_rgb, d_var, st));
-    if ((rc = tm.mark(T_OTHER))) return rc;
+    cur = nxt;
   }
-  uint64_t c[CTR_COUNT];
-  HIPCHK(hipMemcpyAsync(c, ctr, sizeof(c), hipMemcpyDeviceToHost, st));
+  return PRT_OK;
+}
+
+static int finish_stats(prt_scene* s, const RenderSetup& R, Timer& tm, const CallTotals& tot, prt_stats* stats,
+                        uint64_t* c) {
+  hipStream_t st = s->stream;
+  HIPCHK(hipMemcpyAsync(c, s->ws.ctr.p, CTR_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (c[CTR_DROP] != 0) return fail(PRT_E_INVALID, "ray queue overflow (" + std::to_string(c[CTR_DROP]) + " rays)");
   double cat[T_NCAT] = {0, 0, 0, 0, 0, 0};
@@ -685,26 +709,47 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     fill_stats(stats, c);
-    stats->camera_samples = (uint64_t)n_pix * P.spp;
-    stats->primary_rays = primary_rays;
-    stats->extension_rays = ext_launch_rays;
+    stats->camera_samples = (uint64_t)R.n_pix * R.P.spp;
+    stats->primary_rays = tot.primary_rays;
+    stats->extension_rays = tot.ext_rays;
     stats->shadow_rays = c[CTR_SHADOW];
-    stats->ms_extend = cat[T_EXTEND];
+    stats->ms_extend = cat[T_EXTEND] + cat[T_EXTEND0];
     stats->ms_shade = cat[T_SHADE];
     stats->ms_shadow = cat[T_SHADOW];
     stats->ms_other = cat[T_OTHER];
     stats->ms_extend_primary = cat[T_EXTEND0];
     stats->ms_nee = cat[T_NEE];
-    stats->ms_extend += cat[T_EXTEND0];
     stats->ms_total = cat[0] + cat[1] + cat[2] + cat[3] + cat[4] + cat[5];
-    stats->launches_extend = launches;
+    stats->launches_extend = tot.launches;
     // per ray query: 128 B per node record, 48 B per triangle record, 48 B
     // of ray in + hit out
-    stats->bytes_extend = 128 * c[CTR_NODES] + 48 * c[CTR_TRIS] + 48 * (primary_rays + ext_launch_rays);
+    stats->bytes_extend = 128 * c[CTR_NODES] + 48 * c[CTR_TRIS] + 48 * (tot.primary_rays + tot.ext_rays);
     stats->bytes_shadow = 128 * c[CTR_SNODES] + 48 * c[CTR_STRIS] + 48 * c[CTR_SHADOW];
     stats->bytes_algorithmic = stats->bytes_extend + stats->bytes_shadow;
   }
   return PRT_OK;
+}
+
+static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, float* d_var, prt_stats* stats) {
+  HIPCHK(hipSetDevice(s->device));
+  RenderSetup R;
+  int rc;
+  if ((rc = setup_render(s, o, R))) return rc;
+  if ((rc = ensure_workspace(s, R.bp * R.P.spp, false))) return rc;
+  hipStream_t st = s->stream;
+  HIPCHK(launch_fill_u64((uint64_t*)s->ws.ctr.p, 0ull, CTR_COUNT, st));
+  Timer tm;
+  if ((rc = tm.begin(o && o->timing != 0, st))) return rc;
+  CallTotals tot;
+  Adjoint none{};
+  for (int64_t pb = 0; pb < R.n_pix; pb += R.bp) {
+    Batch B{pb, std::min(R.bp, R.n_pix - pb)};
+    if ((rc = run_batch(s, R.P, B, (uint64_t*)s->ws.accum.p, none, tm, tot))) return rc;
+    HIPCHK(launch_resolve(R.P, B, (const uint64_t*)s->ws.accum.p, d_rgb, d_var, o && o->image_f64, st));
+    if ((rc = tm.mark(T_OTHER))) return rc;
+  }
+  uint64_t c[CTR_COUNT];
+  return finish_stats(s, R, tm, tot, stats, c);
 }
 
 int prt_render_device(prt_scene* s, const prt_render_opts* opts, float* d_rgb, float* d_var, prt_stats* stats) {
@@ -719,7 +764,10 @@ int prt_render(prt_scene* s, const prt_render_opts* opts, float* out_rgb, float*
   int rc;
   if ((rc = dev_alloc(s->out_rgb, bytes))) return rc;
   if (out_var && (rc = dev_alloc(s->out_var, bytes))) return rc;
-  rc = render_device(s, opts, (float*)s->out_rgb.p, out_var ? (float*)s->out_var.p : nullptr, stats);
+  prt_render_opts o32{};
+  if (opts) o32 = *opts;
+  o32.image_f64 = 0;  // host outputs are float
+  rc = render_device(s, &o32, (float*)s->out_rgb.p, out_var ? (float*)s->out_var.p : nullptr, stats);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(out_rgb, s->out_rgb.p, bytes, hipMemcpyDeviceToHost, s->stream));
   if (out_var) HIPCHK(hipMemcpyAsync(out_var, s->out_var.p, bytes, hipMemcpyDeviceToHost, s->stream));
@@ -733,19 +781,126 @@ int prt_render_batch_device(prt_scene* const* scenes, int n_scenes, const prt_ba
   return fail(PRT_E_UNSUPPORTED, "batched rendering is not built yet");
 }
 
+// ---------------------------------------------------------------- differentiable parameters
 int prt_set_face_albedo_device(prt_scene* s, int entity, const float* d_albedo) {
-  (void)s; (void)entity; (void)d_albedo;
-  return fail(PRT_E_UNSUPPORTED, "not built yet");
+  if (!s || !d_albedo || entity < 0 || entity >= (int)s->h_ents.size()) return fail(PRT_E_INVALID, "bad arguments");
+  const DevEntity& e = s->h_ents[entity];
+  if (e.type != PRT_ENTITY_POLYGON || !e.has_albedo)
+    return fail(PRT_E_INVALID, "entity is not a POLYGON created with per-face albedo");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(launch_scatter_albedo((float*)s->tri_alb.p, (const uint32_t*)s->d_slot_of_tri.p, s->ent_first_tri[entity],
+                               s->ent_n_tris[entity], d_albedo, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return PRT_OK;
 }
+
+static void sort_lights(prt_scene* s) {
+  struct L { int32_t ent; double power; };
+  std::vector<L> ls;
+  for (int i = 0; i < (int)s->h_ents.size(); ++i) {
+    const DevEntity& de = s->h_ents[i];
+    if (!de.light || de.type == PRT_ENTITY_POLYGON) continue;
+    ls.push_back(L{i, de.area * length(from(s->h_mats[de.material].emission))});
+  }
+  std::stable_sort(ls.begin(), ls.end(), [](const L& a, const L& b) { return a.power > b.power; });
+  s->h_lights.clear();
+  for (const L& l : ls) s->h_lights.push_back(l.ent);
+}
+
+int prt_hold_rr_albedo(prt_scene* s, int hold) {
+  if (!s) return fail(PRT_E_INVALID, "null scene");
+  HIPCHK(hipSetDevice(s->device));
+  if (!hold) {
+    s->P.tri_alb_rr = nullptr;
+    return PRT_OK;
+  }
+  if (!s->tri_alb.p) return fail(PRT_E_INVALID, "scene has no per-face albedo");
+  const size_t bytes = (size_t)s->n_tris * 3 * sizeof(float);
+  int rc;
+  if ((rc = dev_alloc(s->tri_alb_rr, bytes))) return rc;
+  HIPCHK(hipMemcpyAsync(s->tri_alb_rr.p, s->tri_alb.p, bytes, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->P.tri_alb_rr = (const float*)s->tri_alb_rr.p;
+  return PRT_OK;
+}
+
 int prt_set_emission(prt_scene* s, int entity, const double emission[3]) {
-  (void)s; (void)entity; (void)emission;
-  return fail(PRT_E_UNSUPPORTED, "not built yet");
+  if (!s || !emission || entity < 0 || entity >= (int)s->h_ents.size()) return fail(PRT_E_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(s->device));
+  DevEntity& e = s->h_ents[entity];
+  DevMaterial& m = s->h_mats[e.material];
+  std::memcpy(m.emission, emission, sizeof(m.emission));
+  const int was_light = m.is_light;
+  m.is_light = (m.type == PRT_REFLECTION_DIFFUSE && length(from(m.color)) < 0.001 && length(from(m.emission)) > 0.001);
+  if (m.is_light != was_light) return fail(PRT_E_UNSUPPORTED, "changing whether an entity is a light");
+  // the light list order follows power (light.h:99)
+  sort_lights(s);
+  int rc;
+  if ((rc = dev_upload(s->mats, s->h_mats, s->stream))) return rc;
+  if ((rc = dev_upload(s->lights, s->h_lights, s->stream))) return rc;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return PRT_OK;
 }
-int prt_adjoint_device(prt_scene* s, const prt_render_opts* opts, const float* d_dimage, int entity,
-                       float* d_albedo_grad, int light, double* d_emission_grad, prt_stats* stats) {
-  (void)s; (void)opts; (void)d_dimage; (void)entity; (void)d_albedo_grad; (void)light; (void)d_emission_grad;
-  (void)stats;
-  return fail(PRT_E_UNSUPPORTED, "not built yet");
+
+// Path-replay adjoint: per batch, a primal pass records every sample's
+// radiance; a replay with the same random numbers re-traces the paths and
+// scatters d(loss)/d(albedo) and d(loss)/d(emission).  Memory is O(samples
+// of a batch); nothing per path vertex is stored.
+int prt_adjoint_device(prt_scene* s, const prt_render_opts* o, const float* d_dimage, int entity,
+                       float* d_albedo_grad, int light, double* emission_grad, prt_stats* stats) {
+  if (!s || !d_dimage) return fail(PRT_E_INVALID, "bad arguments");
+  if (s->has_glass)
+    return fail(PRT_E_UNSUPPORTED, "the adjoint needs one path per sample (no glass splits)");
+  if (s->P.cutoff > 0.0f) return fail(PRT_E_UNSUPPORTED, "the adjoint needs luminance_cutoff <= 0");
+  const bool want_alb = d_albedo_grad != nullptr;
+  if (want_alb) {
+    if (entity < 0 || entity >= (int)s->h_ents.size() || s->h_ents[entity].type != PRT_ENTITY_POLYGON ||
+        !s->h_ents[entity].has_albedo)
+      return fail(PRT_E_INVALID, "albedo gradient needs a POLYGON entity created with per-face albedo");
+  }
+  if (light >= (int)s->h_ents.size()) return fail(PRT_E_INVALID, "bad light entity");
+  HIPCHK(hipSetDevice(s->device));
+  RenderSetup R;
+  int rc;
+  if ((rc = setup_render(s, o, R))) return rc;
+  if ((rc = ensure_workspace(s, R.bp * R.P.spp, true))) return rc;
+  hipStream_t st = s->stream;
+  const int64_t nf = want_alb ? s->ent_n_tris[entity] : 0;
+  if ((rc = dev_alloc(s->grad_buf, (size_t)(3 * nf + 3) * sizeof(double)))) return rc;
+  double* g_alb = (double*)s->grad_buf.p;
+  double* g_e = g_alb + 3 * nf;
+  HIPCHK(launch_fill_u64((uint64_t*)g_alb, 0ull, 3 * nf + 3, st));
+  HIPCHK(launch_fill_u64((uint64_t*)s->ws.ctr.p, 0ull, CTR_COUNT, st));
+  Timer tm;
+  if ((rc = tm.begin(o && o->timing != 0, st))) return rc;
+  CallTotals tot;
+  Adjoint none{};
+  Adjoint A{};
+  A.on = 1;
+  A.entity = want_alb ? entity : -1;
+  A.light = light;
+  A.dimage = d_dimage;
+  A.ltot = (const uint64_t*)s->ws.accum2.p;
+  A.grad_alb = g_alb;
+  A.grad_e = g_e;
+  A.scale = R.P.sensor_response * R.P.smp;
+  for (int64_t pb = 0; pb < R.n_pix; pb += R.bp) {
+    Batch B{pb, std::min(R.bp, R.n_pix - pb)};
+    if ((rc = run_batch(s, R.P, B, (uint64_t*)s->ws.accum2.p, none, tm, tot))) return rc;
+    if ((rc = run_batch(s, R.P, B, (uint64_t*)s->ws.accum.p, A, tm, tot))) return rc;
+    HIPCHK(launch_count_mismatch((const uint64_t*)s->ws.accum.p, (const uint64_t*)s->ws.accum2.p,
+                                 3 * B.n_pixels * R.P.spp, (uint64_t*)s->ws.ctr.p, st));
+  }
+  if (want_alb) HIPCHK(launch_grad_to_f32(g_alb, d_albedo_grad, 3 * nf, st));
+  double ge[3] = {0, 0, 0};
+  HIPCHK(hipMemcpyAsync(ge, g_e, sizeof(ge), hipMemcpyDeviceToHost, st));
+  uint64_t c[CTR_COUNT];
+  if ((rc = finish_stats(s, R, tm, tot, stats, c))) return rc;
+  if (c[CTR_REPLAY_MISMATCH] != 0)
+    return fail(PRT_E_INVALID, "adjoint replay diverged from the primal pass on " +
+                                   std::to_string(c[CTR_REPLAY_MISMATCH]) + " sample channels");
+  if (emission_grad) std::memcpy(emission_grad, ge, sizeof(ge));
+  return PRT_OK;
 }
 
 }  // extern "C"

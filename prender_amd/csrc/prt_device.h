@@ -73,6 +73,7 @@ struct Params {
   const float* tris;         // read as float4 on the device
   const float* tri_nrm;
   const float* tri_alb;
+  const float* tri_alb_rr;   // albedo for Russian-roulette probabilities (null: tri_alb)
   const DevEntity* ents;
   const DevMaterial* mats;
   const int32_t* analytic;   // entity indices of spheres / uvplanes

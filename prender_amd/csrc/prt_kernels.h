@@ -16,6 +16,7 @@ enum {
   CTR_DROP = 6,    // appends beyond a queue's capacity (must stay 0)
   CTR_SNODES = 8,  // BVH nodes fetched by shadow rays
   CTR_STRIS = 9,   // triangle tests of shadow rays
+  CTR_REPLAY_MISMATCH = 10,  // adjoint: samples whose replayed radiance differs from the primal
   CTR_COUNT = 12
 };
 
@@ -50,7 +51,25 @@ struct ShadowQ {
   double *ox, *oy, *oz, *dx, *dy, *dz, *tmax;
   float *cr, *cg, *cb;
   uint32_t* slot;
+  int32_t* light;        // light entity sampled
   uint32_t cap;
+};
+
+// Path-replay adjoint (inactive when on == 0).  The replay re-traces the
+// primal paths with the same random numbers; at a vertex of `entity` the
+// derivative of the sample radiance with respect to the face albedo is the
+// radiance the path still gathers from there on, minus the vertex's own
+// emission, divided by the albedo.
+struct Adjoint {
+  int32_t on;
+  int32_t entity;             // POLYGON entity whose per-face albedo is differentiated
+  int32_t light;              // light entity whose emission is differentiated
+  int32_t pad_;
+  const float* dimage;        // dLoss/dimage, W*H*3 in image row order
+  const uint64_t* ltot;       // primal sample radiance (fixed point)
+  double* grad_alb;           // n_faces*3 of `entity`
+  double* grad_e;             // 3
+  double scale;               // d image / d sample radiance = sensor_response * smp
 };
 
 // A batch: pixels [pixel_base, pixel_base + n_pixels) of the y*W + x order,
@@ -67,12 +86,18 @@ hipError_t launch_extend(const Params& P, const RayQ& q, const uint32_t* count, 
                          double* hit_t, int32_t* hit_id, uint64_t* ctr, int max_blocks, hipStream_t s);
 hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const uint32_t* count, int64_t n_max,
                       const ShadowQ& sq, uint32_t* count_sq, uint64_t* ctr, hipStream_t s);
-hipError_t launch_shadow(const Params& P, const ShadowQ& q, const uint32_t* count, int64_t n_max, uint32_t* fetch,
-                         float* accum, uint64_t* ctr, int max_blocks, hipStream_t s);
+hipError_t launch_shadow(const Params& P, const Batch& B, const ShadowQ& q, const uint32_t* count, int64_t n_max,
+                         uint32_t* fetch, uint64_t* accum, const Adjoint& A, uint64_t* ctr, int max_blocks,
+                         hipStream_t s);
 hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const uint32_t* count_in,
                         int64_t n_host, const double* hit_t, const int32_t* hit_id, const RayQ& qout,
-                        uint32_t* count_out, float* accum, uint64_t* ctr, hipStream_t s);
-hipError_t launch_resolve(const Params& P, const Batch& B, const float* accum, float* rgb, float* var,
+                        uint32_t* count_out, uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s);
+hipError_t launch_resolve(const Params& P, const Batch& B, const uint64_t* accum, void* rgb, void* var, int f64,
                           hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s);
+hipError_t launch_scatter_albedo(float* tri_alb, const uint32_t* slot_of_tri, int64_t first, int64_t n_faces,
+                                 const float* albedo, hipStream_t s);
+hipError_t launch_grad_to_f32(const double* g, float* out, int64_t n, hipStream_t s);
+hipError_t launch_count_mismatch(const uint64_t* a, const uint64_t* b, int64_t n, uint64_t* ctr, hipStream_t s);

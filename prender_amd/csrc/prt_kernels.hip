@@ -90,6 +90,13 @@ __global__ __launch_bounds__(256) void k_gen_primary(Params P, Batch B, RayQ q) 
   }
 }
 
+// Image element (render.h:1234: row-flipped) of batch-local camera sample slot.
+__device__ __forceinline__ size_t sample_image_index(const Params& P, const Batch& B, uint32_t slot) {
+  const int64_t pix = B.pixel_base + slot / P.spp;
+  const int x = (int)(pix % P.W), y = (int)(pix / P.W);
+  return (size_t)(P.H - y - 1) * P.W + x;
+}
+
 // ------------------------------------------------------------ extend / shadow
 // Closest hit for every ray of a queue; each wave fetches 64 rays at a time.
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_extend(Params P, RayQ q, const uint32_t* __restrict__ count_ptr,
@@ -117,11 +124,13 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_exte
 
 // Any hit for NEE shadow rays; an unoccluded ray adds its contribution to the
 // camera sample's accumulator.
-__global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_shadow(Params P, ShadowQ q, const uint32_t* __restrict__ count_ptr,
-                                                uint32_t* fetch, float* __restrict__ accum, uint64_t* ctr) {
+__global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_shadow(
+    Params P, Batch B, ShadowQ q, const uint32_t* __restrict__ count_ptr, uint32_t* fetch,
+    uint64_t* __restrict__ accum, Adjoint A, uint64_t* ctr) {
   PRT_DECLARE_STACK(S, P);
   const uint32_t n = *count_ptr;
   TravStats ts{0, 0};
+  double ge[3] = {0.0, 0.0, 0.0};
   while (true) {
     uint32_t base = 0;
     if (lane_id() == 0) base = atomicAdd(fetch, 64u);
@@ -131,13 +140,27 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_shad
     if (i < n) {
       if (!occluded(P, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), q.tmax[i], S, ts)) {
         const uint32_t s = q.slot[i];
-        atomicAdd(&accum[3 * (size_t)s + 0], q.cr[i]);
-        atomicAdd(&accum[3 * (size_t)s + 1], q.cg[i]);
-        atomicAdd(&accum[3 * (size_t)s + 2], q.cb[i]);
+        const float c[3] = {q.cr[i], q.cg[i], q.cb[i]};
+        accum_add(&accum[3 * (size_t)s + 0], c[0]);
+        accum_add(&accum[3 * (size_t)s + 1], c[1]);
+        accum_add(&accum[3 * (size_t)s + 2], c[2]);
+        if (A.on && q.light[i] == A.light) {
+          // d(contribution)/d(emission) = contribution / emission
+          const double* E = P.mats[P.ents[A.light].material].emission;
+          const size_t px = sample_image_index(P, B, s);
+          for (int k = 0; k < 3; ++k)
+            if (E[k] != 0.0) ge[k] += (double)A.dimage[3 * px + k] * A.scale * (double)c[k] / E[k];
+        }
       }
     }
   }
   flush_stats(ctr, CTR_SNODES, ts);
+  if (A.on) {
+    for (int k = 0; k < 3; ++k) {
+      const double w = wave_sum_f64(ge[k]);
+      if (lane_id() == 0 && w != 0.0) atomicAdd(&A.grad_e[k], w);
+    }
+  }
 }
 
 // ------------------------------------------------------------ shade
@@ -146,6 +169,17 @@ struct ShadeCtx {
   uint32_t* count_out;
   uint32_t slot;
   uint32_t truncated, n_ext, pickups, dropped;
+};
+
+// What the adjoint needs to know about one shaded vertex.
+struct VertexAdj {
+  int32_t entity;  // hit entity (-1: miss)
+  int32_t face;    // face of a POLYGON hit
+  bool albedo;     // the continuation and NEE of this vertex scale with its colour
+  d3 rho;          // that colour
+  d3 own;          // radiance added at this vertex that does not scale with it (T*(Env+E))
+  d3 pre;          // pickup added for the parent vertex (belongs to the radiance before this vertex)
+  d3 add_e;        // radiance added here that is linear in the hit entity's emission, / emission
 };
 
 __device__ __forceinline__ void emit_child(ShadeCtx& C, d3 o, d3 d, d3 ct, d3 pw, uint32_t meta, d3 nrm) {
@@ -170,8 +204,14 @@ __device__ __forceinline__ void emit_child(ShadeCtx& C, d3 o, d3 d, d3 ct, d3 pw
 // Returns the radiance added to the camera sample; child rays go to the
 // output queue, and a diffuse vertex marks its child for k_nee.
 __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d3 d, double t, int32_t id, d3 T,
-                                           d3 Pw, uint32_t pixel, uint32_t sample, int depth, uint32_t branch) {
+                                           d3 Pw, uint32_t pixel, uint32_t sample, int depth, uint32_t branch,
+                                           VertexAdj& va) {
   d3 add = mk(0.0, 0.0, 0.0);
+  const d3 zero = mk(0.0, 0.0, 0.0);
+  va.entity = -1;
+  va.face = -1;
+  va.albedo = false;
+  va.rho = va.own = va.pre = va.add_e = zero;
   if (depth >= P.max_vertices) {  // safety cap (the reference has no hard cap)
     C.truncated++;
     return add;
@@ -198,6 +238,8 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
     }
   }
   const DevEntity& e = P.ents[ei];
+  va.entity = ei;
+  if (id >= 0) va.face = __float_as_int(reinterpret_cast<const float4*>(P.tris)[(size_t)id * 3 + 1].w);
   if (e.ninv < 0) n = n * -1.0;
   const DevMaterial& m = P.mats[e.material];
   d3 color;
@@ -209,11 +251,16 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
   // pickup of the parent's mirror / glass vertex (directLight_radiance.h:516-556)
   if (e.light && (Pw.x != 0.0 || Pw.y != 0.0 || Pw.z != 0.0)) {
     add = add + vmul(Pw, E);
+    va.pre = vmul(Pw, E);
+    va.add_e = Pw;
     C.pickups++;
   }
   const d3 n_orient = dot(n, d) < 0.0 ? n : (-1.0 * n);
   // RGB_MAX = std::max(x, std::max(y, z)) (def.h:19); RR (radiance.h:77-126)
-  const d3 rw = E + color;
+  d3 color_rr = color;
+  if (P.tri_alb_rr && e.has_albedo && id >= 0)
+    color_rr = mk(P.tri_alb_rr[3 * (size_t)id], P.tri_alb_rr[3 * (size_t)id + 1], P.tri_alb_rr[3 * (size_t)id + 2]);
+  const d3 rw = E + color_rr;
   const double myz = (rw.y < rw.z) ? rw.z : rw.y;
   double rr = (rw.x < myz) ? myz : rw.x;
   if (depth > P.depth_limit) rr *= ldexp(1.0, -(depth - P.depth_limit));  // pow(0.5, k), exact
@@ -221,7 +268,11 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
   const uint32_t c2 = rng_c2(depth, branch);
   philox(pixel, sample, c2, 0u, P.key0, P.key1, b0);
   if (depth > P.depth) {
-    if (u01(b0[0]) >= rr) return add + vmul(T, E);  // terminate with the emission
+    if (u01(b0[0]) >= rr) {  // terminate with the emission
+      va.own = vmul(T, E);
+      va.add_e = va.add_e + T;
+      return add + vmul(T, E);
+    }
   } else {
     rr = 1.0;
   }
@@ -231,15 +282,22 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
   else kind = 2;
   const uint32_t meta_same = rng_c2(depth + 1, branch);
   const double pk = P.nee ? 1.0 : 0.0;
-  const d3 zero = mk(0.0, 0.0, 0.0);
   if (kind == 0) {
     // BRDF_Diffuse, brdf.h:408-700
     if (e.light && e.type == ENT_UVPLANE && dot(n, d) >= 0.0) return add;
     if (e.light) {
-      if (!P.nee || depth == 0) add = add + vmul(T, E / rr / (1.0 - 0.0));
+      if (!P.nee || depth == 0) {
+        add = add + vmul(T, E / rr / (1.0 - 0.0));
+        va.own = vmul(T, E / rr);
+        va.add_e = va.add_e + T / rr;
+      }
       return add;
     }
     add = add + vmul(T, ld3(P.env_light) + E);
+    va.own = vmul(T, ld3(P.env_light) + E);
+    va.add_e = va.add_e + T;
+    va.albedo = true;
+    va.rho = color;
     // OrthonormalBasis (vector3d.h:244-253), cosine sampling (brdf.h:498-505)
     const d3 w = n_orient;
     const d3 uu = (fabs(w.x) > PS_EPS) ? normalize(cross(mk(0.0, 1.0, 0.0), w)) : normalize(cross(mk(1.0, 0.0, 0.0), w));
@@ -255,6 +313,10 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
     return add;
   }
   add = add + vmul(T, ld3(P.env_light) + E);
+  va.own = vmul(T, ld3(P.env_light) + E);
+  va.add_e = va.add_e + T;
+  va.albedo = true;
+  va.rho = color;
   if (kind == 1) {
     // BRDF_Specular, brdf.h:702-761
     const d3 dir = normalize(d - n * 2.0 * dot(n, d));
@@ -302,10 +364,10 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
 
 // Shade one queue (or, with primary = 1, every camera sample of the batch
 // against the hit of its sub-pixel's primary ray).
-__global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(Params P, Batch B, int primary, RayQ qin,
-                                               const uint32_t* __restrict__ count_in, const double* __restrict__ hit_t,
-                                               const int32_t* __restrict__ hit_id, RayQ qout, uint32_t* count_out,
-                                               float* __restrict__ accum, uint64_t* ctr) {
+__global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
+    Params P, Batch B, int primary, RayQ qin, const uint32_t* __restrict__ count_in, const double* __restrict__ hit_t,
+    const int32_t* __restrict__ hit_id, RayQ qout, uint32_t* count_out, uint64_t* __restrict__ accum, Adjoint A,
+    uint64_t* ctr) {
   const uint32_t n = primary ? (uint32_t)(B.n_pixels * P.spp) : *count_in;
   const uint32_t stride = gridDim.x * blockDim.x;
   const int ss2 = P.SS * P.SS;
@@ -313,6 +375,7 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(P
   C.qout = qout;
   C.count_out = count_out;
   C.truncated = C.n_ext = C.pickups = C.dropped = 0;
+  double ge[3] = {0.0, 0.0, 0.0};
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     d3 o, d, T, Pw;
     double t;
@@ -346,11 +409,38 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(P
     }
     C.slot = slot;
     const uint32_t pl = slot / P.spp, k = slot % P.spp;
-    const d3 add = shade_vertex(P, C, o, d, t, id, T, Pw, (uint32_t)(B.pixel_base + pl), k, depth, branch);
-    if (add.x != 0.0 || add.y != 0.0 || add.z != 0.0) {
-      atomicAdd(&accum[3 * (size_t)slot + 0], (float)add.x);
-      atomicAdd(&accum[3 * (size_t)slot + 1], (float)add.y);
-      atomicAdd(&accum[3 * (size_t)slot + 2], (float)add.z);
+    VertexAdj va;
+    const d3 add = shade_vertex(P, C, o, d, t, id, T, Pw, (uint32_t)(B.pixel_base + pl), k, depth, branch, va);
+    if (A.on) {
+      const size_t px = sample_image_index(P, B, slot);
+      const d3 g = mk(A.dimage[3 * px] * A.scale, A.dimage[3 * px + 1] * A.scale, A.dimage[3 * px + 2] * A.scale);
+      if (va.albedo && va.entity == A.entity && va.face >= 0) {
+        // radiance still to come from this vertex on (no glass splits: one
+        // path per sample), without the vertex's own emission
+        const uint64_t* lt = A.ltot + 3 * (size_t)slot;
+        const uint64_t* lb = accum + 3 * (size_t)slot;
+        const double rest[3] = {accum_get(lt) - accum_get(lb) - va.pre.x - va.own.x,
+                                accum_get(lt + 1) - accum_get(lb + 1) - va.pre.y - va.own.y,
+                                accum_get(lt + 2) - accum_get(lb + 2) - va.pre.z - va.own.z};
+        const double rho[3] = {va.rho.x, va.rho.y, va.rho.z};
+        const double gg[3] = {g.x, g.y, g.z};
+        for (int c = 0; c < 3; ++c)
+          if (rho[c] > 0.0 && rest[c] != 0.0) atomicAdd(&A.grad_alb[3 * (size_t)va.face + c], gg[c] * rest[c] / rho[c]);
+      }
+      if (va.entity == A.light) {
+        ge[0] += g.x * va.add_e.x;
+        ge[1] += g.y * va.add_e.y;
+        ge[2] += g.z * va.add_e.z;
+      }
+    }
+    accum_add(&accum[3 * (size_t)slot + 0], add.x);
+    accum_add(&accum[3 * (size_t)slot + 1], add.y);
+    accum_add(&accum[3 * (size_t)slot + 2], add.z);
+  }
+  if (A.on) {
+    for (int c = 0; c < 3; ++c) {
+      const double w = wave_sum_f64(ge[c]);
+      if (lane_id() == 0 && w != 0.0) atomicAdd(&A.grad_e[c], w);
     }
   }
   const uint64_t s_pick = wave_sum_u32(C.pickups), s_tr = wave_sum_u32(C.truncated);
@@ -418,7 +508,8 @@ __device__ __forceinline__ bool nee_sample(const Params& P, int li, double ra, d
   return true;
 }
 
-__device__ __forceinline__ void put_shadow(ShadowQ& q, uint32_t j, const ShadowRay& s, uint32_t slot) {
+__device__ __forceinline__ void put_shadow(ShadowQ& q, uint32_t j, const ShadowRay& s, uint32_t slot, int32_t light) {
+  q.light[j] = light;
   q.ox[j] = s.o2.x; q.oy[j] = s.o2.y; q.oz[j] = s.o2.z;
   q.dx[j] = s.dir.x; q.dy[j] = s.dir.y; q.dz[j] = s.dir.z;
   q.tmax[j] = s.tmax;
@@ -440,6 +531,7 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(Param
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
     ShadowRay s0, s1, s2;
     bool v0 = false, v1 = false, v2 = false;
+    int32_t l0 = -1, l1 = -1, l2 = -1;
     uint32_t slot = 0;
     if (i < n) {
       const uint32_t meta = q.meta[i];
@@ -465,9 +557,12 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(Param
         i0 = i0 >= NL ? NL - 1 : i0;
         i1 = i1 >= NL ? NL - 1 : i1;
         i2 = i2 >= NL ? NL - 1 : i2;
-        v0 = nee_sample(P, P.lights[i0], u01(b1[1]), u01(b1[2]), x, nrm, ct, s0);
-        v1 = nee_sample(P, P.lights[i1], u01(b2[0]), u01(b2[1]), x, nrm, ct, s1);
-        v2 = nee_sample(P, P.lights[i2], u01(b2[3]), u01(b3[0]), x, nrm, ct, s2);
+        l0 = P.lights[i0];
+        l1 = P.lights[i1];
+        l2 = P.lights[i2];
+        v0 = nee_sample(P, l0, u01(b1[1]), u01(b1[2]), x, nrm, ct, s0);
+        v1 = nee_sample(P, l1, u01(b2[0]), u01(b2[1]), x, nrm, ct, s1);
+        v2 = nee_sample(P, l2, u01(b2[3]), u01(b3[0]), x, nrm, ct, s2);
       }
     }
     const uint32_t cnt = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
@@ -476,9 +571,9 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(Param
       dropped += cnt;
       continue;
     }
-    if (v0) put_shadow(sq, j++, s0, slot);
-    if (v1) put_shadow(sq, j++, s1, slot);
-    if (v2) put_shadow(sq, j++, s2, slot);
+    if (v0) put_shadow(sq, j++, s0, slot, l0);
+    if (v1) put_shadow(sq, j++, s1, slot, l1);
+    if (v2) put_shadow(sq, j++, s2, slot, l2);
     emitted += cnt;
   }
   const uint64_t s_sh = wave_sum_u32(emitted), s_drop = wave_sum_u32(dropped);
@@ -497,17 +592,18 @@ __device__ __forceinline__ double luminance(double r, double g, double b) {
 // Per pixel of the batch: v_k = L_k * smp, luminance clamp (render.h:1375-1384),
 // image += v_k; then *= sensor_response (render.h:1405-1409) and the variance
 // of the pixel mean.
-__global__ __launch_bounds__(256) void k_resolve(Params P, Batch B, const float* __restrict__ accum,
-                                                 float* __restrict__ out_rgb, float* __restrict__ out_var) {
+// With f64 set, out_rgb / out_var point to doubles.
+__global__ __launch_bounds__(256) void k_resolve(Params P, Batch B, const uint64_t* __restrict__ accum,
+                                                 void* __restrict__ out_rgb, void* __restrict__ out_var, int f64) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= B.n_pixels) return;
   const int64_t pix = B.pixel_base + i;
   const int x = (int)(pix % P.W), y = (int)(pix / P.W);
   const size_t image_index = (size_t)(P.H - y - 1) * P.W + x;
   double s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
-  const float* a = accum + 3 * (size_t)i * P.spp;
+  const uint64_t* a = accum + 3 * (size_t)i * P.spp;
   for (int k = 0; k < P.spp; ++k) {
-    double v0 = (double)a[3 * k] * P.smp, v1 = (double)a[3 * k + 1] * P.smp, v2 = (double)a[3 * k + 2] * P.smp;
+    double v0 = accum_get(a + 3 * k) * P.smp, v1 = accum_get(a + 3 * k + 1) * P.smp, v2 = accum_get(a + 3 * k + 2) * P.smp;
     if (P.cutoff > 0.0f) {
       const float lv = (float)luminance(v0, v1, v2);
       if (lv > P.cutoff) {
@@ -521,10 +617,15 @@ __global__ __launch_bounds__(256) void k_resolve(Params P, Batch B, const float*
   const double sr = P.sensor_response;
   const double nn = (double)P.spp;
   for (int c = 0; c < 3; ++c) {
-    out_rgb[image_index * 3 + c] = (float)(s1[c] * sr);
-    if (out_var)
-      out_var[image_index * 3 + c] =
-          (float)(nn > 1 ? fmax(0.0, (nn * s2[c] - s1[c] * s1[c]) / (nn - 1)) * sr * sr : 0.0);
+    const double rgb = s1[c] * sr;
+    const double var = nn > 1 ? fmax(0.0, (nn * s2[c] - s1[c] * s1[c]) / (nn - 1)) * sr * sr : 0.0;
+    if (f64) {
+      static_cast<double*>(out_rgb)[image_index * 3 + c] = rgb;
+      if (out_var) static_cast<double*>(out_var)[image_index * 3 + c] = var;
+    } else {
+      static_cast<float*>(out_rgb)[image_index * 3 + c] = (float)rgb;
+      if (out_var) static_cast<float*>(out_var)[image_index * 3 + c] = (float)var;
+    }
   }
 }
 
@@ -533,6 +634,32 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, int64_t n) {
 }
 __global__ void k_fill_f32(float* p, float v, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+// Per-face albedo of one POLYGON entity -> the leaf-ordered triangle records.
+__global__ void k_scatter_albedo(float* tri_alb, const uint32_t* slot_of_tri, int64_t first, int64_t n_faces,
+                                 const float* albedo) {
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < n_faces; f += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = slot_of_tri[first + f];
+    tri_alb[3 * (size_t)s + 0] = albedo[3 * f + 0];
+    tri_alb[3 * (size_t)s + 1] = albedo[3 * f + 1];
+    tri_alb[3 * (size_t)s + 2] = albedo[3 * f + 2];
+  }
+}
+__global__ void k_grad_to_f32(const double* g, float* out, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (float)g[i];
+}
+// The replay must rebuild the primal sample radiance bit for bit.
+__global__ void k_count_mismatch(const uint64_t* a, const uint64_t* b, int64_t n, uint64_t* ctr) {
+  uint32_t bad = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    bad += a[i] != b[i];
+  const uint64_t w = wave_sum_u32(bad);
+  if (lane_id() == 0 && w) atomicAdd((unsigned long long*)&ctr[CTR_REPLAY_MISMATCH], (unsigned long long)w);
 }
 
 // ------------------------------------------------------------ launchers
@@ -558,17 +685,18 @@ hipError_t launch_extend(const Params& P, const RayQ& q, const uint32_t* count, 
                      hit_id, ctr);
   return hipGetLastError();
 }
-hipError_t launch_shadow(const Params& P, const ShadowQ& q, const uint32_t* count, int64_t n_max, uint32_t* fetch,
-                         float* accum, uint64_t* ctr, int max_blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid_for(n_max, 256, max_blocks)), dim3(256), 0, s, P, q, count, fetch, accum,
-                     ctr);
+hipError_t launch_shadow(const Params& P, const Batch& B, const ShadowQ& q, const uint32_t* count, int64_t n_max,
+                         uint32_t* fetch, uint64_t* accum, const Adjoint& A, uint64_t* ctr, int max_blocks,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_shadow, dim3(grid_for(n_max, 256, max_blocks)), dim3(256), 0, s, P, B, q, count, fetch, accum,
+                     A, ctr);
   return hipGetLastError();
 }
 hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const uint32_t* count_in,
                         int64_t n_host, const double* hit_t, const int32_t* hit_id, const RayQ& qout,
-                        uint32_t* count_out, float* accum, uint64_t* ctr, hipStream_t s) {
+                        uint32_t* count_out, uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s) {
   hipLaunchKernelGGL(k_shade, dim3(grid_for(n_host)), dim3(256), 0, s, P, B, primary, qin, count_in, hit_t, hit_id,
-                     qout, count_out, accum, ctr);
+                     qout, count_out, accum, A, ctr);
   return hipGetLastError();
 }
 hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const uint32_t* count, int64_t n_max,
@@ -576,9 +704,9 @@ hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const uint
   hipLaunchKernelGGL(k_nee, dim3(grid_for(n_max, 256, 8192)), dim3(256), 0, s, P, B, q, count, sq, count_sq, ctr);
   return hipGetLastError();
 }
-hipError_t launch_resolve(const Params& P, const Batch& B, const float* accum, float* rgb, float* var,
+hipError_t launch_resolve(const Params& P, const Batch& B, const uint64_t* accum, void* rgb, void* var, int f64,
                           hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve, dim3(grid_for(B.n_pixels)), dim3(256), 0, s, P, B, accum, rgb, var);
+  hipLaunchKernelGGL(k_resolve, dim3(grid_for(B.n_pixels)), dim3(256), 0, s, P, B, accum, rgb, var, f64);
   return hipGetLastError();
 }
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
@@ -587,5 +715,23 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s) {
 }
 hipError_t launch_fill_f32(float* p, float v, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_fill_u64(uint64_t* p, uint64_t v, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, p, v, n);
+  return hipGetLastError();
+}
+hipError_t launch_scatter_albedo(float* tri_alb, const uint32_t* slot_of_tri, int64_t first, int64_t n_faces,
+                                 const float* albedo, hipStream_t s) {
+  hipLaunchKernelGGL(k_scatter_albedo, dim3(grid_for(n_faces, 256, 4096)), dim3(256), 0, s, tri_alb, slot_of_tri,
+                     first, n_faces, albedo);
+  return hipGetLastError();
+}
+hipError_t launch_grad_to_f32(const double* g, float* out, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_grad_to_f32, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, g, out, n);
+  return hipGetLastError();
+}
+hipError_t launch_count_mismatch(const uint64_t* a, const uint64_t* b, int64_t n, uint64_t* ctr, hipStream_t s) {
+  hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, a, b, n, ctr);
   return hipGetLastError();
 }

@@ -135,8 +135,27 @@ __device__ __forceinline__ bool analytic_t(const DevEntity& e, d3 o, d3 d, doubl
   return e.type == ENT_SPHERE ? sphere_t(e, o, d, t) : uvplane_t(e, o, d, t);
 }
 
+// ------------------------------------------------------------ sample accumulator
+// Per-sample radiance is summed in 32.32 fixed point with 64-bit integer
+// atomics: the sum is exact and independent of the order in which the
+// shade and shadow kernels add to a sample, so renders are bit-reproducible
+// (which the finite-difference gradient checks rely on).
+#define PRT_FIX_ONE 4294967296.0
+#define PRT_FIX_INV 2.3283064365386963e-10
+
+__device__ __forceinline__ void accum_add(uint64_t* a, double v) {
+  if (v > 0.0) atomicAdd((unsigned long long*)a, (unsigned long long)(v * PRT_FIX_ONE + 0.5));
+}
+__device__ __forceinline__ double accum_get(const uint64_t* a) { return (double)(*a) * PRT_FIX_INV; }
+
 // ------------------------------------------------------------ wave helpers
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ double wave_sum_f64(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
 
 __device__ __forceinline__ uint64_t wave_sum_u32(uint32_t x) {
   uint32_t s = x;

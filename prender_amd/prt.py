@@ -36,6 +36,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise PrtError(f"{LIB_PATH} is missing: build it with `python -m prender_amd.build`")
+    # libprt and PyTorch both need libamdhip64.so.7, and the first one loaded
+    # serves the whole process; load torch's first (when torch is installed)
+    # so that tensors and libprt share one HIP runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.prt_abi_version.restype = C.c_int
     L.prt_last_error.restype = C.c_char_p
@@ -53,6 +60,7 @@ def lib():
                                           C.POINTER(C.c_void_p), C.POINTER(PrtStats)]
     L.prt_set_face_albedo_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     L.prt_set_emission.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+    L.prt_hold_rr_albedo.argtypes = [C.c_void_p, C.c_int]
     L.prt_adjoint_device.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_int, C.c_void_p,
                                      C.c_int, C.POINTER(C.c_double), C.POINTER(PrtStats)]
     _lib = L
@@ -61,7 +69,8 @@ def lib():
 
 EXPORTED = ["prt_abi_version", "prt_last_error", "prt_device_info", "prt_scene_create", "prt_scene_destroy",
             "prt_scene_info", "prt_trace_closest", "prt_trace_closest_device", "prt_render", "prt_render_device",
-            "prt_render_batch_device", "prt_set_face_albedo_device", "prt_set_emission", "prt_adjoint_device"]
+            "prt_render_batch_device", "prt_set_face_albedo_device", "prt_set_emission", "prt_hold_rr_albedo",
+            "prt_adjoint_device"]
 
 
 def check(rc: int, what: str):
@@ -77,7 +86,7 @@ def device_info():
 
 
 def render_opts(seed: int = 0, samples: Optional[int] = None, batch_pixels: int = 0, max_depth: int = 0,
-                use_graph: bool = False, timing: bool = False) -> PrtRenderOpts:
+                use_graph: bool = False, timing: bool = False, image_f64: bool = False) -> PrtRenderOpts:
     o = PrtRenderOpts()
     o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
     o.samples_override = int(samples or 0)
@@ -85,6 +94,7 @@ def render_opts(seed: int = 0, samples: Optional[int] = None, batch_pixels: int 
     o.max_depth = int(max_depth)
     o.use_graph = 1 if use_graph else 0
     o.timing = 1 if timing else 0
+    o.image_f64 = 1 if image_f64 else 0
     return o
 
 
@@ -159,3 +169,28 @@ class Scene:
                                       C.c_void_p(var_ptr) if var_ptr else None, C.byref(st)),
               "prt_render_device")
         return st.as_dict()
+
+    # ---- differentiable parameters (per-face albedo, light emission)
+    def set_face_albedo_device(self, entity: int, albedo_ptr: int):
+        check(lib().prt_set_face_albedo_device(self.h, entity, C.c_void_p(albedo_ptr)), "prt_set_face_albedo_device")
+
+    def set_emission(self, entity: int, emission):
+        e = (C.c_double * 3)(*[float(x) for x in emission])
+        check(lib().prt_set_emission(self.h, entity, e), "prt_set_emission")
+
+    def hold_rr_albedo(self, hold: bool = True):
+        """Freeze (or release) the albedo that Russian-roulette probabilities use."""
+        check(lib().prt_hold_rr_albedo(self.h, 1 if hold else 0), "prt_hold_rr_albedo")
+
+    def adjoint_device(self, dimage_ptr: int, entity: int, grad_albedo_ptr: Optional[int], light: int,
+                       seed: int = 0, samples: Optional[int] = None, **kw):
+        """Path-replay adjoint: writes d loss/d albedo (n_faces*3 floats) of
+        `entity` to grad_albedo_ptr and returns (d loss/d emission [3] of
+        `light`, stats) for the adjoint image at dimage_ptr (W*H*3 floats)."""
+        st = PrtStats()
+        ge = (C.c_double * 3)()
+        o = render_opts(seed, samples, **kw)
+        check(lib().prt_adjoint_device(self.h, C.byref(o), C.c_void_p(dimage_ptr), entity,
+                                       C.c_void_p(grad_albedo_ptr) if grad_albedo_ptr else None, light, ge,
+                                       C.byref(st)), "prt_adjoint_device")
+        return np.array(ge[:]), st.as_dict()
