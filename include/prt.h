@@ -215,18 +215,41 @@ int prt_render(prt_scene* scene, const prt_render_opts* opts,
 int prt_render_device(prt_scene* scene, const prt_render_opts* opts,
                       float* d_rgb, float* d_var, prt_stats* stats);
 
-/* ---- batched persistent rendering of many scenes ------------------------- */
+/* ---- batched rendering of many scenes ------------------------------------- */
+/* No reference counterpart: the reference renders one scene per process
+ * (main.cpp:173-200).  All scenes of a batch go through one shared work queue
+ * of pixel tiles (a tile = consecutive pixels of one scene with all their
+ * samples); a fixed pool of tile slots keeps rays of many tiles, of whatever
+ * scenes, in flight, and two iterations of the wavefront are replayed as one
+ * captured hipGraph until the queue is drained.  Each scene's image is
+ * bit-identical to prt_render_device of that scene with the same seed. */
 typedef struct prt_batch_opts {
     uint64_t seed;
-    int32_t tile_pixels;     /* pixels per work item (0 = autotuned) */
-    int32_t waves_per_cu;    /* persistent grid size (0 = autotuned) */
-    int32_t queue_rays;      /* in-flight ray capacity (0 = autotuned) */
-    int32_t use_graph;
+    int32_t tile_samples;       /* camera samples per tile, rounded up to a power of two
+                                   and to the largest spp (0 = 4096) */
+    int32_t inflight_tiles;     /* tile slots (0 = 2^21 samples in flight) */
+    int32_t stack_lds;          /* LDS traversal stack entries: 8, 16 or 32 (0 = 16) */
+    int32_t trav_blocks_per_cu; /* persistent traversal blocks per CU, <= 16 (0 = 8) */
+    int32_t use_graph;          /* >= 0: hipGraph replay; -1: plain launches */
+    int32_t tile_stride;        /* render only every k-th tile (tuning trials; 0 = all) */
+    int32_t max_depth;          /* safety cap on path vertices (0 = 128) */
+    int32_t pad_;
 } prt_batch_opts;
-/* outs[i]: W_i*H_i*3 float device buffers (one per scene). */
+/* d_rgb[i] / d_var[i]: W_i*H_i*3 float device buffers of scene i (d_var or
+ * its entries may be NULL).  All scenes must live on one device; the call
+ * runs on scenes[0]'s stream, which also owns the batch workspace. */
 int prt_render_batch_device(prt_scene* const* scenes, int n_scenes,
-                            const prt_batch_opts* opts, float* const* d_outs,
-                            prt_stats* stats);
+                            const prt_batch_opts* opts, float* const* d_rgb,
+                            float* const* d_var, prt_stats* stats);
+/* Measures candidate in-flight sizes (2^20..2^22 samples), tile sizes
+ * (1024..16384 samples), LDS stack depths and traversal grids by coordinate
+ * descent on a subsample of the batch's tiles (every k-th tile, k from
+ * base->tile_stride or chosen for ~16M samples per trial), and writes the
+ * fastest options to *best (tile_stride copied from base).  The trial renders
+ * write the output buffers. */
+int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts* base,
+                   float* const* d_rgb, float* const* d_var, prt_batch_opts* best,
+                   double* best_ns_per_sample);
 
 /* ---- differentiable rendering (per-face albedo + light emission) --------- */
 /* Replace the per-face albedo of POLYGON entity `entity` (float n_faces*3,

@@ -20,23 +20,90 @@
 #include "../../include/prt.h"
 #include "bvh.h"
 #include "prt_device.h"
+#include "prt_internal.h"
 #include "prt_kernels.h"
 
-namespace {
+namespace prt {
 
 thread_local std::string g_err;
 
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
+int dev_alloc(DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return PRT_OK;
+  b.release();
+  if (bytes == 0) return PRT_OK;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(PRT_E_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return PRT_OK;
 }
 
-#define HIPCHK(expr)                                                                     \
-  do {                                                                                   \
-    hipError_t e_ = (expr);                                                              \
-    if (e_ != hipSuccess)                                                                \
-      return fail(PRT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
-  } while (0)
+void Workspace::release() {
+  for (auto& a : q)
+    for (auto& b : a) b.release();
+  hit_t.release();
+  hit_id.release();
+  for (auto& b : sq) b.release();
+  accum.release();
+  accum2.release();
+  counts.release();
+  fetch.release();
+  ctr.release();
+  if (h_counts) (void)hipHostFree(h_counts);
+  h_counts = nullptr;
+  batch_samples = queue_cap = shadow_cap = 0;
+}
+
+int alloc_ray_queue(DevBuf* cols, int64_t cap) {
+  static const size_t dsz[RAYQ_COLS] = {8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+  for (int k = 0; k < RAYQ_COLS; ++k) {
+    const int rc = dev_alloc(cols[k], (size_t)cap * dsz[k]);
+    if (rc) return rc;
+  }
+  return PRT_OK;
+}
+
+int alloc_shadow_queue(DevBuf* cols, int64_t cap) {
+  static const size_t ssz[SHADOWQ_COLS] = {8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4};
+  for (int k = 0; k < SHADOWQ_COLS; ++k) {
+    const int rc = dev_alloc(cols[k], (size_t)cap * ssz[k]);
+    if (rc) return rc;
+  }
+  return PRT_OK;
+}
+
+RayQ ray_queue(DevBuf* b, int64_t cap) {
+  RayQ q;
+  q.ox = (double*)b[0].p; q.oy = (double*)b[1].p; q.oz = (double*)b[2].p;
+  q.dx = (double*)b[3].p; q.dy = (double*)b[4].p; q.dz = (double*)b[5].p;
+  q.tr = (float*)b[6].p; q.tg = (float*)b[7].p; q.tb = (float*)b[8].p;
+  q.pr = (float*)b[9].p; q.pg = (float*)b[10].p; q.pb = (float*)b[11].p;
+  q.slot = (uint32_t*)b[12].p;
+  q.meta = (uint32_t*)b[13].p;
+  q.nx = (float*)b[14].p; q.ny = (float*)b[15].p; q.nz = (float*)b[16].p;
+  q.cap = (uint32_t)cap;
+  return q;
+}
+
+ShadowQ shadow_queue(DevBuf* b, int64_t cap) {
+  ShadowQ q;
+  q.ox = (double*)b[0].p; q.oy = (double*)b[1].p; q.oz = (double*)b[2].p;
+  q.dx = (double*)b[3].p; q.dy = (double*)b[4].p; q.dz = (double*)b[5].p;
+  q.tmax = (double*)b[6].p;
+  q.cr = (float*)b[7].p; q.cg = (float*)b[8].p; q.cb = (float*)b[9].p;
+  q.slot = (uint32_t*)b[10].p;
+  q.light = (int32_t*)b[11].p;
+  q.cap = (uint32_t)cap;
+  return q;
+}
+
+}  // namespace prt
+
+using namespace prt;
+
+namespace {
 
 // Host Vector3d with the reference's operation order (vector3d.h).
 struct V3 {
@@ -53,115 +120,7 @@ V3 cross(V3 a, V3 b) { return mk((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x *
 V3 from(const double* p) { return mk(p[0], p[1], p[2]); }
 void put(double* d, V3 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; }
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-};
-
-int dev_alloc(DevBuf& b, size_t bytes) {
-  if (b.bytes >= bytes && b.p) return PRT_OK;
-  b.release();
-  if (bytes == 0) return PRT_OK;
-  hipError_t e = hipMalloc(&b.p, bytes);
-  if (e != hipSuccess) {
-    b.p = nullptr;
-    return fail(PRT_E_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
-  }
-  b.bytes = bytes;
-  return PRT_OK;
-}
-
-template <class T>
-int dev_upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
-  int rc = dev_alloc(b, std::max<size_t>(v.size() * sizeof(T), 256));
-  if (rc) return rc;
-  if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  return PRT_OK;
-}
-
-// Render workspace sized for a batch of samples.
-struct Workspace {
-  int64_t batch_samples = 0;  // camera samples per batch
-  int64_t queue_cap = 0;      // rays per queue
-  int64_t shadow_cap = 0;
-  DevBuf q[2][17];            // two ray queues (SoA)
-  DevBuf hit_t, hit_id;
-  DevBuf sq[12];
-  DevBuf accum, accum2;       // fixed-point sample radiance (accum2: adjoint primal)
-  DevBuf counts;              // uint32 [4]: q0 count, q1 count, shadow count, spare
-  DevBuf fetch;               // uint32 [512] work-fetch counters, one per launch
-  DevBuf ctr;                 // uint64 [CTR_COUNT]
-  uint32_t* h_counts = nullptr;
-  void release() {
-    for (auto& a : q)
-      for (auto& b : a) b.release();
-    hit_t.release();
-    hit_id.release();
-    for (auto& b : sq) b.release();
-    accum.release();
-    accum2.release();
-    counts.release();
-    fetch.release();
-    ctr.release();
-    if (h_counts) (void)hipHostFree(h_counts);
-    h_counts = nullptr;
-    batch_samples = queue_cap = shadow_cap = 0;
-  }
-};
-
-RayQ ray_queue(DevBuf* b) {
-  RayQ q;
-  q.ox = (double*)b[0].p; q.oy = (double*)b[1].p; q.oz = (double*)b[2].p;
-  q.dx = (double*)b[3].p; q.dy = (double*)b[4].p; q.dz = (double*)b[5].p;
-  q.tr = (float*)b[6].p; q.tg = (float*)b[7].p; q.tb = (float*)b[8].p;
-  q.pr = (float*)b[9].p; q.pg = (float*)b[10].p; q.pb = (float*)b[11].p;
-  q.slot = (uint32_t*)b[12].p;
-  q.meta = (uint32_t*)b[13].p;
-  q.nx = (float*)b[14].p; q.ny = (float*)b[15].p; q.nz = (float*)b[16].p;
-  return q;
-}
-ShadowQ shadow_queue(DevBuf* b) {
-  ShadowQ q;
-  q.ox = (double*)b[0].p; q.oy = (double*)b[1].p; q.oz = (double*)b[2].p;
-  q.dx = (double*)b[3].p; q.dy = (double*)b[4].p; q.dz = (double*)b[5].p;
-  q.tmax = (double*)b[6].p;
-  q.cr = (float*)b[7].p; q.cg = (float*)b[8].p; q.cb = (float*)b[9].p;
-  q.slot = (uint32_t*)b[10].p;
-  q.light = (int32_t*)b[11].p;
-  return q;
-}
-
 }  // namespace
-
-struct prt_scene {
-  int device = 0;
-  int cu_count = 0;
-  hipStream_t stream = nullptr;
-  Params P{};  // scene + sampling part; per-call fields are patched in
-  DevBuf nodes, tris, tri_nrm, tri_alb, ents, mats, analytic, lights, ovf;
-  std::vector<DevEntity> h_ents;
-  std::vector<DevMaterial> h_mats;
-  std::vector<int32_t> h_lights;
-  std::vector<int64_t> face_offset;  // per entity: first slot index is not contiguous; kept for albedo updates
-  std::vector<uint32_t> slot_of_tri; // global input triangle -> leaf slot
-  std::vector<int64_t> ent_first_tri;// per entity: first global input triangle
-  std::vector<int64_t> ent_n_tris;
-  int64_t n_tris = 0;
-  int bvh_depth = 0;
-  double bvh_ms = 0;
-  bool has_glass = false;
-  int max_trav_blocks = 0;
-  Workspace ws;
-  DevBuf out_rgb, out_var;  // staging for host-output renders
-  DevBuf d_slot_of_tri;     // input triangle -> leaf slot (per-face albedo updates)
-  DevBuf grad_buf;          // adjoint accumulators (double)
-  DevBuf tri_alb_rr;        // held albedo for Russian-roulette probabilities
-};
 
 extern "C" {
 
@@ -190,6 +149,8 @@ void prt_scene_destroy(prt_scene* s) {
   s->nodes.release(); s->tris.release(); s->tri_nrm.release(); s->tri_alb.release();
   s->ents.release(); s->mats.release(); s->analytic.release(); s->lights.release(); s->ovf.release();
   s->ws.release();
+  release_stream_workspace(s->sws);
+  s->sws = nullptr;
   s->out_rgb.release();
   s->out_var.release();
   s->d_slot_of_tri.release();
@@ -368,7 +329,7 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   // traversal stack overflow: one area per thread of the largest grid
   s->max_trav_blocks = std::max(1, s->cu_count) * 8;
   const int64_t ovf_threads = (int64_t)s->max_trav_blocks * 256;
-  if ((rc = dev_alloc(s->ovf, (size_t)ovf_threads * 2 * (PRT_STACK_TOTAL - PRT_STACK_LDS) * sizeof(uint32_t))))
+  if ((rc = dev_alloc(s->ovf, (size_t)ovf_threads * 2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * sizeof(uint32_t))))
     return rc;
   HIPCHK(hipStreamSynchronize(st));
 
@@ -536,15 +497,11 @@ static int ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum)
     return PRT_OK;
   w.release();
   int rc;
-  const size_t dsz[17] = {8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
   for (int qi = 0; qi < 2; ++qi)
-    for (int k = 0; k < 17; ++k)
-      if ((rc = dev_alloc(w.q[qi][k], (size_t)qcap * dsz[k]))) return rc;
+    if ((rc = alloc_ray_queue(w.q[qi], qcap))) return rc;
   if ((rc = dev_alloc(w.hit_t, (size_t)qcap * 8))) return rc;
   if ((rc = dev_alloc(w.hit_id, (size_t)qcap * 4))) return rc;
-  const size_t ssz[12] = {8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4};
-  for (int k = 0; k < 12; ++k)
-    if ((rc = dev_alloc(w.sq[k], (size_t)scap * ssz[k]))) return rc;
+  if ((rc = alloc_shadow_queue(w.sq, scap))) return rc;
   if ((rc = dev_alloc(w.accum, (size_t)batch_samples * 3 * sizeof(uint64_t)))) return rc;
   if (two_accum && (rc = dev_alloc(w.accum2, (size_t)batch_samples * 3 * sizeof(uint64_t)))) return rc;
   if ((rc = dev_alloc(w.counts, 16 * sizeof(uint32_t)))) return rc;
@@ -638,10 +595,8 @@ static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* ac
   uint32_t* counts = (uint32_t*)w.counts.p;
   uint32_t* fetch = (uint32_t*)w.fetch.p;
   uint64_t* ctr = (uint64_t*)w.ctr.p;
-  RayQ q[2] = {ray_queue(w.q[0]), ray_queue(w.q[1])};
-  ShadowQ sq = shadow_queue(w.sq);
-  q[0].cap = q[1].cap = (uint32_t)w.queue_cap;
-  sq.cap = (uint32_t)w.shadow_cap;
+  RayQ q[2] = {ray_queue(w.q[0], w.queue_cap), ray_queue(w.q[1], w.queue_cap)};
+  ShadowQ sq = shadow_queue(w.sq, w.shadow_cap);
   const int tb = s->max_trav_blocks;
   const bool nee = P.nee && P.n_lights > 0;
   int rc;
@@ -773,12 +728,6 @@ int prt_render(prt_scene* s, const prt_render_opts* opts, float* out_rgb, float*
   if (out_var) HIPCHK(hipMemcpyAsync(out_var, s->out_var.p, bytes, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   return PRT_OK;
-}
-
-int prt_render_batch_device(prt_scene* const* scenes, int n_scenes, const prt_batch_opts* opts, float* const* d_outs,
-                            prt_stats* stats) {
-  (void)scenes; (void)n_scenes; (void)opts; (void)d_outs; (void)stats;
-  return fail(PRT_E_UNSUPPORTED, "batched rendering is not built yet");
 }
 
 // ---------------------------------------------------------------- differentiable parameters

@@ -1,0 +1,456 @@
+// prt_batch.cpp — batched rendering of many scenes through one shared work
+// queue of pixel tiles (prt_render_batch_device) and the measurement-driven
+// choice of its tile / in-flight / stack / grid sizes (prt_tune_batch).
+//
+// The device side is prt_stream.hip; every iteration is refill, gen, extend,
+// shade, nee, shadow, resolve on the lead scene's stream, two iterations per
+// captured hipGraph.  The host only replays the graph and, once per group of
+// replays, reads the control block to learn whether the work queue is drained.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "prt_internal.h"
+#include "prt_stream.h"
+
+namespace prt {
+
+struct StreamWorkspace {
+  int64_t n_slots = 0, tile_shift = 0, q_cap = 0, sq_cap = 0;
+  DevBuf q[2][RAYQ_COLS], hit_t, hit_id, sq[SHADOWQ_COLS];
+  DevBuf accum, ctr, ctl, slot_info, live, free_list, done_list, new_tiles;
+  DevBuf scenes, tiles, outs, ovf;
+  int64_t ovf_threads = 0;
+  StreamCtl* h_ctl = nullptr;  // pinned, two copies (double-buffered status reads)
+  void release() {
+    for (auto& a : q)
+      for (auto& b : a) b.release();
+    for (auto& b : sq) b.release();
+    DevBuf* all[] = {&hit_t, &hit_id, &accum, &ctr, &ctl, &slot_info, &live, &free_list, &done_list,
+                     &new_tiles, &scenes, &tiles, &outs, &ovf};
+    for (DevBuf* b : all) b->release();
+    if (h_ctl) (void)hipHostFree(h_ctl);
+    h_ctl = nullptr;
+    n_slots = tile_shift = q_cap = sq_cap = ovf_threads = 0;
+  }
+};
+
+void release_stream_workspace(StreamWorkspace* w) {
+  if (!w) return;
+  w->release();
+  delete w;
+}
+
+}  // namespace prt
+
+using namespace prt;
+
+namespace {
+
+// Defaults used where prt_batch_opts leaves a field at 0.
+constexpr int kDefaultTileSamples = 4096;
+constexpr int64_t kDefaultInflightSamples = (int64_t)1 << 21;
+constexpr int kDefaultStackLds = 16;
+constexpr int kDefaultTravBlocksPerCu = 8;
+constexpr int kMaxTravBlocksPerCu = 16;
+constexpr int kReplaysPerGroup = 4;  // graph replays (2 iterations each) between status reads
+constexpr int64_t kMaxIterations = 2000000;
+
+struct BatchConfig {
+  int tile_shift = 12;
+  int n_slots = 512;
+  int stack_lds = kDefaultStackLds;
+  int trav_blocks_per_cu = kDefaultTravBlocksPerCu;
+  bool use_graph = true;
+  int tile_stride = 1;
+  int max_vertices = 128;
+  uint64_t seed = 0;
+};
+
+int ceil_log2(int64_t v) {
+  int s = 0;
+  while (((int64_t)1 << s) < v) ++s;
+  return s;
+}
+
+int resolve_config(prt_scene* const* scenes, int n, const prt_batch_opts* o, BatchConfig& c) {
+  prt_batch_opts opts{};
+  if (o) opts = *o;
+  int max_spp = 1;
+  for (int i = 0; i < n; ++i) max_spp = std::max(max_spp, scenes[i]->P.spp);
+  const int64_t ts = opts.tile_samples > 0 ? opts.tile_samples : kDefaultTileSamples;
+  c.tile_shift = std::max(ceil_log2(ts), ceil_log2(max_spp));
+  if (c.tile_shift > 24) return fail(PRT_E_INVALID, "tile_samples above 2^24");
+  const int64_t T = (int64_t)1 << c.tile_shift;
+  c.n_slots = opts.inflight_tiles > 0 ? opts.inflight_tiles
+                                      : (int)std::max<int64_t>(1, kDefaultInflightSamples / T);
+  if ((int64_t)c.n_slots * T > ((int64_t)1 << 28)) return fail(PRT_E_INVALID, "more than 2^28 samples in flight");
+  c.stack_lds = opts.stack_lds > 0 ? opts.stack_lds : kDefaultStackLds;
+  if (c.stack_lds != 8 && c.stack_lds != 16 && c.stack_lds != 32)
+    return fail(PRT_E_INVALID, "stack_lds must be 8, 16 or 32");
+  c.trav_blocks_per_cu = opts.trav_blocks_per_cu > 0 ? opts.trav_blocks_per_cu : kDefaultTravBlocksPerCu;
+  if (c.trav_blocks_per_cu > kMaxTravBlocksPerCu) return fail(PRT_E_INVALID, "trav_blocks_per_cu above 16");
+  c.use_graph = opts.use_graph >= 0;  // -1 = plain launches
+  c.tile_stride = std::max(1, opts.tile_stride);
+  c.max_vertices = opts.max_depth > 0 ? std::min(opts.max_depth, 255) : 128;
+  c.seed = opts.seed;
+  return PRT_OK;
+}
+
+int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, bool glass, int n_scenes, int64_t n_tiles) {
+  if (!lead->sws) lead->sws = new (std::nothrow) StreamWorkspace();
+  if (!lead->sws) return fail(PRT_E_NOMEM, "out of host memory");
+  StreamWorkspace& w = *lead->sws;
+  const int64_t T = (int64_t)1 << c.tile_shift;
+  const int64_t inflight = (int64_t)c.n_slots * T;
+  const int64_t fan = glass ? 8 : 1;  // glass splits at depths 0..2 give up to 8 rays per sample
+  const int64_t qcap = (fan + 1) * inflight;
+  const int64_t scap = 3 * fan * inflight;
+  if (qcap >= ((int64_t)1 << 32) || scap >= ((int64_t)1 << 32)) return fail(PRT_E_INVALID, "queue capacity above 2^32");
+  int rc;
+  if (w.q_cap < qcap) {
+    for (int qi = 0; qi < 2; ++qi)
+      if ((rc = alloc_ray_queue(w.q[qi], qcap))) return rc;
+    if ((rc = dev_alloc(w.hit_t, (size_t)qcap * 8))) return rc;
+    if ((rc = dev_alloc(w.hit_id, (size_t)qcap * 4))) return rc;
+    w.q_cap = qcap;
+  }
+  if (w.sq_cap < scap) {
+    if ((rc = alloc_shadow_queue(w.sq, scap))) return rc;
+    w.sq_cap = scap;
+  }
+  if ((rc = dev_alloc(w.accum, (size_t)inflight * 3 * sizeof(uint64_t)))) return rc;
+  if ((rc = dev_alloc(w.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
+  if ((rc = dev_alloc(w.ctl, sizeof(StreamCtl)))) return rc;
+  if ((rc = dev_alloc(w.slot_info, (size_t)c.n_slots * sizeof(SlotInfo)))) return rc;
+  if ((rc = dev_alloc(w.live, (size_t)c.n_slots * sizeof(int32_t)))) return rc;
+  if ((rc = dev_alloc(w.free_list, (size_t)c.n_slots * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(w.done_list, (size_t)c.n_slots * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(w.new_tiles, (size_t)c.n_slots * sizeof(NewTile)))) return rc;
+  if ((rc = dev_alloc(w.scenes, (size_t)n_scenes * sizeof(Params)))) return rc;
+  if ((rc = dev_alloc(w.tiles, (size_t)n_tiles * sizeof(TileDesc)))) return rc;
+  if ((rc = dev_alloc(w.outs, (size_t)n_scenes * 2 * sizeof(float*)))) return rc;
+  const int64_t ovf_threads = (int64_t)std::max(1, lead->cu_count) * kMaxTravBlocksPerCu * 256;
+  if ((rc = dev_alloc(w.ovf, (size_t)ovf_threads * 2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * sizeof(uint32_t))))
+    return rc;
+  w.ovf_threads = ovf_threads;
+  if (!w.h_ctl) HIPCHK(hipHostMalloc((void**)&w.h_ctl, 2 * sizeof(StreamCtl), hipHostMallocDefault));
+  w.n_slots = c.n_slots;
+  w.tile_shift = c.tile_shift;
+  return PRT_OK;
+}
+
+struct BatchResult {
+  double ms = 0;
+  uint64_t samples = 0;
+};
+
+// One batched render with a resolved configuration.
+int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* const* d_rgb, float* const* d_var,
+               prt_stats* stats, BatchResult* res) {
+  prt_scene* lead = scenes[0];
+  HIPCHK(hipSetDevice(lead->device));
+  const int64_t T = (int64_t)1 << c.tile_shift;
+  // work queue: scene-major runs of T / spp pixels; every tile_stride-th tile
+  std::vector<TileDesc> tiles;
+  bool glass = false, nee = false;
+  uint64_t samples = 0;
+  int64_t tile_counter = 0;
+  for (int i = 0; i < n; ++i) {
+    const Params& P = scenes[i]->P;
+    glass |= scenes[i]->has_glass;
+    nee |= (P.nee && P.n_lights > 0);
+    const int64_t npix = (int64_t)P.W * P.H;
+    const int64_t per = std::max<int64_t>(1, T / P.spp);
+    for (int64_t p = 0; p < npix; p += per, ++tile_counter) {
+      if (tile_counter % c.tile_stride != 0) continue;
+      TileDesc td;
+      td.scene = i;
+      td.n_pixels = (int32_t)std::min(per, npix - p);
+      td.pixel_base = p;
+      tiles.push_back(td);
+      samples += (uint64_t)td.n_pixels * P.spp;
+    }
+  }
+  if (tiles.size() >= ((size_t)1 << 31)) return fail(PRT_E_INVALID, "too many tiles");
+  int rc;
+  if ((rc = ensure_stream_workspace(lead, c, glass, n, (int64_t)tiles.size()))) return rc;
+  StreamWorkspace& w = *lead->sws;
+  hipStream_t s = lead->stream;
+
+  std::vector<Params> params(n);
+  std::vector<float*> outs(2 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    Params P = scenes[i]->P;
+    P.key0 = (uint32_t)c.seed;
+    P.key1 = (uint32_t)(c.seed >> 32);
+    P.max_vertices = c.max_vertices;
+    P.stack_ovf = (uint32_t*)w.ovf.p;
+    P.ovf_threads = (int32_t)w.ovf_threads;
+    params[i] = P;
+    outs[i] = d_rgb[i];
+    outs[n + i] = d_var ? d_var[i] : nullptr;
+  }
+  // the lead scene's stream must see the other scenes' uploads and updates
+  for (int i = 1; i < n; ++i)
+    if (scenes[i]->stream != s) HIPCHK(hipStreamSynchronize(scenes[i]->stream));
+  HIPCHK(hipMemcpyAsync(w.scenes.p, params.data(), params.size() * sizeof(Params), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(w.tiles.p, tiles.data(), tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(w.outs.p, outs.data(), outs.size() * sizeof(float*), hipMemcpyHostToDevice, s));
+
+  Stream S{};
+  S.scenes = (const Params*)w.scenes.p;
+  S.tiles = (const TileDesc*)w.tiles.p;
+  S.out_rgb = (float* const*)w.outs.p;
+  S.out_var = (float* const*)w.outs.p + n;
+  S.slot_info = (SlotInfo*)w.slot_info.p;
+  S.live = (int32_t*)w.live.p;
+  S.free_list = (uint32_t*)w.free_list.p;
+  S.done_list = (uint32_t*)w.done_list.p;
+  S.new_tiles = (NewTile*)w.new_tiles.p;
+  S.ctl = (StreamCtl*)w.ctl.p;
+  S.accum = (uint64_t*)w.accum.p;
+  S.ctr = (uint64_t*)w.ctr.p;
+  S.n_scenes = n;
+  S.n_tiles = (int32_t)tiles.size();
+  S.n_slots = c.n_slots;
+  S.tile_shift = c.tile_shift;
+  S.q_cap = (uint32_t)w.q_cap;
+  S.sq_cap = (uint32_t)w.sq_cap;
+  RayQ q[2] = {ray_queue(w.q[0], w.q_cap), ray_queue(w.q[1], w.q_cap)};
+  ShadowQ sq = shadow_queue(w.sq, w.sq_cap);
+  double* hit_t = (double*)w.hit_t.p;
+  int32_t* hit_id = (int32_t*)w.hit_id.p;
+  const int cu = std::max(1, lead->cu_count);
+  const int trav_blocks = cu * c.trav_blocks_per_cu;
+  const int work_blocks = cu * 8;
+
+  hipEvent_t e0, e1, ev[2];
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventCreate(&ev[0]));
+  HIPCHK(hipEventCreate(&ev[1]));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  auto cleanup = [&]() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+  };
+  auto iteration = [&](int cur) -> hipError_t {
+    const int nxt = cur ^ 1;
+    hipError_t e;
+    if ((e = launch_stream_refill(S, cur, s)) != hipSuccess) return e;
+    if ((e = launch_stream_gen(S, q[cur], work_blocks, s)) != hipSuccess) return e;
+    if ((e = launch_stream_extend(S, q[cur], cur, hit_t, hit_id, c.stack_lds, trav_blocks, s)) != hipSuccess) return e;
+    if ((e = launch_stream_shade(S, q[cur], hit_t, hit_id, q[nxt], nxt, work_blocks, s)) != hipSuccess) return e;
+    if (nee) {
+      if ((e = launch_stream_nee(S, q[nxt], nxt, sq, work_blocks, s)) != hipSuccess) return e;
+      if ((e = launch_stream_shadow(S, sq, c.stack_lds, trav_blocks, s)) != hipSuccess) return e;
+    }
+    return launch_stream_resolve(S, cu * 2, s);
+  };
+
+  hipError_t he = hipEventRecord(e0, s);
+  if (he == hipSuccess) he = launch_fill_u64(S.accum, 0ull, (int64_t)c.n_slots * T * 3, s);
+  if (he == hipSuccess) he = launch_fill_u64(S.ctr, 0ull, CTR_COUNT, s);
+  if (he == hipSuccess) he = launch_stream_init(S, s);
+  if (he != hipSuccess) {
+    cleanup();
+    return fail(PRT_E_HIP, std::string("stream init: ") + hipGetErrorString(he));
+  }
+  if (c.use_graph) {
+    he = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (he == hipSuccess) {
+      hipError_t e_it = iteration(0);
+      if (e_it == hipSuccess) e_it = iteration(1);
+      he = hipStreamEndCapture(s, &graph);
+      if (he == hipSuccess) he = e_it;
+    }
+    if (he == hipSuccess) he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (he != hipSuccess) {
+      cleanup();
+      return fail(PRT_E_HIP, std::string("hipGraph capture of the batch iteration: ") + hipGetErrorString(he));
+    }
+  }
+  // replay in groups; read the status of group g-1 while group g runs
+  int64_t iters = 0;
+  int g = 0;
+  int status = 0;
+  while (true) {
+    for (int r = 0; r < kReplaysPerGroup && he == hipSuccess; ++r) {
+      if (c.use_graph) {
+        he = hipGraphLaunch(exec, s);
+      } else {
+        he = iteration(0);
+        if (he == hipSuccess) he = iteration(1);
+      }
+      iters += 2;
+    }
+    if (he == hipSuccess) he = hipMemcpyAsync(&w.h_ctl[g & 1], S.ctl, sizeof(StreamCtl), hipMemcpyDeviceToHost, s);
+    if (he == hipSuccess) he = hipEventRecord(ev[g & 1], s);
+    if (he == hipSuccess && g >= 1) {
+      he = hipEventSynchronize(ev[(g - 1) & 1]);
+      status = (int)w.h_ctl[(g - 1) & 1].status;
+    }
+    if (he != hipSuccess) {
+      (void)hipStreamSynchronize(s);
+      cleanup();
+      return fail(PRT_E_HIP, std::string("batch iteration: ") + hipGetErrorString(he));
+    }
+    ++g;
+    if (status != 0) break;
+    if (iters > kMaxIterations) {
+      (void)hipStreamSynchronize(s);
+      cleanup();
+      return fail(PRT_E_INVALID, "batched render did not finish within the iteration limit");
+    }
+  }
+  he = hipEventRecord(e1, s);
+  uint64_t ctrh[CTR_COUNT];
+  StreamCtl final_ctl;
+  if (he == hipSuccess) he = hipMemcpyAsync(ctrh, S.ctr, sizeof(ctrh), hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipMemcpyAsync(&final_ctl, S.ctl, sizeof(StreamCtl), hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  float ms = 0;
+  if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+  cleanup();
+  if (he != hipSuccess) return fail(PRT_E_HIP, std::string("batch completion: ") + hipGetErrorString(he));
+  if (status == 2) return fail(PRT_E_INVALID, "batched render stalled: tiles in flight without rays");
+  if (status == 3) return fail(PRT_E_INVALID, "batched render: ray queue capacity exceeded");
+  if (ctrh[CTR_DROP] != 0)
+    return fail(PRT_E_INVALID, "ray queue overflow (" + std::to_string(ctrh[CTR_DROP]) + " rays)");
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->node_visits = ctrh[CTR_NODES] + ctrh[CTR_SNODES];
+    stats->tri_tests = ctrh[CTR_TRIS] + ctrh[CTR_STRIS];
+    stats->shadow_node_visits = ctrh[CTR_SNODES];
+    stats->shadow_tri_tests = ctrh[CTR_STRIS];
+    stats->pickup_hits = ctrh[CTR_PICKUP];
+    stats->truncated_paths = ctrh[CTR_TRUNC];
+    stats->camera_samples = samples;
+    stats->primary_rays = final_ctl.total_prim;
+    stats->extension_rays = final_ctl.total_cont;
+    stats->shadow_rays = ctrh[CTR_SHADOW];
+    stats->ms_total = ms;
+    stats->launches_extend = (int64_t)final_ctl.iter;
+    stats->bytes_extend = 128 * ctrh[CTR_NODES] + 48 * ctrh[CTR_TRIS] + 48 * (final_ctl.total_prim + final_ctl.total_cont);
+    stats->bytes_shadow = 128 * ctrh[CTR_SNODES] + 48 * ctrh[CTR_STRIS] + 48 * ctrh[CTR_SHADOW];
+    stats->bytes_algorithmic = stats->bytes_extend + stats->bytes_shadow;
+  }
+  if (res) {
+    res->ms = ms;
+    res->samples = samples;
+  }
+  return PRT_OK;
+}
+
+int check_batch_args(prt_scene* const* scenes, int n, float* const* d_rgb) {
+  if (!scenes || n <= 0 || !d_rgb) return fail(PRT_E_INVALID, "bad arguments");
+  for (int i = 0; i < n; ++i) {
+    if (!scenes[i] || !d_rgb[i]) return fail(PRT_E_INVALID, "null scene or output in the batch");
+    if (scenes[i]->device != scenes[0]->device) return fail(PRT_E_INVALID, "batched scenes must share one device");
+  }
+  return PRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int prt_render_batch_device(prt_scene* const* scenes, int n_scenes, const prt_batch_opts* opts, float* const* d_rgb,
+                            float* const* d_var, prt_stats* stats) {
+  int rc;
+  if ((rc = check_batch_args(scenes, n_scenes, d_rgb))) return rc;
+  BatchConfig c;
+  if ((rc = resolve_config(scenes, n_scenes, opts, c))) return rc;
+  return run_stream(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
+}
+
+// Coordinate descent over the in-flight sample count, tile size, LDS stack
+// depth and traversal grid, each trial rendering every k-th tile so that a
+// trial costs about 1/k of the whole batch; ranked by device time per camera
+// sample.
+int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts* base, float* const* d_rgb,
+                   float* const* d_var, prt_batch_opts* best, double* best_ns_per_sample) {
+  int rc;
+  if ((rc = check_batch_args(scenes, n_scenes, d_rgb))) return rc;
+  if (!best) return fail(PRT_E_INVALID, "null result");
+  prt_batch_opts cur{};
+  if (base) cur = *base;
+  uint64_t total = 0;
+  for (int i = 0; i < n_scenes; ++i) total += (uint64_t)scenes[i]->P.W * scenes[i]->P.H * scenes[i]->P.spp;
+  if (cur.tile_stride <= 0) {
+    // about 16M camera samples per trial (at least the whole batch / 16)
+    cur.tile_stride = (int)std::max<uint64_t>(1, std::min<uint64_t>(16, total / ((uint64_t)1 << 24)));
+  }
+  if (cur.tile_samples <= 0) cur.tile_samples = kDefaultTileSamples;
+  if (cur.stack_lds <= 0) cur.stack_lds = kDefaultStackLds;
+  if (cur.trav_blocks_per_cu <= 0) cur.trav_blocks_per_cu = kDefaultTravBlocksPerCu;
+  auto inflight_of = [](const prt_batch_opts& o) {
+    return o.inflight_tiles > 0 ? (int64_t)o.inflight_tiles * o.tile_samples : kDefaultInflightSamples;
+  };
+  int64_t inflight = inflight_of(cur);
+  auto trial = [&](prt_batch_opts o, int64_t infl, double& ns) -> int {
+    o.inflight_tiles = (int32_t)std::max<int64_t>(1, infl / o.tile_samples);
+    BatchConfig c;
+    int r = resolve_config(scenes, n_scenes, &o, c);
+    if (r) return r;
+    BatchResult res;
+    r = run_stream(scenes, n_scenes, c, d_rgb, d_var, nullptr, &res);
+    if (r) return r;
+    ns = res.samples ? res.ms * 1e6 / (double)res.samples : 1e30;
+    return PRT_OK;
+  };
+  double best_ns = 0;
+  if ((rc = trial(cur, inflight, best_ns))) return rc;  // also allocates the workspace
+  if ((rc = trial(cur, inflight, best_ns))) return rc;
+  const int64_t inflights[] = {(int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22};
+  const int tiles[] = {1024, 4096, 16384};
+  const int stacks[] = {8, 16, 32};
+  const int blocks[] = {4, 8, 16};
+  for (int64_t v : inflights) {
+    if (v == inflight) continue;
+    double ns;
+    if ((rc = trial(cur, v, ns))) return rc;
+    if (ns < best_ns) { best_ns = ns; inflight = v; }
+  }
+  for (int v : tiles) {
+    if (v == cur.tile_samples) continue;
+    prt_batch_opts o = cur;
+    o.tile_samples = v;
+    double ns;
+    if ((rc = trial(o, inflight, ns))) return rc;
+    if (ns < best_ns) { best_ns = ns; cur = o; }
+  }
+  for (int v : stacks) {
+    if (v == cur.stack_lds) continue;
+    prt_batch_opts o = cur;
+    o.stack_lds = v;
+    double ns;
+    if ((rc = trial(o, inflight, ns))) return rc;
+    if (ns < best_ns) { best_ns = ns; cur = o; }
+  }
+  for (int v : blocks) {
+    if (v == cur.trav_blocks_per_cu) continue;
+    prt_batch_opts o = cur;
+    o.trav_blocks_per_cu = v;
+    double ns;
+    if ((rc = trial(o, inflight, ns))) return rc;
+    if (ns < best_ns) { best_ns = ns; cur = o; }
+  }
+  cur.inflight_tiles = (int32_t)std::max<int64_t>(1, inflight / cur.tile_samples);
+  cur.tile_stride = base ? base->tile_stride : 0;
+  *best = cur;
+  if (best_ns_per_sample) *best_ns_per_sample = best_ns;
+  return PRT_OK;
+}
+
+}  // extern "C"

@@ -18,6 +18,8 @@
 #ifndef PRT_STACK_LDS
 #define PRT_STACK_LDS 16
 #endif
+// smallest LDS stack depth any kernel variant uses (sizes the overflow area)
+#define PRT_STACK_LDS_MIN 8
 #define PRT_STACK_TOTAL 128
 
 struct __attribute__((aligned(16))) DevNode {

@@ -1,0 +1,105 @@
+// prt_internal.h — host-side internals of libprt shared by the C ABI
+// translation units (prt_api.cpp: scenes, queries, per-scene renders, the
+// adjoint; prt_batch.cpp: batched multi-scene renders and their tuner).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/prt.h"
+#include "prt_device.h"
+#include "prt_kernels.h"
+
+namespace prt {
+
+// message of the last failure on this thread (prt_last_error)
+extern thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ::prt::fail(PRT_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+// (Re)allocates b to at least `bytes` (contents are not kept).
+int dev_alloc(DevBuf& b, size_t bytes);
+
+template <class T>
+int dev_upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  int rc = dev_alloc(b, std::max<size_t>(v.size() * sizeof(T), 256));
+  if (rc) return rc;
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  return PRT_OK;
+}
+
+// Ray / shadow queues as 17 / 12 separately allocated SoA columns.
+enum { RAYQ_COLS = 17, SHADOWQ_COLS = 12 };
+int alloc_ray_queue(DevBuf* cols, int64_t cap);
+int alloc_shadow_queue(DevBuf* cols, int64_t cap);
+RayQ ray_queue(DevBuf* b, int64_t cap);
+ShadowQ shadow_queue(DevBuf* b, int64_t cap);
+
+// Per-scene render workspace sized for a batch of samples.
+struct Workspace {
+  int64_t batch_samples = 0;  // camera samples per batch
+  int64_t queue_cap = 0;      // rays per queue
+  int64_t shadow_cap = 0;
+  DevBuf q[2][RAYQ_COLS];     // two ray queues (SoA)
+  DevBuf hit_t, hit_id;
+  DevBuf sq[SHADOWQ_COLS];
+  DevBuf accum, accum2;       // fixed-point sample radiance (accum2: adjoint primal)
+  DevBuf counts;              // uint32 [4]: q0 count, q1 count, shadow count, spare
+  DevBuf fetch;               // uint32 [1024] work-fetch counters, one per launch
+  DevBuf ctr;                 // uint64 [CTR_COUNT]
+  uint32_t* h_counts = nullptr;
+  void release();
+};
+
+// Workspace of the multi-scene streaming renderer (prt_batch.cpp).
+struct StreamWorkspace;
+void release_stream_workspace(StreamWorkspace* w);
+
+}  // namespace prt
+
+struct prt_scene {
+  int device = 0;
+  int cu_count = 0;
+  hipStream_t stream = nullptr;
+  Params P{};  // scene + sampling part; per-call fields are patched in
+  prt::DevBuf nodes, tris, tri_nrm, tri_alb, ents, mats, analytic, lights, ovf;
+  std::vector<DevEntity> h_ents;
+  std::vector<DevMaterial> h_mats;
+  std::vector<int32_t> h_lights;
+  std::vector<uint32_t> slot_of_tri;  // global input triangle -> leaf slot
+  std::vector<int64_t> ent_first_tri; // per entity: first global input triangle
+  std::vector<int64_t> ent_n_tris;
+  int64_t n_tris = 0;
+  int bvh_depth = 0;
+  double bvh_ms = 0;
+  bool has_glass = false;
+  int max_trav_blocks = 0;
+  prt::Workspace ws;
+  prt::StreamWorkspace* sws = nullptr;  // batched renders led by this scene
+  prt::DevBuf out_rgb, out_var;         // staging for host-output renders
+  prt::DevBuf d_slot_of_tri;            // input triangle -> leaf slot (per-face albedo updates)
+  prt::DevBuf grad_buf;                 // adjoint accumulators (double)
+  prt::DevBuf tri_alb_rr;               // held albedo for Russian-roulette probabilities
+};

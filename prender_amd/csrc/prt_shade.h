@@ -1,0 +1,353 @@
+// prt_shade.h — one path vertex and its next-event estimation, shared by the
+// per-scene wavefront kernels (prt_kernels.hip) and the multi-scene streaming
+// kernels (prt_stream.hip).
+//
+// The light transport follows radiance() (radiance.h:25-556) and the BRDFs
+// (brdf.h:408-953) turned from recursion into forward throughput: a vertex
+// returns Env + E + direct + W * L(next), so each path segment carries T, adds
+// T*(Env + E) and the NEE term, and its child carries T*W.  The "pickup" ray
+// of mirror/glass vertices (direct_light_perfect_refrection,
+// directLight_radiance.h:516-556) is the same ray as the child, so the child's
+// hit supplies it: the child records a pickup weight and adds weight * emission
+// when it lands on a light.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "prt_device.h"
+#include "prt_kernels.h"
+#include "prt_math.h"
+
+// Camera ray of sub-pixel (sx, sy) of pixel (x, y) (render.h:1248-1269:
+// r = (s + 0.5)/SS, no jitter).
+__device__ __forceinline__ void camera_ray(const Params& P, int x, int y, int sx, int sy, d3& o, d3& d) {
+  const double rate = 1.0 / P.SS;
+  const double r1 = (sx + 0.5 + 0.0) * rate;
+  const double r2 = (sy + 0.5 + 0.0) * rate;
+  const d3 sp = ld3(P.center) + ld3(P.screen_x) * ((r1 + x) * P.iw - 0.5) + ld3(P.screen_y) * ((r2 + y) * P.ih - 0.5);
+  o = ld3(P.cam_pos);
+  d = normalize(sp - o);
+}
+
+struct ShadeCtx {
+  RayQ qout;
+  uint32_t* count_out;
+  uint32_t slot;
+  uint32_t truncated, n_ext, pickups, dropped;
+};
+
+// What the adjoint needs to know about one shaded vertex.
+struct VertexAdj {
+  int32_t entity;  // hit entity (-1: miss)
+  int32_t face;    // face of a POLYGON hit
+  bool albedo;     // the continuation and NEE of this vertex scale with its colour
+  d3 rho;          // that colour
+  d3 own;          // radiance added at this vertex that does not scale with it (T*(Env+E))
+  d3 pre;          // pickup added for the parent vertex (belongs to the radiance before this vertex)
+  d3 add_e;        // radiance added here that is linear in the hit entity's emission, / emission
+};
+
+__device__ __forceinline__ void emit_child(ShadeCtx& C, d3 o, d3 d, d3 ct, d3 pw, uint32_t meta, d3 nrm) {
+  const uint32_t j = agg_append(C.count_out);
+  if (j >= C.qout.cap) {
+    C.dropped++;
+    return;
+  }
+  C.qout.ox[j] = o.x; C.qout.oy[j] = o.y; C.qout.oz[j] = o.z;
+  C.qout.dx[j] = d.x; C.qout.dy[j] = d.y; C.qout.dz[j] = d.z;
+  C.qout.tr[j] = (float)ct.x; C.qout.tg[j] = (float)ct.y; C.qout.tb[j] = (float)ct.z;
+  C.qout.pr[j] = (float)pw.x; C.qout.pg[j] = (float)pw.y; C.qout.pb[j] = (float)pw.z;
+  C.qout.slot[j] = C.slot;
+  C.qout.meta[j] = meta;
+  if (meta & META_NEE) {
+    C.qout.nx[j] = (float)nrm.x; C.qout.ny[j] = (float)nrm.y; C.qout.nz[j] = (float)nrm.z;
+  }
+  C.n_ext++;
+}
+
+// One path vertex: the radiance() body for the hit (or miss) of ray (o, d).
+// Returns the radiance added to the camera sample; child rays go to the
+// output queue, and a diffuse vertex marks its child for next-event
+// estimation.
+__device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d3 d, double t, int32_t id, d3 T,
+                                           d3 Pw, uint32_t pixel, uint32_t sample, int depth, uint32_t branch,
+                                           VertexAdj& va) {
+  d3 add = mk(0.0, 0.0, 0.0);
+  const d3 zero = mk(0.0, 0.0, 0.0);
+  va.entity = -1;
+  va.face = -1;
+  va.albedo = false;
+  va.rho = va.own = va.pre = va.add_e = zero;
+  if (depth >= P.max_vertices) {  // safety cap (the reference has no hard cap)
+    C.truncated++;
+    return add;
+  }
+  if (id == -1)  // miss: BackgroundColor (radiance.h:42)
+    return vmul(T, ld3(P.background));
+  const d3 o2 = o + d * 0.000001;
+  d3 x, n;
+  int ei;
+  if (id >= 0) {
+    const float4* tr = reinterpret_cast<const float4*>(P.tris) + (size_t)id * 3;
+    ei = __float_as_int(tr[0].w);
+    x = o2 + d * t;  // polygon.h:170
+    n = mk(P.tri_nrm[3 * (size_t)id], P.tri_nrm[3 * (size_t)id + 1], P.tri_nrm[3 * (size_t)id + 2]);
+  } else {
+    ei = -2 - id;
+    const DevEntity& e = P.ents[ei];
+    if (e.type == ENT_SPHERE) {
+      x = o2 + t * d;  // sphere.h:89-95
+      n = normalize(x - ld3(e.p0));
+    } else {
+      x = d * t + o2;
+      n = ld3(e.normal);
+    }
+  }
+  const DevEntity& e = P.ents[ei];
+  va.entity = ei;
+  if (id >= 0) va.face = __float_as_int(reinterpret_cast<const float4*>(P.tris)[(size_t)id * 3 + 1].w);
+  if (e.ninv < 0) n = n * -1.0;
+  const DevMaterial& m = P.mats[e.material];
+  d3 color;
+  if (e.has_albedo && id >= 0)
+    color = mk(P.tri_alb[3 * (size_t)id], P.tri_alb[3 * (size_t)id + 1], P.tri_alb[3 * (size_t)id + 2]);
+  else
+    color = ld3(m.color);
+  const d3 E = ld3(m.emission);
+  // pickup of the parent's mirror / glass vertex (directLight_radiance.h:516-556)
+  if (e.light && (Pw.x != 0.0 || Pw.y != 0.0 || Pw.z != 0.0)) {
+    add = add + vmul(Pw, E);
+    va.pre = vmul(Pw, E);
+    va.add_e = Pw;
+    C.pickups++;
+  }
+  const d3 n_orient = dot(n, d) < 0.0 ? n : (-1.0 * n);
+  // RGB_MAX = std::max(x, std::max(y, z)) (def.h:19); RR (radiance.h:77-126)
+  d3 color_rr = color;
+  if (P.tri_alb_rr && e.has_albedo && id >= 0)
+    color_rr = mk(P.tri_alb_rr[3 * (size_t)id], P.tri_alb_rr[3 * (size_t)id + 1], P.tri_alb_rr[3 * (size_t)id + 2]);
+  const d3 rw = E + color_rr;
+  const double myz = (rw.y < rw.z) ? rw.z : rw.y;
+  double rr = (rw.x < myz) ? myz : rw.x;
+  if (depth > P.depth_limit) rr *= ldexp(1.0, -(depth - P.depth_limit));  // pow(0.5, k), exact
+  uint32_t b0[4];
+  const uint32_t c2 = rng_c2(depth, branch);
+  philox(pixel, sample, c2, 0u, P.key0, P.key1, b0);
+  if (depth > P.depth) {
+    if (u01(b0[0]) >= rr) {  // terminate with the emission
+      va.own = vmul(T, E);
+      va.add_e = va.add_e + T;
+      return add + vmul(T, E);
+    }
+  } else {
+    rr = 1.0;
+  }
+  int kind;  // 0 diffuse, 1 specular, 2 refraction (radiance.h:514-551)
+  if (m.type == REFL_DIFFUSE) kind = (u01(b0[1]) < m.roughness) ? 0 : 1;
+  else if (m.type == REFL_SPECULAR) kind = (u01(b0[1]) < 1.0 - m.roughness) ? 1 : 0;
+  else kind = 2;
+  const uint32_t meta_same = rng_c2(depth + 1, branch);
+  const double pk = P.nee ? 1.0 : 0.0;
+  if (kind == 0) {
+    // BRDF_Diffuse, brdf.h:408-700
+    if (e.light && e.type == ENT_UVPLANE && dot(n, d) >= 0.0) return add;
+    if (e.light) {
+      if (!P.nee || depth == 0) {
+        add = add + vmul(T, E / rr / (1.0 - 0.0));
+        va.own = vmul(T, E / rr);
+        va.add_e = va.add_e + T / rr;
+      }
+      return add;
+    }
+    add = add + vmul(T, ld3(P.env_light) + E);
+    va.own = vmul(T, ld3(P.env_light) + E);
+    va.add_e = va.add_e + T;
+    va.albedo = true;
+    va.rho = color;
+    // OrthonormalBasis (vector3d.h:244-253), cosine sampling (brdf.h:498-505)
+    const d3 w = n_orient;
+    const d3 uu = (fabs(w.x) > PS_EPS) ? normalize(cross(mk(0.0, 1.0, 0.0), w)) : normalize(cross(mk(1.0, 0.0, 0.0), w));
+    const d3 vv = normalize(cross(w, uu));
+    const double r1 = PS_TWOPI * u01(b0[2]);
+    const double r2 = u01(b0[3]), r2s = sqrt(r2);
+    double s1, c1;
+    sincos(r1, &s1, &c1);
+    const d3 dir = normalize((uu * c1 * r2s + vv * s1 * r2s + w * sqrt(1.0 - r2)));
+    const d3 ct = vmul(T, color / rr / (1.0 - 0.0));
+    const bool nee = P.nee && P.n_lights > 0;
+    emit_child(C, x, dir, ct, zero, meta_same | (nee ? META_NEE : 0u), n_orient);
+    return add;
+  }
+  add = add + vmul(T, ld3(P.env_light) + E);
+  va.own = vmul(T, ld3(P.env_light) + E);
+  va.add_e = va.add_e + T;
+  va.albedo = true;
+  va.rho = color;
+  if (kind == 1) {
+    // BRDF_Specular, brdf.h:702-761
+    const d3 dir = normalize(d - n * 2.0 * dot(n, d));
+    const d3 ct = vmul(T, color / rr / (1.0 - 0.0));
+    emit_child(C, x, dir, ct, ct * pk, meta_same, zero);
+    return add;
+  }
+  // BRDF_Refraction, brdf.h:763-953
+  const d3 refl = normalize(d - n * 2.0 * dot(n, d));
+  const bool into = dot(n, n_orient) > 0.0;
+  const double nc = 1.0, nt = m.ior;
+  const double nnt = into ? nc / nt : nt / nc;
+  const double ddn = dot(d, n_orient);
+  const double cos2t = 1.0 - nnt * nnt * (1.0 - ddn * ddn);
+  if (cos2t < 0.0) {  // total internal reflection
+    const d3 ct = vmul(T, color / rr / (1.0 - 0.0));
+    emit_child(C, x, refl, ct, ct * pk, meta_same, zero);
+    return add;
+  }
+  const d3 tdir = normalize(d * nnt - n * (into ? 1.0 : -1.0) * (ddn * nnt + sqrt(fabs(cos2t))));
+  const double a = nt - nc, b = nt + nc;
+  const double R0 = (a * a) / (b * b);
+  const double c = 1.0 - (into ? -ddn : dot(tdir, -1.0 * n_orient));
+  const double Re = R0 + (1.0 - R0) * pow(c, 5.0);
+  const double q = into ? nc / nt : nt / nc;
+  const double nnt2 = q * q;  // pow(q, 2.0), exact
+  const double Tr = (1.0 - Re) * nnt2;
+  const double probability = 0.25 + 0.5 * Re;
+  if (depth > 2) {
+    uint32_t b3[4];
+    philox(pixel, sample, c2, 3u, P.key0, P.key1, b3);
+    const bool take_refl = u01(b3[1]) < probability;  // dimension 13
+    const d3 wgt = take_refl ? color / (probability * rr) / (1.0 - 0.0)
+                             : color / ((1.0 - probability) * rr) / (1.0 - 0.0);
+    const d3 tw = vmul(T, wgt);
+    emit_child(C, x, take_refl ? refl : tdir, tw * (take_refl ? Re : Tr), tw * pk, meta_same, zero);
+    return add;
+  }
+  // depth <= 2: trace both (reflection = branch*2, refraction = branch*2+1)
+  const d3 tw = vmul(T, color / (rr) / (1.0 - 0.0));
+  emit_child(C, x, refl, tw * Re, tw * pk, rng_c2(depth + 1, branch * 2u), zero);
+  emit_child(C, x, tdir, tw * Tr, tw * pk, rng_c2(depth + 1, branch * 2u + 1u), zero);
+  return add;
+}
+
+// ------------------------------------------------------------ NEE
+struct ShadowRay {
+  d3 o2, dir;
+  double tmax;
+  float c[3];
+};
+
+// direct_radiance_sample + eval_direct_radiance (diffuse branch),
+// directLight_radiance.h:19-387, for one light sample.  Produces a shadow ray
+// only when the light's own first hit along it lies within 1e-3 of the
+// sampled distance (the reference's acceptance test, line 141); the shadow
+// kernel then checks that nothing nearer blocks it.
+__device__ __forceinline__ bool nee_sample(const Params& P, int li, double ra, double rb, d3 x, d3 n_orient, d3 ct,
+                                           ShadowRay& s) {
+  const DevEntity& le = P.ents[li];
+  d3 light_pos, light_normal;
+  if (le.type == ENT_SPHERE) {
+    const double r1 = PS_TWOPI * ra;
+    const double r2 = 1.0 - 2.0 * rb;
+    const double s22 = sqrt(1.0 - r2 * r2);
+    double sn, cs;
+    sincos(r1, &sn, &cs);
+    light_pos = ld3(le.p0) + ((le.radius + PS_EPS) * mk(s22 * cs, s22 * sn, r2));
+    light_normal = normalize(light_pos - ld3(le.p0));
+  } else {
+    const double tt = le.v_len * ra;
+    const double ss = le.u_len * rb;
+    light_pos = ld3(le.p0) + ld3(le.u_axis) * ss + ld3(le.v_axis) * tt + ld3(le.normal) * PS_EPS;
+    light_normal = ld3(le.normal);
+  }
+  if (le.ninv < 0) light_normal = light_normal * -1.0;
+  const double pdf = 1.0 / le.area;
+  const d3 light_dir = normalize(light_pos - x);
+  const double dist2 = sqr(light_pos - x);
+  const double dot0 = dot(n_orient, light_dir);
+  const double dot1 = dot(light_normal, -1.0 * light_dir);
+  if (!(dot0 >= 0 && dot1 >= 0)) return false;
+  const d3 o2 = x + light_dir * 0.000001;
+  double tl;
+  if (!analytic_t(le, o2, light_dir, tl)) return false;
+  if (!(fabs(sqrt(dist2) - tl) < 1e-3)) return false;
+  const double G = dot0 * dot1 / dist2;
+  const d3 E = ld3(P.mats[le.material].emission);
+  // (E * colour/pi) * G / pdf, averaged over the 3 samples, / rr, times T;
+  // ct already holds T * colour / rr
+  const d3 rad = E * (PS_INV_PI * G / pdf / 3.0);
+  s.o2 = o2;
+  s.dir = light_dir;
+  s.tmax = tl;
+  s.c[0] = (float)(ct.x * rad.x);
+  s.c[1] = (float)(ct.y * rad.y);
+  s.c[2] = (float)(ct.z * rad.z);
+  return true;
+}
+
+__device__ __forceinline__ void put_shadow(ShadowQ& q, uint32_t j, const ShadowRay& s, uint32_t slot, int32_t light) {
+  q.light[j] = light;
+  q.ox[j] = s.o2.x; q.oy[j] = s.o2.y; q.oz[j] = s.o2.z;
+  q.dx[j] = s.dir.x; q.dy[j] = s.dir.y; q.dz[j] = s.dir.z;
+  q.tmax[j] = s.tmax;
+  q.cr[j] = s.c[0]; q.cg[j] = s.c[1]; q.cb[j] = s.c[2];
+  q.slot[j] = slot;
+}
+
+// The three light samples of a diffuse vertex at x (orienting normal nrm,
+// throughput ct = T*colour/rr), with the parent vertex's random numbers
+// (dimensions 4..12).  Returns how many shadow rays it produced.
+__device__ __forceinline__ uint32_t nee_vertex(const Params& P, uint32_t pixel, uint32_t sample, int depth,
+                                               uint32_t branch, d3 x, d3 nrm, d3 ct, ShadowRay& s0, ShadowRay& s1,
+                                               ShadowRay& s2, bool& v0, bool& v1, bool& v2, int32_t& l0, int32_t& l1,
+                                               int32_t& l2) {
+  const int NL = P.n_lights;
+  const uint32_t c2 = rng_c2(depth, branch);
+  uint32_t b1[4], b2[4], b3[4];
+  philox(pixel, sample, c2, 1u, P.key0, P.key1, b1);
+  philox(pixel, sample, c2, 2u, P.key0, P.key1, b2);
+  philox(pixel, sample, c2, 3u, P.key0, P.key1, b3);
+  // LightList::randomLight (light.h:146-151): index = (int)(u*(N-1)+0.5),
+  // probability 1; dimensions 4+3i (pick), 5+3i and 6+3i (point)
+  int i0 = (int)(u01(b1[0]) * (double)(NL - 1) + 0.5);
+  int i1 = (int)(u01(b1[3]) * (double)(NL - 1) + 0.5);
+  int i2 = (int)(u01(b2[2]) * (double)(NL - 1) + 0.5);
+  i0 = i0 >= NL ? NL - 1 : i0;
+  i1 = i1 >= NL ? NL - 1 : i1;
+  i2 = i2 >= NL ? NL - 1 : i2;
+  l0 = P.lights[i0];
+  l1 = P.lights[i1];
+  l2 = P.lights[i2];
+  v0 = nee_sample(P, l0, u01(b1[1]), u01(b1[2]), x, nrm, ct, s0);
+  v1 = nee_sample(P, l1, u01(b2[0]), u01(b2[1]), x, nrm, ct, s1);
+  v2 = nee_sample(P, l2, u01(b2[3]), u01(b3[0]), x, nrm, ct, s2);
+  return (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
+}
+
+// Luminance of mlt.h:171-174 with the Vector3d dot pairing.
+__device__ __forceinline__ double luminance(double r, double g, double b) {
+  return (0.2126 * r + 0.0722 * b) + 0.7152 * g;
+}
+
+// Folds the spp samples of one pixel (a[3*k..3*k+2], fixed point): v_k = L_k *
+// smp, luminance clamp (render.h:1375-1384), image += v_k, *= sensor_response
+// (render.h:1405-1409), and the variance of the pixel mean.
+__device__ __forceinline__ void fold_pixel(const Params& P, const uint64_t* a, double rgb[3], double var[3]) {
+  double s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+  for (int k = 0; k < P.spp; ++k) {
+    double v0 = accum_get(a + 3 * k) * P.smp, v1 = accum_get(a + 3 * k + 1) * P.smp, v2 = accum_get(a + 3 * k + 2) * P.smp;
+    if (P.cutoff > 0.0f) {
+      const float lv = (float)luminance(v0, v1, v2);
+      if (lv > P.cutoff) {
+        const float f = P.cutoff / lv;
+        v0 *= f; v1 *= f; v2 *= f;
+      }
+    }
+    s1[0] += v0; s1[1] += v1; s1[2] += v2;
+    s2[0] += v0 * v0; s2[1] += v1 * v1; s2[2] += v2 * v2;
+  }
+  const double sr = P.sensor_response;
+  const double nn = (double)P.spp;
+  for (int c = 0; c < 3; ++c) {
+    rgb[c] = s1[c] * sr;
+    var[c] = nn > 1 ? fmax(0.0, (nn * s2[c] - s1[c] * s1[c]) / (nn - 1)) * sr * sr : 0.0;
+  }
+}

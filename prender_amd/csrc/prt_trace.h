@@ -13,40 +13,46 @@ struct TravStats {
   uint32_t nodes, tris;
 };
 
-// Per-lane traversal stack: the first PRT_STACK_LDS entries in LDS (lane
-// interleaved, bank-conflict free), deeper ones in a per-thread global area.
-struct Stack {
-  uint32_t* lds_node;  // [PRT_STACK_LDS][blockDim]
+// Per-lane traversal stack: the first NL entries in LDS (lane interleaved,
+// bank-conflict free), deeper ones in a per-thread global area of
+// (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * 2 words.
+template <int NL>
+struct StackN {
+  uint32_t* lds_node;  // [NL][blockDim]
   float* lds_t;
-  uint32_t* ovf;       // (PRT_STACK_TOTAL - PRT_STACK_LDS) * 2 words
+  uint32_t* ovf;       // (PRT_STACK_TOTAL - NL) * 2 words used
   int stride;
   __device__ __forceinline__ void push(int sp, int32_t node, float t) {
-    if (sp < PRT_STACK_LDS) {
+    if (sp < NL) {
       lds_node[sp * stride] = (uint32_t)node;
       lds_t[sp * stride] = t;
     } else {
-      ovf[2 * (sp - PRT_STACK_LDS)] = (uint32_t)node;
-      ovf[2 * (sp - PRT_STACK_LDS) + 1] = __float_as_uint(t);
+      ovf[2 * (sp - NL)] = (uint32_t)node;
+      ovf[2 * (sp - NL) + 1] = __float_as_uint(t);
     }
   }
   __device__ __forceinline__ void pop(int sp, int32_t& node, float& t) {
-    if (sp < PRT_STACK_LDS) {
+    if (sp < NL) {
       node = (int32_t)lds_node[sp * stride];
       t = lds_t[sp * stride];
     } else {
-      node = (int32_t)ovf[2 * (sp - PRT_STACK_LDS)];
-      t = __uint_as_float(ovf[2 * (sp - PRT_STACK_LDS) + 1]);
+      node = (int32_t)ovf[2 * (sp - NL)];
+      t = __uint_as_float(ovf[2 * (sp - NL) + 1]);
     }
   }
 };
+typedef StackN<PRT_STACK_LDS> Stack;
 
-#define PRT_DECLARE_STACK(S, P)                                                                              \
-  __shared__ uint32_t s_node_[PRT_STACK_LDS * 256];                                                          \
-  __shared__ float s_t_[PRT_STACK_LDS * 256];                                                                \
-  Stack S{s_node_ + threadIdx.x, s_t_ + threadIdx.x,                                                         \
-          (P).stack_ovf + (size_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % (P).ovf_threads) * 2 * \
-                              (PRT_STACK_TOTAL - PRT_STACK_LDS),                                             \
-          (int)blockDim.x}
+// Blocks of 256 threads; a thread's overflow area is picked by its global
+// index modulo ovf_threads (the host sizes the area for the largest grid).
+#define PRT_DECLARE_STACK_N(S, P, NL)                                                                        \
+  __shared__ uint32_t s_node_[(NL) * 256];                                                                   \
+  __shared__ float s_t_[(NL) * 256];                                                                         \
+  StackN<NL> S{s_node_ + threadIdx.x, s_t_ + threadIdx.x,                                                    \
+               (P).stack_ovf + (size_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % (P).ovf_threads) * \
+                                   2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN),                                \
+               (int)blockDim.x}
+#define PRT_DECLARE_STACK(S, P) PRT_DECLARE_STACK_N(S, P, PRT_STACK_LDS)
 
 // a float >= t (t > 0): the culling bound never drops below the double hit
 __device__ __forceinline__ float fup(double t) {
@@ -56,9 +62,9 @@ __device__ __forceinline__ float fup(double t) {
 
 // Closest (ANY = false) or any (ANY = true) hit over the BVH triangles with
 // t in (1e-6, t_best).  o is the already-nudged origin.
-template <bool ANY>
+template <bool ANY, class Stk>
 __device__ __forceinline__ bool traverse(const Params& P, d3 o, d3 d, double& t_best, int32_t& id_best, double& ub,
-                                         double& vb, Stack& S, TravStats& ts) {
+                                         double& vb, Stk& S, TravStats& ts) {
   if (P.n_nodes <= 0) return false;
   const float ox = (float)o.x, oy = (float)o.y, oz = (float)o.z;
   float ix = (float)(1.0 / d.x), iy = (float)(1.0 / d.y), iz = (float)(1.0 / d.z);
@@ -155,8 +161,9 @@ __device__ __forceinline__ bool traverse(const Params& P, d3 o, d3 d, double& t_
 // Closest hit over analytic entities then the BVH; o_in is the ray origin
 // before the 1e-6 nudge.  id >= 0: triangle slot; id <= -2: analytic entity
 // -2-id; -1: miss.
+template <class Stk>
 __device__ __forceinline__ void closest_hit(const Params& P, d3 o_in, d3 d, double& t, int32_t& id, double& u,
-                                            double& v, Stack& S, TravStats& ts) {
+                                            double& v, Stk& S, TravStats& ts) {
   const d3 o = o_in + d * 0.000001;
   t = PS_INF;
   id = -1;
@@ -173,7 +180,8 @@ __device__ __forceinline__ void closest_hit(const Params& P, d3 o_in, d3 d, doub
 }
 
 // Any blocker with t in (1e-6, tmax) from the already-nudged origin o.
-__device__ __forceinline__ bool occluded(const Params& P, d3 o, d3 d, double tmax, Stack& S, TravStats& ts) {
+template <class Stk>
+__device__ __forceinline__ bool occluded(const Params& P, d3 o, d3 d, double tmax, Stk& S, TravStats& ts) {
   for (int k = 0; k < P.n_analytic; ++k) {
     double tt;
     if (analytic_t(P.ents[P.analytic[k]], o, d, tt) && tt < tmax) return true;

@@ -57,7 +57,9 @@ def lib():
     L.prt_render_device.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_void_p,
                                     C.POINTER(PrtStats)]
     L.prt_render_batch_device.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(PrtBatchOpts),
-                                          C.POINTER(C.c_void_p), C.POINTER(PrtStats)]
+                                          C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(PrtStats)]
+    L.prt_tune_batch.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(PrtBatchOpts), C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_void_p), C.POINTER(PrtBatchOpts), C.POINTER(C.c_double)]
     L.prt_set_face_albedo_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     L.prt_set_emission.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
     L.prt_hold_rr_albedo.argtypes = [C.c_void_p, C.c_int]
@@ -69,8 +71,8 @@ def lib():
 
 EXPORTED = ["prt_abi_version", "prt_last_error", "prt_device_info", "prt_scene_create", "prt_scene_destroy",
             "prt_scene_info", "prt_trace_closest", "prt_trace_closest_device", "prt_render", "prt_render_device",
-            "prt_render_batch_device", "prt_set_face_albedo_device", "prt_set_emission", "prt_hold_rr_albedo",
-            "prt_adjoint_device"]
+            "prt_render_batch_device", "prt_tune_batch", "prt_set_face_albedo_device", "prt_set_emission",
+            "prt_hold_rr_albedo", "prt_adjoint_device"]
 
 
 def check(rc: int, what: str):
@@ -194,3 +196,49 @@ class Scene:
                                        C.c_void_p(grad_albedo_ptr) if grad_albedo_ptr else None, light, ge,
                                        C.byref(st)), "prt_adjoint_device")
         return np.array(ge[:]), st.as_dict()
+
+
+# ---- batched scenes
+def batch_opts(seed: int = 0, tile_samples: int = 0, inflight_tiles: int = 0, stack_lds: int = 0,
+               trav_blocks_per_cu: int = 0, use_graph: bool = True, tile_stride: int = 0,
+               max_depth: int = 0) -> PrtBatchOpts:
+    o = PrtBatchOpts()
+    o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    o.tile_samples = int(tile_samples)
+    o.inflight_tiles = int(inflight_tiles)
+    o.stack_lds = int(stack_lds)
+    o.trav_blocks_per_cu = int(trav_blocks_per_cu)
+    o.use_graph = 0 if use_graph else -1
+    o.tile_stride = int(tile_stride)
+    o.max_depth = int(max_depth)
+    return o
+
+
+def _batch_args(scenes, rgb_ptrs, var_ptrs):
+    n = len(scenes)
+    if n == 0 or len(rgb_ptrs) != n or (var_ptrs is not None and len(var_ptrs) != n):
+        raise PrtError("one output buffer per scene is required")
+    hs = (C.c_void_p * n)(*[s.h.value for s in scenes])
+    rg = (C.c_void_p * n)(*[int(p) for p in rgb_ptrs])
+    vr = (C.c_void_p * n)(*[int(p) if p else None for p in var_ptrs]) if var_ptrs is not None else None
+    return n, hs, rg, vr
+
+
+def render_batch_device(scenes, rgb_ptrs, var_ptrs=None, opts: Optional[PrtBatchOpts] = None, **kw):
+    """Renders every scene of `scenes` (Scene objects on one device) in one
+    batched call into the device images rgb_ptrs[i] (W_i*H_i*3 floats)."""
+    n, hs, rg, vr = _batch_args(scenes, rgb_ptrs, var_ptrs)
+    o = opts if opts is not None else batch_opts(**kw)
+    st = PrtStats()
+    check(lib().prt_render_batch_device(hs, n, C.byref(o), rg, vr, C.byref(st)), "prt_render_batch_device")
+    return st.as_dict()
+
+
+def tune_batch(scenes, rgb_ptrs, var_ptrs=None, base: Optional[PrtBatchOpts] = None, **kw):
+    """Runs the batch tuner; returns (best PrtBatchOpts, ns per camera sample)."""
+    n, hs, rg, vr = _batch_args(scenes, rgb_ptrs, var_ptrs)
+    b = base if base is not None else batch_opts(**kw)
+    best = PrtBatchOpts()
+    ns = C.c_double()
+    check(lib().prt_tune_batch(hs, n, C.byref(b), rg, vr, C.byref(best), C.byref(ns)), "prt_tune_batch")
+    return best, ns.value
