@@ -170,6 +170,8 @@ typedef struct prt_stats {
     uint64_t bytes_shadow;        /* algorithmic bytes of the shadow kernels */
     double ms_extend_primary;     /* part of ms_extend spent on camera rays */
     double ms_nee;                /* next-event-estimation kernels (light sampling) */
+    uint64_t mixed_scene_passes;  /* batched renders: extra passes of waves whose rays
+                                     belong to more than one scene */
 } prt_stats;
 
 typedef struct prt_render_opts {
@@ -233,7 +235,7 @@ typedef struct prt_batch_opts {
     int32_t use_graph;          /* >= 0: hipGraph replay; -1: plain launches */
     int32_t tile_stride;        /* render only every k-th tile (tuning trials; 0 = all) */
     int32_t max_depth;          /* safety cap on path vertices (0 = 128) */
-    int32_t pad_;
+    int32_t timing;             /* 1 = per-kernel device timing into prt_stats (plain launches only) */
 } prt_batch_opts;
 /* d_rgb[i] / d_var[i]: W_i*H_i*3 float device buffers of scene i (d_var or
  * its entries may be NULL).  All scenes must live on one device; the call

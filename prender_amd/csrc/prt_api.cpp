@@ -514,42 +514,6 @@ static int ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum)
   return PRT_OK;
 }
 
-struct Timer {
-  bool on = false;
-  hipStream_t s = nullptr;
-  std::vector<std::pair<hipEvent_t, int>> marks;  // event + category of the span that ends here
-  hipEvent_t start{};
-  int begin(bool enable, hipStream_t st) {
-    on = enable;
-    s = st;
-    if (!on) return PRT_OK;
-    HIPCHK(hipEventCreate(&start));
-    HIPCHK(hipEventRecord(start, s));
-    return PRT_OK;
-  }
-  int mark(int cat) {
-    if (!on) return PRT_OK;
-    hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
-    HIPCHK(hipEventRecord(e, s));
-    marks.push_back({e, cat});
-    return PRT_OK;
-  }
-  void collect(double* cat_ms) {
-    if (!on) return;
-    hipEvent_t prev = start;
-    for (auto& m : marks) {
-      float ms = 0;
-      if (hipEventElapsedTime(&ms, prev, m.first) == hipSuccess) cat_ms[m.second] += ms;
-      prev = m.first;
-    }
-    for (auto& m : marks) (void)hipEventDestroy(m.first);
-    (void)hipEventDestroy(start);
-    marks.clear();
-  }
-};
-enum { T_EXTEND = 0, T_SHADE = 1, T_SHADOW = 2, T_OTHER = 3, T_EXTEND0 = 4, T_NEE = 5, T_NCAT = 6 };
-
 // Settings of one render call derived from the scene and the options.
 struct RenderSetup {
   Params P;

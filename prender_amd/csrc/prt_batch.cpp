@@ -22,16 +22,16 @@ namespace prt {
 struct StreamWorkspace {
   int64_t n_slots = 0, tile_shift = 0, q_cap = 0, sq_cap = 0;
   DevBuf q[2][RAYQ_COLS], hit_t, hit_id, sq[SHADOWQ_COLS];
-  DevBuf accum, ctr, ctl, slot_info, live, free_list, done_list, new_tiles;
-  DevBuf scenes, tiles, outs, ovf;
+  DevBuf accum, ctr, ctl, slot_info, live, free_list, new_tiles;
+  DevBuf scenes, tiles, outs, ovf, scene_fan, seg, new_samp, work, work_pref, nee_pref;
   int64_t ovf_threads = 0;
   StreamCtl* h_ctl = nullptr;  // pinned, two copies (double-buffered status reads)
   void release() {
     for (auto& a : q)
       for (auto& b : a) b.release();
     for (auto& b : sq) b.release();
-    DevBuf* all[] = {&hit_t, &hit_id, &accum, &ctr, &ctl, &slot_info, &live, &free_list, &done_list,
-                     &new_tiles, &scenes, &tiles, &outs, &ovf};
+    DevBuf* all[] = {&hit_t, &hit_id, &accum, &ctr, &ctl, &slot_info, &live, &free_list, &new_tiles, &scenes,
+                     &tiles, &outs, &ovf, &scene_fan, &seg, &new_samp, &work, &work_pref, &nee_pref};
     for (DevBuf* b : all) b->release();
     if (h_ctl) (void)hipHostFree(h_ctl);
     h_ctl = nullptr;
@@ -68,6 +68,7 @@ struct BatchConfig {
   bool use_graph = true;
   int tile_stride = 1;
   int max_vertices = 128;
+  bool timing = false;
   uint64_t seed = 0;
 };
 
@@ -83,7 +84,8 @@ int resolve_config(prt_scene* const* scenes, int n, const prt_batch_opts* o, Bat
   int max_spp = 1;
   for (int i = 0; i < n; ++i) max_spp = std::max(max_spp, scenes[i]->P.spp);
   const int64_t ts = opts.tile_samples > 0 ? opts.tile_samples : kDefaultTileSamples;
-  c.tile_shift = std::max(ceil_log2(ts), ceil_log2(max_spp));
+  // at least 64 samples per tile: a wave's chunk of camera samples lies in one tile
+  c.tile_shift = std::max(std::max(ceil_log2(ts), ceil_log2(max_spp)), 6);
   if (c.tile_shift > 24) return fail(PRT_E_INVALID, "tile_samples above 2^24");
   const int64_t T = (int64_t)1 << c.tile_shift;
   c.n_slots = opts.inflight_tiles > 0 ? opts.inflight_tiles
@@ -95,21 +97,28 @@ int resolve_config(prt_scene* const* scenes, int n, const prt_batch_opts* o, Bat
   c.trav_blocks_per_cu = opts.trav_blocks_per_cu > 0 ? opts.trav_blocks_per_cu : kDefaultTravBlocksPerCu;
   if (c.trav_blocks_per_cu > kMaxTravBlocksPerCu) return fail(PRT_E_INVALID, "trav_blocks_per_cu above 16");
   c.use_graph = opts.use_graph >= 0;  // -1 = plain launches
+  c.timing = opts.timing != 0 && !c.use_graph;
   c.tile_stride = std::max(1, opts.tile_stride);
   c.max_vertices = opts.max_depth > 0 ? std::min(opts.max_depth, 255) : 128;
   c.seed = opts.seed;
   return PRT_OK;
 }
 
-int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, bool glass, int n_scenes, int64_t n_tiles) {
+// Queue capacities.  A sample has at most 8 rays in flight with glass (splits
+// at depths 0..2), else 1; a queue's scene regions are bounded by 2 (glass) or
+// 1 child per shaded vertex, i.e. by 16x or 1x the samples in flight, plus
+// 64-entry padding per region; the primary region by the samples in flight
+// plus padding per tile.  Shadow rays: 3 per child of an NEE scene.
+int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, bool glass, bool nee_glass, int n_scenes,
+                            int64_t n_tiles) {
   if (!lead->sws) lead->sws = new (std::nothrow) StreamWorkspace();
   if (!lead->sws) return fail(PRT_E_NOMEM, "out of host memory");
   StreamWorkspace& w = *lead->sws;
   const int64_t T = (int64_t)1 << c.tile_shift;
   const int64_t inflight = (int64_t)c.n_slots * T;
-  const int64_t fan = glass ? 8 : 1;  // glass splits at depths 0..2 give up to 8 rays per sample
-  const int64_t qcap = (fan + 1) * inflight;
-  const int64_t scap = 3 * fan * inflight;
+  const int64_t pad = 64 * ((int64_t)n_scenes + c.n_slots);
+  const int64_t qcap = (glass ? 17 : 2) * inflight + pad;
+  const int64_t scap = 3 * (nee_glass ? 16 : 1) * inflight + 64 * (int64_t)n_scenes;
   if (qcap >= ((int64_t)1 << 32) || scap >= ((int64_t)1 << 32)) return fail(PRT_E_INVALID, "queue capacity above 2^32");
   int rc;
   if (w.q_cap < qcap) {
@@ -129,11 +138,16 @@ int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, bool glass, i
   if ((rc = dev_alloc(w.slot_info, (size_t)c.n_slots * sizeof(SlotInfo)))) return rc;
   if ((rc = dev_alloc(w.live, (size_t)c.n_slots * sizeof(int32_t)))) return rc;
   if ((rc = dev_alloc(w.free_list, (size_t)c.n_slots * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(w.done_list, (size_t)c.n_slots * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(w.new_tiles, (size_t)c.n_slots * sizeof(NewTile)))) return rc;
   if ((rc = dev_alloc(w.scenes, (size_t)n_scenes * sizeof(Params)))) return rc;
   if ((rc = dev_alloc(w.tiles, (size_t)n_tiles * sizeof(TileDesc)))) return rc;
   if ((rc = dev_alloc(w.outs, (size_t)n_scenes * 2 * sizeof(float*)))) return rc;
+  if ((rc = dev_alloc(w.scene_fan, (size_t)n_scenes * sizeof(int32_t)))) return rc;
+  if ((rc = dev_alloc(w.seg, (size_t)n_scenes * 6 * sizeof(uint32_t)))) return rc;  // off, cap, cnt x 2 queues
+  if ((rc = dev_alloc(w.new_samp, (size_t)n_scenes * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(w.work, ((size_t)n_scenes + c.n_slots) * sizeof(WorkEntry)))) return rc;
+  if ((rc = dev_alloc(w.work_pref, ((size_t)n_scenes + c.n_slots + 1) * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(w.nee_pref, ((size_t)n_scenes + 1) * sizeof(uint32_t)))) return rc;
   const int64_t ovf_threads = (int64_t)std::max(1, lead->cu_count) * kMaxTravBlocksPerCu * 256;
   if ((rc = dev_alloc(w.ovf, (size_t)ovf_threads * 2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * sizeof(uint32_t))))
     return rc;
@@ -157,13 +171,17 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   const int64_t T = (int64_t)1 << c.tile_shift;
   // work queue: scene-major runs of T / spp pixels; every tile_stride-th tile
   std::vector<TileDesc> tiles;
-  bool glass = false, nee = false;
+  std::vector<int32_t> fan(n);
+  bool glass = false, nee = false, nee_glass = false;
   uint64_t samples = 0;
   int64_t tile_counter = 0;
   for (int i = 0; i < n; ++i) {
     const Params& P = scenes[i]->P;
+    const bool scene_nee = P.nee && P.n_lights > 0;
     glass |= scenes[i]->has_glass;
-    nee |= (P.nee && P.n_lights > 0);
+    nee |= scene_nee;
+    nee_glass |= scene_nee && scenes[i]->has_glass;
+    fan[i] = scenes[i]->has_glass ? 2 : 1;
     const int64_t npix = (int64_t)P.W * P.H;
     const int64_t per = std::max<int64_t>(1, T / P.spp);
     for (int64_t p = 0; p < npix; p += per, ++tile_counter) {
@@ -178,7 +196,7 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   }
   if (tiles.size() >= ((size_t)1 << 31)) return fail(PRT_E_INVALID, "too many tiles");
   int rc;
-  if ((rc = ensure_stream_workspace(lead, c, glass, n, (int64_t)tiles.size()))) return rc;
+  if ((rc = ensure_stream_workspace(lead, c, glass, nee_glass, n, (int64_t)tiles.size()))) return rc;
   StreamWorkspace& w = *lead->sws;
   hipStream_t s = lead->stream;
 
@@ -201,6 +219,7 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   HIPCHK(hipMemcpyAsync(w.scenes.p, params.data(), params.size() * sizeof(Params), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(w.tiles.p, tiles.data(), tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(w.outs.p, outs.data(), outs.size() * sizeof(float*), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(w.scene_fan.p, fan.data(), fan.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
 
   Stream S{};
   S.scenes = (const Params*)w.scenes.p;
@@ -210,8 +229,15 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   S.slot_info = (SlotInfo*)w.slot_info.p;
   S.live = (int32_t*)w.live.p;
   S.free_list = (uint32_t*)w.free_list.p;
-  S.done_list = (uint32_t*)w.done_list.p;
   S.new_tiles = (NewTile*)w.new_tiles.p;
+  S.scene_fan = (const int32_t*)w.scene_fan.p;
+  S.seg_off = (uint32_t*)w.seg.p;
+  S.seg_cap = S.seg_off + 2 * (size_t)n;
+  S.seg_cnt = S.seg_off + 4 * (size_t)n;
+  S.new_samp = (uint32_t*)w.new_samp.p;
+  S.work = (WorkEntry*)w.work.p;
+  S.work_pref = (uint32_t*)w.work_pref.p;
+  S.nee_pref = (uint32_t*)w.nee_pref.p;
   S.ctl = (StreamCtl*)w.ctl.p;
   S.accum = (uint64_t*)w.accum.p;
   S.ctr = (uint64_t*)w.ctr.p;
@@ -244,18 +270,28 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
   };
+  Timer tm;
+  // one iteration; with timing on, an event after each kernel (plain launches)
   auto iteration = [&](int cur) -> hipError_t {
     const int nxt = cur ^ 1;
     hipError_t e;
+    auto mark = [&](int cat) { return tm.mark(cat) == PRT_OK ? hipSuccess : hipErrorUnknown; };
     if ((e = launch_stream_refill(S, cur, s)) != hipSuccess) return e;
-    if ((e = launch_stream_gen(S, q[cur], work_blocks, s)) != hipSuccess) return e;
-    if ((e = launch_stream_extend(S, q[cur], cur, hit_t, hit_id, c.stack_lds, trav_blocks, s)) != hipSuccess) return e;
+    if ((e = launch_stream_gen(S, q[cur], cur, work_blocks, s)) != hipSuccess) return e;
+    if ((e = mark(T_OTHER)) != hipSuccess) return e;
+    if ((e = launch_stream_extend(S, q[cur], hit_t, hit_id, c.stack_lds, trav_blocks, s)) != hipSuccess) return e;
+    if ((e = mark(T_EXTEND)) != hipSuccess) return e;
     if ((e = launch_stream_shade(S, q[cur], hit_t, hit_id, q[nxt], nxt, work_blocks, s)) != hipSuccess) return e;
+    if ((e = mark(T_SHADE)) != hipSuccess) return e;
     if (nee) {
+      if ((e = launch_stream_scan(S, nxt, s)) != hipSuccess) return e;
       if ((e = launch_stream_nee(S, q[nxt], nxt, sq, work_blocks, s)) != hipSuccess) return e;
+      if ((e = mark(T_NEE)) != hipSuccess) return e;
       if ((e = launch_stream_shadow(S, sq, c.stack_lds, trav_blocks, s)) != hipSuccess) return e;
+      if ((e = mark(T_SHADOW)) != hipSuccess) return e;
     }
-    return launch_stream_resolve(S, cu * 2, s);
+    if ((e = launch_stream_resolve(S, cu * 2, s)) != hipSuccess) return e;
+    return mark(T_OTHER);
   };
 
   hipError_t he = hipEventRecord(e0, s);
@@ -265,6 +301,10 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   if (he != hipSuccess) {
     cleanup();
     return fail(PRT_E_HIP, std::string("stream init: ") + hipGetErrorString(he));
+  }
+  if ((rc = tm.begin(c.timing, s))) {
+    cleanup();
+    return rc;
   }
   if (c.use_graph) {
     he = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
@@ -321,6 +361,8 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   if (he == hipSuccess) he = hipStreamSynchronize(s);
   float ms = 0;
   if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+  double cat[T_NCAT] = {0, 0, 0, 0, 0, 0};
+  tm.collect(cat);
   cleanup();
   if (he != hipSuccess) return fail(PRT_E_HIP, std::string("batch completion: ") + hipGetErrorString(he));
   if (status == 2) return fail(PRT_E_INVALID, "batched render stalled: tiles in flight without rays");
@@ -335,11 +377,17 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
     stats->shadow_tri_tests = ctrh[CTR_STRIS];
     stats->pickup_hits = ctrh[CTR_PICKUP];
     stats->truncated_paths = ctrh[CTR_TRUNC];
+    stats->mixed_scene_passes = ctrh[CTR_MIXED];
     stats->camera_samples = samples;
     stats->primary_rays = final_ctl.total_prim;
     stats->extension_rays = final_ctl.total_cont;
     stats->shadow_rays = ctrh[CTR_SHADOW];
     stats->ms_total = ms;
+    stats->ms_extend = cat[T_EXTEND];
+    stats->ms_shade = cat[T_SHADE];
+    stats->ms_nee = cat[T_NEE];
+    stats->ms_shadow = cat[T_SHADOW];
+    stats->ms_other = cat[T_OTHER];
     stats->launches_extend = (int64_t)final_ctl.iter;
     stats->bytes_extend = 128 * ctrh[CTR_NODES] + 48 * ctrh[CTR_TRIS] + 48 * (final_ctl.total_prim + final_ctl.total_cont);
     stats->bytes_shadow = 128 * ctrh[CTR_SNODES] + 48 * ctrh[CTR_STRIS] + 48 * ctrh[CTR_SHADOW];

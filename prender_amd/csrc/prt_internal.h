@@ -73,6 +73,44 @@ struct Workspace {
   void release();
 };
 
+// Device-time spans between successive marks, summed per category.
+struct Timer {
+  bool on = false;
+  hipStream_t s = nullptr;
+  std::vector<std::pair<hipEvent_t, int>> marks;  // event + category of the span that ends here
+  hipEvent_t start{};
+  int begin(bool enable, hipStream_t st) {
+    on = enable;
+    s = st;
+    if (!on) return PRT_OK;
+    HIPCHK(hipEventCreate(&start));
+    HIPCHK(hipEventRecord(start, s));
+    return PRT_OK;
+  }
+  int mark(int cat) {
+    if (!on) return PRT_OK;
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(e, s));
+    marks.push_back({e, cat});
+    return PRT_OK;
+  }
+  // after the stream has completed
+  void collect(double* cat_ms) {
+    if (!on) return;
+    hipEvent_t prev = start;
+    for (auto& m : marks) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, prev, m.first) == hipSuccess) cat_ms[m.second] += ms;
+      prev = m.first;
+    }
+    for (auto& m : marks) (void)hipEventDestroy(m.first);
+    (void)hipEventDestroy(start);
+    marks.clear();
+  }
+};
+enum { T_EXTEND = 0, T_SHADE = 1, T_SHADOW = 2, T_OTHER = 3, T_EXTEND0 = 4, T_NEE = 5, T_NCAT = 6 };
+
 // Workspace of the multi-scene streaming renderer (prt_batch.cpp).
 struct StreamWorkspace;
 void release_stream_workspace(StreamWorkspace* w);

@@ -14,6 +14,7 @@ enum {
   CTR_PICKUP = 4,  // specular pickups taken from a child hit
   CTR_TRUNC = 5,   // children dropped by the vertex cap
   CTR_DROP = 6,    // appends beyond a queue's capacity (must stay 0)
+  CTR_MIXED = 7,   // batched renders: extra passes of waves holding rays of several scenes
   CTR_SNODES = 8,  // BVH nodes fetched by shadow rays
   CTR_STRIS = 9,   // triangle tests of shadow rays
   CTR_REPLAY_MISMATCH = 10,  // adjoint: samples whose replayed radiance differs from the primal

@@ -158,6 +158,8 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
   ShadeCtx C;
   C.qout = qout;
   C.count_out = count_out;
+  C.off = 0;
+  C.cap = qout.cap;
   C.truncated = C.n_ext = C.pickups = C.dropped = 0;
   double ge[3] = {0.0, 0.0, 0.0};
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {

@@ -29,9 +29,11 @@ __device__ __forceinline__ void camera_ray(const Params& P, int x, int y, int sx
   d = normalize(sp - o);
 }
 
+// Children go to qout positions off + j, j < cap, with j from *count_out.
 struct ShadeCtx {
   RayQ qout;
   uint32_t* count_out;
+  uint32_t off, cap;
   uint32_t slot;
   uint32_t truncated, n_ext, pickups, dropped;
 };
@@ -48,11 +50,12 @@ struct VertexAdj {
 };
 
 __device__ __forceinline__ void emit_child(ShadeCtx& C, d3 o, d3 d, d3 ct, d3 pw, uint32_t meta, d3 nrm) {
-  const uint32_t j = agg_append(C.count_out);
-  if (j >= C.qout.cap) {
+  uint32_t j = agg_append(C.count_out);
+  if (j >= C.cap) {
     C.dropped++;
     return;
   }
+  j += C.off;
   C.qout.ox[j] = o.x; C.qout.oy[j] = o.y; C.qout.oz[j] = o.z;
   C.qout.dx[j] = d.x; C.qout.dy[j] = d.y; C.qout.dz[j] = d.z;
   C.qout.tr[j] = (float)ct.x; C.qout.tg[j] = (float)ct.y; C.qout.tb[j] = (float)ct.z;

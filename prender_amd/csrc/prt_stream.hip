@@ -1,14 +1,14 @@
 // prt_stream.hip — gfx950 kernels of the multi-scene streaming renderer
 // (the iteration structure is described in prt_stream.h).
 //
-// Rays of different scenes share the queues.  A wave's rays usually come from
-// one tile (refill writes tiles contiguously and the appends keep a wave's
-// children together), so the kernels run each distinct scene of a wave in
-// turn ("waterfall") with that scene's parameters at a wave-uniform address.
-// The per-vertex arithmetic is the per-scene renderer's (prt_shade.h), and a
-// camera sample's random numbers depend only on (pixel, sample, depth,
-// branch), so a tile renders bit-identically to the same pixels of a
-// per-scene render.
+// Extend, shade and NEE walk padded work tables whose entries are 64-aligned
+// runs of one scene's rays, so a wave's scene is uniform and its parameters
+// are read with scalar loads.  The shadow queue is shared by all scenes (each
+// wave appends its shadow rays contiguously); its kernel runs each distinct
+// scene of a wave in turn.  The per-vertex arithmetic is the per-scene
+// renderer's (prt_shade.h), and a camera sample's random numbers depend only
+// on (pixel, sample, depth, branch), so a tile renders bit-identically to the
+// same pixels of a per-scene render.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -19,37 +19,66 @@
 #include "prt_stream.h"
 #include "prt_trace.h"
 
+__device__ __forceinline__ uint32_t round64(uint32_t n) { return (n + 63u) & ~63u; }
+
+// Parameters of scene s (s wave-uniform) through a constant-address-space
+// pointer, so that their loads are scalar loads as kernel arguments' are.
+__device__ __forceinline__ const Params& scene_params(const Params* scenes, int s) {
+  const uint64_t a = (uint64_t)(scenes + s);
+  const uint64_t au = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
+                      (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+  const __attribute__((address_space(4))) Params* Pc = (const __attribute__((address_space(4))) Params*)au;
+  return *(const Params*)Pc;
+}
+
+// Entry e of a padded prefix table with pref[e] <= x < pref[e + 1]
+// (pref[0] = 0, pref[n] > x); x wave-uniform.
+__device__ __forceinline__ uint32_t find_entry(const uint32_t* __restrict__ pref, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pref[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // Calls f(scenes[s]) for the lanes whose scene is s, one distinct s at a time
-// (s wave-uniform); lanes with sid < 0 take no part.  Every lane of the wave
-// must reach this call.
+// (s wave-uniform); lanes with sid < 0 take no part.  Returns the number of
+// passes (distinct scenes).  Every lane of the wave must reach this call.
 template <class F>
-__device__ __forceinline__ void per_scene(const Params* __restrict__ scenes, int sid, F&& f) {
+__device__ __forceinline__ uint32_t per_scene(const Params* __restrict__ scenes, int sid, F&& f) {
   bool pend = sid >= 0;
   uint64_t m = __ballot(pend);
+  uint32_t passes = 0;
   while (m) {
+    ++passes;
     const int leader = __ffsll((unsigned long long)m) - 1;
     const int s = __builtin_amdgcn_readfirstlane(__shfl(sid, leader, 64));
-    // the address is formed from the uniform s before the lane test (so the
-    // compiler cannot substitute the per-lane sid, equal inside the branch)
-    // and lives in the constant address space: the parameters are read
-    // with scalar loads, as kernel arguments are
-    const uint64_t a = (uint64_t)(scenes + s);
-    const uint64_t au = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
-                        (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
-    const __attribute__((address_space(4))) Params* Pc = (const __attribute__((address_space(4))) Params*)au;
+    // the parameters' address is formed from the uniform s before the lane
+    // test, so the compiler cannot substitute the per-lane sid (equal inside
+    // the branch) and the loads stay scalar
+    const Params& P = scene_params(scenes, s);
     if (pend && sid == s) {
-      f(*(const Params*)Pc);
+      f(P);
       pend = false;
     }
     m = __ballot(pend);
   }
+  return passes;
+}
+
+// Extra waterfall passes of a wave (waves holding rays of several scenes),
+// summed per wave and added to the counter at the end of a kernel.
+__device__ __forceinline__ void flush_mixed(uint64_t* ctr, uint32_t extra) {
+  if (lane_id() == 0 && extra) atomicAdd((unsigned long long*)&ctr[CTR_MIXED], (unsigned long long)extra);
 }
 
 // Adds delta to the live-ray count of tile slot ts (one atomic per distinct
-// slot of the wave).  The update that brings a count to zero queues the slot
-// for resolve: a ray's own retirement and the children it spawned are one
-// update, so a count cannot reach zero while any ray of its tile exists.
-// Every lane of the wave must reach this call.
+// slot of the wave, without a return value, so no wave waits on it).  A ray's
+// own retirement and the children it spawned are one update, so a count is
+// zero at the end of an iteration exactly when no ray of its tile is left;
+// resolve looks for those slots.  Every lane of the wave must reach this call.
 __device__ __forceinline__ void live_update(const Stream& st, int ts, int delta) {
   bool pend = ts >= 0 && delta != 0;
   uint64_t m = __ballot(pend);
@@ -58,85 +87,107 @@ __device__ __forceinline__ void live_update(const Stream& st, int ts, int delta)
     const int t0 = __builtin_amdgcn_readfirstlane(__shfl(ts, leader, 64));
     const bool mine = pend && ts == t0;
     const int sum = (int)(uint32_t)wave_sum_u32((uint32_t)(mine ? delta : 0));
-    if ((int)lane_id() == leader && sum != 0) {
-      const int old = atomicAdd(&st.live[t0], sum);
-      if (old + sum == 0) {
-        const uint32_t k = atomicAdd(&st.ctl->done_count, 1u);
-        if (k < (uint32_t)st.n_slots) st.done_list[k] = (uint32_t)t0;  // a slot finishes once per iteration
-      }
-    }
+    if ((int)lane_id() == leader && sum != 0) (void)atomicAdd(&st.live[t0], sum);
     pend = pend && !mine;
     m = __ballot(pend);
   }
 }
 
+// Exclusive prefix of val(i), i < n, over one 256-thread block: each thread
+// owns a contiguous chunk [b, e) and gets the prefix at b; every thread gets
+// the total.  Every thread of the block must call it.
+struct ChunkScan {
+  uint32_t b, e, base, total;
+};
+template <class F>
+__device__ __forceinline__ ChunkScan block_scan(uint32_t n, F val, uint32_t* s_part /* [257] */) {
+  ChunkScan r;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t chunk = (n + 255u) / 256u;
+  r.b = min(tid * chunk, n);
+  r.e = min(r.b + chunk, n);
+  uint32_t sum = 0;
+  for (uint32_t i = r.b; i < r.e; ++i) sum += val(i);
+  s_part[tid] = sum;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t acc = 0;
+    for (int t = 0; t < 256; ++t) {
+      const uint32_t v = s_part[t];
+      s_part[t] = acc;
+      acc += v;
+    }
+    s_part[256] = acc;
+  }
+  __syncthreads();
+  r.base = s_part[tid];
+  r.total = s_part[256];
+  __syncthreads();
+  return r;
+}
+
 __global__ void k_stream_init(Stream st) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  for (int i = gid; i < st.n_slots; i += gridDim.x * blockDim.x) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = gid; i < st.n_slots; i += stride) {
     st.free_list[i] = (uint32_t)i;
     st.live[i] = 0;
+    st.slot_info[i].tile = -1;  // free
+  }
+  for (int i = gid; i < 2 * st.n_scenes; i += stride) {
+    st.seg_off[i] = 0;
+    st.seg_cap[i] = 0;
+    st.seg_cnt[i] = 0;
   }
   if (gid == 0) {
     StreamCtl* c = st.ctl;
     c->next_tile = 0;
     c->free_count = (uint32_t)st.n_slots;
-    c->done_count = 0;
     c->n_new = c->n_cont = c->n_prim = 0;
     c->status = 0;
     c->iter = 0;
-    c->cnt[0] = c->cnt[1] = 0;
     c->cnt_sq = 0;
     c->fetch[0] = c->fetch[1] = 0;
-    c->pad_ = 0;
+    c->region_end[0] = c->region_end[1] = 0;
+    c->work_width = c->cont_width = c->nee_width = c->n_work = 0;
     c->total_prim = c->total_cont = 0;
   }
 }
 
-// One block: hand the free slots to the next tiles of the work queue and lay
-// out their camera rays after the carried-over rays of queue `cur`; reset the
-// per-iteration counters.
+// One block: hand the free slots to the next tiles of the work queue, lay out
+// the next queue's scene regions and build the current queue's work table;
+// reset the per-iteration counters.
 __global__ __launch_bounds__(256) void k_stream_refill(Stream st, int cur) {
-  __shared__ uint32_t s_part[256];
-  __shared__ uint32_t s_hdr[5];
+  __shared__ uint32_t s_part[257];
+  __shared__ uint32_t s_hdr[4];
   StreamCtl* c = st.ctl;
+  const int nxt = cur ^ 1;
   const uint32_t tid = threadIdx.x;
+  const uint32_t NS = (uint32_t)st.n_scenes;
   if (tid == 0) {
     const uint32_t free_n = min(c->free_count, (uint32_t)st.n_slots), next = c->next_tile;
-    uint32_t n_cont = c->cnt[cur];
-    if (n_cont > st.q_cap) n_cont = st.q_cap;  // only after an overflow, which the drop counter reports
     const uint32_t left = (uint32_t)st.n_tiles - next;
     s_hdr[0] = free_n < left ? free_n : left;
-    s_hdr[1] = n_cont;
-    s_hdr[2] = free_n;
-    s_hdr[3] = next;
+    s_hdr[1] = free_n;
+    s_hdr[2] = next;
+    s_hdr[3] = c->region_end[cur];  // the primary region of the current queue starts here
   }
+  for (uint32_t s = tid; s < NS; s += 256) st.new_samp[s] = 0;
   __syncthreads();
-  const uint32_t n_new = s_hdr[0], n_cont = s_hdr[1], free_n = s_hdr[2], next = s_hdr[3];
-  const uint32_t chunk = (n_new + 255u) / 256u;
-  const uint32_t b = min(tid * chunk, n_new), e = min(b + chunk, n_new);
-  uint32_t rays = 0;
-  for (uint32_t k = b; k < e; ++k) {
+  const uint32_t n_want = s_hdr[0], free_n = s_hdr[1], next = s_hdr[2], prim_base = s_hdr[3];
+
+  // 1. camera rays of the candidate tiles (64-aligned per tile)
+  auto tile_rays = [&](uint32_t k) {
     const TileDesc td = st.tiles[next + k];
     const Params& P = st.scenes[td.scene];
-    rays += (uint32_t)td.n_pixels * (uint32_t)(P.SS * P.SS);
-  }
-  s_part[tid] = rays;
-  __syncthreads();
-  if (tid == 0) {
-    uint32_t acc = 0;
-    for (int i = 0; i < 256; ++i) {
-      const uint32_t v = s_part[i];
-      s_part[i] = acc;
-      acc += v;
-    }
-    s_hdr[4] = acc;
-  }
-  __syncthreads();
-  const uint32_t n_prim = s_hdr[4];
-  const bool fits = (uint64_t)n_cont + n_prim <= st.q_cap;
-  if (fits) {
-    uint32_t base = s_part[tid];
-    for (uint32_t k = b; k < e; ++k) {
+    return round64((uint32_t)td.n_pixels * (uint32_t)(P.SS * P.SS));
+  };
+  const ChunkScan tr = block_scan(n_want, tile_rays, s_part);
+  const bool fits_prim = (uint64_t)prim_base + tr.total <= st.q_cap;
+  const uint32_t n_new = fits_prim ? n_want : 0u;
+  if (fits_prim) {
+    uint32_t base = tr.base;
+    for (uint32_t k = tr.b; k < tr.e; ++k) {
       const uint32_t tile = next + k;
       const TileDesc td = st.tiles[tile];
       const Params& P = st.scenes[td.scene];
@@ -146,159 +197,238 @@ __global__ __launch_bounds__(256) void k_stream_refill(Stream st, int cur) {
       nt.ray_base = base;
       nt.n_rays = (uint32_t)td.n_pixels * (uint32_t)(P.SS * P.SS);
       nt.n_samples = (uint32_t)td.n_pixels * (uint32_t)P.spp;
+      nt.spp = (uint32_t)P.spp;
+      nt.ss2 = (uint32_t)(P.SS * P.SS);
+      nt.scene = td.scene;
+      nt.pad_ = 0;
       st.new_tiles[k] = nt;
       SlotInfo si;
       si.scene = td.scene;
       si.tile = (int32_t)tile;
       si.pixel_base = td.pixel_base;
+      si.spp = P.spp;
+      si.pad_ = 0;
       st.slot_info[slot] = si;
       st.live[slot] = (int32_t)nt.n_samples;
-      base += nt.n_rays;
+      atomicAdd(&st.new_samp[td.scene], nt.n_samples);
+      base += round64(nt.n_rays);
     }
   }
+  __syncthreads();
+
+  // 2. scene regions of the next queue: bound = fan * (carried-over rays +
+  //    new camera samples), the vertices this iteration shades
+  const uint32_t* cnt_cur = st.seg_cnt + cur * NS;
+  const uint32_t* cap_cur = st.seg_cap + cur * NS;
+  auto bound = [&](uint32_t s) {
+    const uint32_t cnt = min(cnt_cur[s], cap_cur[s]);
+    return round64((uint32_t)st.scene_fan[s] * (cnt + st.new_samp[s]));
+  };
+  const ChunkScan rb = block_scan(NS, bound, s_part);
+  // regions past the queue's end are never handed out: with no room, every
+  // region gets capacity 0 (shade then drops and counts the children, and the
+  // status below reports the overflow)
+  const bool fits_next = rb.total <= st.q_cap;
+  {
+    uint32_t base = rb.base;
+    for (uint32_t s = rb.b; s < rb.e; ++s) {
+      const uint32_t w = bound(s);
+      st.seg_off[nxt * NS + s] = fits_next ? base : 0u;
+      st.seg_cap[nxt * NS + s] = fits_next ? w : 0u;
+      st.seg_cnt[nxt * NS + s] = 0;
+      base += w;
+    }
+  }
+
+  // 3. work table of the current queue: the scene regions, then the new tiles
+  const uint32_t n_work = NS + n_new;
+  auto width = [&](uint32_t e) {
+    if (e < NS) return round64(min(cnt_cur[e], cap_cur[e]));
+    return round64(st.new_tiles[e - NS].n_rays);
+  };
+  const ChunkScan wt = block_scan(n_work, width, s_part);
+  {
+    uint32_t base = wt.base;
+    for (uint32_t e = wt.b; e < wt.e; ++e) {
+      WorkEntry we;
+      if (e < NS) {
+        we.off = st.seg_off[cur * NS + e];
+        we.cnt = min(cnt_cur[e], cap_cur[e]);
+        we.scene = (int32_t)e;
+      } else {
+        const NewTile nt = st.new_tiles[e - NS];
+        we.off = prim_base + nt.ray_base;
+        we.cnt = nt.n_rays;
+        we.scene = nt.scene;
+      }
+      we.pad_ = 0;
+      st.work[e] = we;
+      st.work_pref[e] = base;
+      base += round64(we.cnt);
+    }
+    if (tid == 255) st.work_pref[n_work] = wt.total;
+  }
+  // carried-over rays (sum of the scene regions' counts)
+  auto cont = [&](uint32_t s) { return min(cnt_cur[s], cap_cur[s]); };
+  const ChunkScan cs = block_scan(NS, cont, s_part);
   if (tid == 0) {
-    const uint32_t taken = fits ? n_new : 0u;
-    const uint32_t prim = fits ? n_prim : 0u;
-    c->n_new = taken;
+    const uint32_t n_cont = cs.total;
+    const uint32_t cont_width = wt.total - (fits_prim ? tr.total : 0u);  // = work_pref[n_scenes]
+    c->n_new = n_new;
     c->n_cont = n_cont;
-    c->n_prim = prim;
-    c->free_count = free_n - taken;
-    c->next_tile = next + taken;
-    c->cnt[cur] = n_cont + prim;
-    c->cnt[cur ^ 1] = 0;
+    c->n_prim = 0;  // counted by gen into total_prim
+    c->free_count = free_n - n_new;
+    c->next_tile = next + n_new;
+    c->region_end[nxt] = fits_next ? rb.total : 0u;
+    c->work_width = wt.total;
+    c->cont_width = cont_width;
+    c->n_work = n_work;
     c->cnt_sq = 0;
     c->fetch[0] = c->fetch[1] = 0;
-    c->done_count = 0;
-    c->total_prim += prim;
     c->total_cont += n_cont;
     c->iter += 1;
     if (c->status < 2) {
-      if (!fits) c->status = 3;
-      else if (n_cont == 0 && taken == 0)
+      if (!fits_prim || !fits_next) c->status = 3;
+      else if (n_cont == 0 && n_new == 0)
         c->status = (next == (uint32_t)st.n_tiles && free_n == (uint32_t)st.n_slots) ? 1u : 2u;
       else c->status = 0;
     }
   }
 }
 
-// Unique sub-pixel camera rays of the new tiles (one wave per tile).
-__global__ __launch_bounds__(256) void k_stream_gen(Stream st, RayQ q) {
-  const uint32_t n_new = st.ctl->n_new, n_cont = st.ctl->n_cont;
+// Unique sub-pixel camera rays of the new tiles (one wave per tile), in the
+// primary region of queue `cur`.
+__global__ __launch_bounds__(256) void k_stream_gen(Stream st, RayQ q, int cur) {
+  const uint32_t n_new = st.ctl->n_new, prim_base = st.ctl->region_end[cur];
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  uint32_t rays = 0;
   for (uint32_t k = wave; k < n_new; k += n_waves) {
     const NewTile nt = st.new_tiles[k];
     const SlotInfo si = st.slot_info[nt.slot];
-    const Params& P = st.scenes[si.scene];
-    const uint32_t ss2 = (uint32_t)(P.SS * P.SS);
+    const Params& P = scene_params(st.scenes, nt.scene);
+    const uint32_t ss2 = nt.ss2;
     for (uint32_t r = lane_id(); r < nt.n_rays; r += 64) {
       const int64_t pix = si.pixel_base + r / ss2;
       const int sub = (int)(r % ss2);
       d3 o, d;
       camera_ray(P, (int)(pix % P.W), (int)(pix / P.W), sub % P.SS, sub / P.SS, o, d);
-      const uint32_t i = n_cont + nt.ray_base + r;
+      const uint32_t i = prim_base + nt.ray_base + r;
       q.ox[i] = o.x; q.oy[i] = o.y; q.oz[i] = o.z;
       q.dx[i] = d.x; q.dy[i] = d.y; q.dz[i] = d.z;
       q.slot[i] = nt.slot << st.tile_shift;
       q.meta[i] = 0u;
     }
+    if (lane_id() == 0) rays += nt.n_rays;
   }
+  if (lane_id() == 0 && rays) atomicAdd((unsigned long long*)&st.ctl->total_prim, (unsigned long long)rays);
 }
 
-// Closest hit for the whole current queue.
+// Closest hit for the current queue: waves fetch 64-wide chunks of the work
+// table's padded index space; a chunk is one scene's rays.
 template <int NL>
-__global__ __launch_bounds__(256) void k_stream_extend(Stream st, RayQ q, int cur, double* __restrict__ hit_t,
+__global__ __launch_bounds__(256) void k_stream_extend(Stream st, RayQ q, double* __restrict__ hit_t,
                                                        int32_t* __restrict__ hit_id) {
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL);
-  const uint32_t n = min(st.ctl->cnt[cur], q.cap);
+  const uint32_t n = st.ctl->work_width, n_work = st.ctl->n_work;
   uint32_t* fetch = &st.ctl->fetch[0];
   TravStats ts{0, 0};
   while (true) {
     uint32_t base = 0;
     if (lane_id() == 0) base = atomicAdd(fetch, 64u);
-    base = (uint32_t)__shfl((int)base, 0, 64);
+    base = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)base, 0, 64));
     if (base >= n) break;
-    const uint32_t i = base + lane_id();
-    const int sid = i < n ? st.slot_info[q.slot[i] >> st.tile_shift].scene : -1;
-    per_scene(st.scenes, sid, [&](const Params& P) {
+    const uint32_t e = find_entry(st.work_pref, n_work, base);
+    const WorkEntry we = st.work[e];
+    const uint32_t local = base - st.work_pref[e] + lane_id();
+    const Params& P = scene_params(st.scenes, we.scene);
+    if (local < we.cnt) {
+      const uint32_t i = we.off + local;
       double t, u, v;
       int32_t id;
       closest_hit(P, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), t, id, u, v, S, ts);
       hit_t[i] = t;
       hit_id[i] = id;
-    });
+    }
   }
   flush_stats(st.ctr, CTR_NODES, ts);
 }
 
-// One vertex per carried-over ray and per camera sample of the new tiles.
+// One vertex per carried-over ray (scene regions of queue cur, through the
+// work table) and per camera sample of the new tiles (T padded entries per
+// tile); children go to their scene's region of queue nxt.
 __global__ __launch_bounds__(256) void k_stream_shade(Stream st, RayQ qin, const double* __restrict__ hit_t,
                                                       const int32_t* __restrict__ hit_id, RayQ qout, int nxt) {
-  const uint32_t n_cont = st.ctl->n_cont, n_new = st.ctl->n_new;
+  const uint32_t cont_width = st.ctl->cont_width, n_new = st.ctl->n_new;
+  const uint32_t prim_base = st.ctl->region_end[nxt ^ 1];
+  const uint32_t NS = (uint32_t)st.n_scenes;
   const uint32_t shift = (uint32_t)st.tile_shift, tmask = (1u << shift) - 1u;
-  const uint32_t total = n_cont + (n_new << shift);
-  const uint32_t n_round = (total + 63u) & ~63u;
-  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t total = cont_width + (n_new << shift);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
   ShadeCtx C;
   C.qout = qout;
-  C.count_out = &st.ctl->cnt[nxt];
   C.truncated = C.n_ext = C.pickups = C.dropped = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
-    int sid = -1;
-    uint32_t slot = 0;
-    bool prim = false;
-    NewTile nt{0, 0, 0, 0};
-    if (i < n_cont) {
-      slot = qin.slot[i];
-      sid = st.slot_info[slot >> shift].scene;
-    } else if (i < total) {
-      const uint32_t j = i - n_cont;
-      nt = st.new_tiles[j >> shift];
-      const uint32_t r = j & tmask;
-      if (r < nt.n_samples) {
-        prim = true;
-        slot = (nt.slot << shift) | r;
-        sid = st.slot_info[nt.slot].scene;
-      }
-    }
-    d3 add = mk(0.0, 0.0, 0.0);
-    int delta = 0;
-    per_scene(st.scenes, sid, [&](const Params& P) {
-      const uint32_t r = slot & tmask;
-      const uint32_t pl = r / (uint32_t)P.spp, k = r % (uint32_t)P.spp;
-      const int64_t pixel = st.slot_info[slot >> shift].pixel_base + pl;
-      d3 o, d, T, Pw;
-      int depth;
-      uint32_t branch, ray;
-      if (prim) {
-        const uint32_t ss2 = (uint32_t)(P.SS * P.SS);
-        ray = n_cont + nt.ray_base + pl * ss2 + k % ss2;
-        T = mk(1.0, 1.0, 1.0);
-        Pw = mk(0.0, 0.0, 0.0);
-        depth = 0;
-        branch = 1;
-      } else {
-        ray = i;
-        T = mk(qin.tr[i], qin.tg[i], qin.tb[i]);
-        Pw = mk(qin.pr[i], qin.pg[i], qin.pb[i]);
-        const uint32_t meta = qin.meta[i];
+  for (uint32_t base = wave * 64u; base < total; base += n_waves * 64u) {
+    bool valid;
+    int scene;
+    uint32_t slot = 0, ray = 0, k = 0, branch = 1;
+    int depth = 0;
+    int64_t pixel = 0;
+    d3 T = mk(1.0, 1.0, 1.0), Pw = mk(0.0, 0.0, 0.0);
+    if (base < cont_width) {
+      const uint32_t e = find_entry(st.work_pref, NS, base);
+      const WorkEntry we = st.work[e];
+      const uint32_t local = base - st.work_pref[e] + lane_id();
+      scene = we.scene;
+      valid = local < we.cnt;
+      if (valid) {
+        ray = we.off + local;
+        slot = qin.slot[ray];
+        T = mk(qin.tr[ray], qin.tg[ray], qin.tb[ray]);
+        Pw = mk(qin.pr[ray], qin.pg[ray], qin.pb[ray]);
+        const uint32_t meta = qin.meta[ray];
         depth = (int)(meta & 0xFFu);
         branch = META_BRANCH(meta);
+        const SlotInfo si = st.slot_info[slot >> shift];
+        const uint32_t r = slot & tmask;
+        k = r % (uint32_t)si.spp;
+        pixel = si.pixel_base + r / (uint32_t)si.spp;
       }
-      o = mk(qin.ox[ray], qin.oy[ray], qin.oz[ray]);
-      d = mk(qin.dx[ray], qin.dy[ray], qin.dz[ray]);
+    } else {
+      // T >= 64 and chunks are 64-aligned: the chunk lies in one tile
+      const uint32_t j = base - cont_width;
+      const NewTile nt = st.new_tiles[j >> shift];
+      const uint32_t r = (j & tmask) + lane_id();
+      scene = nt.scene;
+      valid = r < nt.n_samples;
+      if (valid) {
+        slot = (nt.slot << shift) | r;
+        const uint32_t pl = r / nt.spp;
+        k = r % nt.spp;
+        ray = prim_base + nt.ray_base + pl * nt.ss2 + k % nt.ss2;
+        pixel = st.slot_info[nt.slot].pixel_base + pl;
+      }
+    }
+    scene = __builtin_amdgcn_readfirstlane(scene);
+    const Params& P = scene_params(st.scenes, scene);
+    C.count_out = &st.seg_cnt[nxt * NS + scene];
+    C.off = st.seg_off[nxt * NS + scene];
+    C.cap = st.seg_cap[nxt * NS + scene];
+    d3 add = mk(0.0, 0.0, 0.0);
+    int delta = 0;
+    if (valid) {
       C.slot = slot;
       const uint32_t e0 = C.n_ext;
       VertexAdj va;
-      add = shade_vertex(P, C, o, d, hit_t[ray], hit_id[ray], T, Pw, (uint32_t)pixel, k, depth, branch, va);
+      add = shade_vertex(P, C, mk(qin.ox[ray], qin.oy[ray], qin.oz[ray]), mk(qin.dx[ray], qin.dy[ray], qin.dz[ray]),
+                         hit_t[ray], hit_id[ray], T, Pw, (uint32_t)pixel, k, depth, branch, va);
       delta = (int)(C.n_ext - e0) - 1;
-    });
-    if (sid >= 0) {
       accum_add(&st.accum[3 * (size_t)slot + 0], add.x);
       accum_add(&st.accum[3 * (size_t)slot + 1], add.y);
       accum_add(&st.accum[3 * (size_t)slot + 2], add.z);
     }
-    live_update(st, sid >= 0 ? (int)(slot >> shift) : -1, delta);
+    live_update(st, valid ? (int)(slot >> shift) : -1, delta);
   }
   const uint64_t s_pick = wave_sum_u32(C.pickups), s_tr = wave_sum_u32(C.truncated);
   const uint64_t s_ext = wave_sum_u32(C.n_ext), s_drop = wave_sum_u32(C.dropped);
@@ -310,45 +440,66 @@ __global__ __launch_bounds__(256) void k_stream_shade(Stream st, RayQ qin, const
   }
 }
 
-// Light samples of the diffuse vertices behind the rays of queue `nxt`.
+// One block: padded prefix of the scene regions of queue nxt (NEE's input).
+__global__ __launch_bounds__(256) void k_stream_scan(Stream st, int nxt) {
+  __shared__ uint32_t s_part[257];
+  const uint32_t NS = (uint32_t)st.n_scenes;
+  const uint32_t* cnt = st.seg_cnt + nxt * NS;
+  const uint32_t* cap = st.seg_cap + nxt * NS;
+  auto width = [&](uint32_t s) { return round64(min(cnt[s], cap[s])); };
+  const ChunkScan sc = block_scan(NS, width, s_part);
+  uint32_t base = sc.base;
+  for (uint32_t s = sc.b; s < sc.e; ++s) {
+    st.nee_pref[s] = base;
+    base += width(s);
+  }
+  if (threadIdx.x == 255) st.nee_pref[NS] = sc.total;
+  if (threadIdx.x == 0) st.ctl->nee_width = sc.total;
+}
+
+// Light samples of the diffuse vertices behind the rays of queue nxt.
 __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, ShadowQ sq) {
-  const uint32_t n = min(st.ctl->cnt[nxt], q.cap);
-  const uint32_t n_round = (n + 63u) & ~63u;
-  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t width = st.ctl->nee_width;
+  const uint32_t NS = (uint32_t)st.n_scenes;
   const uint32_t shift = (uint32_t)st.tile_shift, tmask = (1u << shift) - 1u;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
   uint32_t emitted = 0, dropped = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
+  for (uint32_t base = wave * 64u; base < width; base += n_waves * 64u) {
+    const uint32_t s = find_entry(st.nee_pref, NS, base);
+    const uint32_t local = base - st.nee_pref[s] + lane_id();
+    const uint32_t cnt = min(st.seg_cnt[nxt * NS + s], st.seg_cap[nxt * NS + s]);
+    const Params& P = scene_params(st.scenes, (int)s);
     ShadowRay s0, s1, s2;
     bool v0 = false, v1 = false, v2 = false;
     int32_t l0 = -1, l1 = -1, l2 = -1;
-    uint32_t slot = 0, meta = 0;
-    int sid = -1;
-    if (i < n) {
-      meta = q.meta[i];
+    uint32_t slot = 0;
+    if (local < cnt) {
+      const uint32_t i = st.seg_off[nxt * NS + s] + local;
+      const uint32_t meta = q.meta[i];
       if (meta & META_NEE) {
         slot = q.slot[i];
-        sid = st.slot_info[slot >> shift].scene;
+        const SlotInfo si = st.slot_info[slot >> shift];
+        const uint32_t r = slot & tmask;
+        const uint32_t k = r % (uint32_t)si.spp;
+        const int64_t pixel = si.pixel_base + r / (uint32_t)si.spp;
+        nee_vertex(P, (uint32_t)pixel, k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta), mk(q.ox[i], q.oy[i], q.oz[i]),
+                   mk(q.nx[i], q.ny[i], q.nz[i]), mk(q.tr[i], q.tg[i], q.tb[i]), s0, s1, s2, v0, v1, v2, l0, l1,
+                   l2);
       }
     }
-    per_scene(st.scenes, sid, [&](const Params& P) {
-      const uint32_t r = slot & tmask;
-      const uint32_t pl = r / (uint32_t)P.spp, k = r % (uint32_t)P.spp;
-      const int64_t pixel = st.slot_info[slot >> shift].pixel_base + pl;
-      nee_vertex(P, (uint32_t)pixel, k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta), mk(q.ox[i], q.oy[i], q.oz[i]),
-                 mk(q.nx[i], q.ny[i], q.nz[i]), mk(q.tr[i], q.tg[i], q.tb[i]), s0, s1, s2, v0, v1, v2, l0, l1, l2);
-    });
-    uint32_t cnt = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
-    uint32_t j = wave_append_n(&st.ctl->cnt_sq, cnt);
-    if (j + cnt > sq.cap) {
-      dropped += cnt;
-      cnt = 0;
+    uint32_t n_sh = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
+    uint32_t j = wave_append_n(&st.ctl->cnt_sq, n_sh);
+    if (j + n_sh > sq.cap) {
+      dropped += n_sh;
+      n_sh = 0;
     } else {
       if (v0) put_shadow(sq, j++, s0, slot, l0);
       if (v1) put_shadow(sq, j++, s1, slot, l1);
       if (v2) put_shadow(sq, j++, s2, slot, l2);
-      emitted += cnt;
+      emitted += n_sh;
     }
-    live_update(st, cnt > 0 ? (int)(slot >> shift) : -1, (int)cnt);
+    live_update(st, n_sh > 0 ? (int)(slot >> shift) : -1, (int)n_sh);
   }
   const uint64_t s_sh = wave_sum_u32(emitted), s_drop = wave_sum_u32(dropped);
   if (lane_id() == 0) {
@@ -357,7 +508,8 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
   }
 }
 
-// Any-hit tests of the shadow queue; unoccluded rays add their contribution.
+// Any-hit tests of the shared shadow queue; unoccluded rays add their
+// contribution.
 template <int NL>
 __global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq) {
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL);
@@ -365,6 +517,7 @@ __global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq) {
   uint32_t* fetch = &st.ctl->fetch[1];
   const uint32_t shift = (uint32_t)st.tile_shift;
   TravStats ts{0, 0};
+  uint32_t extra = 0;
   while (true) {
     uint32_t base = 0;
     if (lane_id() == 0) base = atomicAdd(fetch, 64u);
@@ -373,14 +526,19 @@ __global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq) {
     const uint32_t i = base + lane_id();
     uint32_t slot = 0;
     int sid = -1;
+    d3 o = mk(0.0, 0.0, 0.0), d = o;
+    double tmax = 0.0;
     if (i < n) {
       slot = sq.slot[i];
+      o = mk(sq.ox[i], sq.oy[i], sq.oz[i]);
+      d = mk(sq.dx[i], sq.dy[i], sq.dz[i]);
+      tmax = sq.tmax[i];
       sid = st.slot_info[slot >> shift].scene;
     }
     bool lit = false;
-    per_scene(st.scenes, sid, [&](const Params& P) {
-      lit = !occluded(P, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), sq.tmax[i], S, ts);
-    });
+    const uint32_t passes =
+        per_scene(st.scenes, sid, [&](const Params& P) { lit = !occluded(P, o, d, tmax, S, ts); });
+    extra += passes > 1 ? passes - 1 : 0;
     if (lit) {
       accum_add(&st.accum[3 * (size_t)slot + 0], sq.cr[i]);
       accum_add(&st.accum[3 * (size_t)slot + 1], sq.cg[i]);
@@ -389,17 +547,18 @@ __global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq) {
     live_update(st, sid >= 0 ? (int)(slot >> shift) : -1, -1);
   }
   flush_stats(st.ctr, CTR_SNODES, ts);
+  flush_mixed(st.ctr, extra);
 }
 
-// Finished tiles: fold every pixel's samples into its scene's image, clear
-// the slot's accumulators and return the slot to the free list.
+// Finished tiles (busy slots whose live count is zero after this iteration's
+// kernels): fold every pixel's samples into its scene's image, clear the
+// slot's accumulators and return the slot to the free list.
 __global__ __launch_bounds__(256) void k_stream_resolve(Stream st) {
-  const uint32_t nd = min(st.ctl->done_count, (uint32_t)st.n_slots);
   const uint32_t shift = (uint32_t)st.tile_shift;
-  for (uint32_t dn = blockIdx.x; dn < nd; dn += gridDim.x) {
-    const uint32_t ts = st.done_list[dn];
+  for (uint32_t ts = blockIdx.x; ts < (uint32_t)st.n_slots; ts += gridDim.x) {
     const SlotInfo si = st.slot_info[ts];
-    const Params& P = st.scenes[si.scene];
+    if (si.tile < 0 || st.live[ts] != 0) continue;  // free, or rays still in flight
+    const Params& P = scene_params(st.scenes, si.scene);
     const TileDesc td = st.tiles[si.tile];
     float* rgb = st.out_rgb[si.scene];
     float* var = st.out_var[si.scene];
@@ -417,6 +576,7 @@ __global__ __launch_bounds__(256) void k_stream_resolve(Stream st) {
       for (int k = 0; k < 3 * P.spp; ++k) a[k] = 0ull;
     }
     if (threadIdx.x == 0) {
+      st.slot_info[ts].tile = -1;
       const uint32_t f = atomicAdd(&st.ctl->free_count, 1u);
       if (f < (uint32_t)st.n_slots) st.free_list[f] = ts;
     }
@@ -432,23 +592,27 @@ hipError_t launch_stream_refill(const Stream& st, int cur, hipStream_t s) {
   hipLaunchKernelGGL(k_stream_refill, dim3(1), dim3(256), 0, s, st, cur);
   return hipGetLastError();
 }
-hipError_t launch_stream_gen(const Stream& st, const RayQ& q, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_stream_gen, dim3(blocks), dim3(256), 0, s, st, q);
+hipError_t launch_stream_gen(const Stream& st, const RayQ& q, int cur, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_stream_gen, dim3(blocks), dim3(256), 0, s, st, q, cur);
   return hipGetLastError();
 }
-hipError_t launch_stream_extend(const Stream& st, const RayQ& q, int cur, double* hit_t, int32_t* hit_id,
-                                int stack_lds, int blocks, hipStream_t s) {
+hipError_t launch_stream_extend(const Stream& st, const RayQ& q, double* hit_t, int32_t* hit_id, int stack_lds,
+                                int blocks, hipStream_t s) {
   if (stack_lds <= 8)
-    hipLaunchKernelGGL(k_stream_extend<8>, dim3(blocks), dim3(256), 0, s, st, q, cur, hit_t, hit_id);
+    hipLaunchKernelGGL(k_stream_extend<8>, dim3(blocks), dim3(256), 0, s, st, q, hit_t, hit_id);
   else if (stack_lds <= 16)
-    hipLaunchKernelGGL(k_stream_extend<16>, dim3(blocks), dim3(256), 0, s, st, q, cur, hit_t, hit_id);
+    hipLaunchKernelGGL(k_stream_extend<16>, dim3(blocks), dim3(256), 0, s, st, q, hit_t, hit_id);
   else
-    hipLaunchKernelGGL(k_stream_extend<32>, dim3(blocks), dim3(256), 0, s, st, q, cur, hit_t, hit_id);
+    hipLaunchKernelGGL(k_stream_extend<32>, dim3(blocks), dim3(256), 0, s, st, q, hit_t, hit_id);
   return hipGetLastError();
 }
 hipError_t launch_stream_shade(const Stream& st, const RayQ& qin, const double* hit_t, const int32_t* hit_id,
                                const RayQ& qout, int nxt, int blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_stream_shade, dim3(blocks), dim3(256), 0, s, st, qin, hit_t, hit_id, qout, nxt);
+  return hipGetLastError();
+}
+hipError_t launch_stream_scan(const Stream& st, int nxt, hipStream_t s) {
+  hipLaunchKernelGGL(k_stream_scan, dim3(1), dim3(256), 0, s, st, nxt);
   return hipGetLastError();
 }
 hipError_t launch_stream_nee(const Stream& st, const RayQ& q, int nxt, const ShadowQ& sq, int blocks, hipStream_t s) {

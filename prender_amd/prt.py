@@ -201,7 +201,7 @@ class Scene:
 # ---- batched scenes
 def batch_opts(seed: int = 0, tile_samples: int = 0, inflight_tiles: int = 0, stack_lds: int = 0,
                trav_blocks_per_cu: int = 0, use_graph: bool = True, tile_stride: int = 0,
-               max_depth: int = 0) -> PrtBatchOpts:
+               max_depth: int = 0, timing: bool = False) -> PrtBatchOpts:
     o = PrtBatchOpts()
     o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
     o.tile_samples = int(tile_samples)
@@ -211,6 +211,7 @@ def batch_opts(seed: int = 0, tile_samples: int = 0, inflight_tiles: int = 0, st
     o.use_graph = 0 if use_graph else -1
     o.tile_stride = int(tile_stride)
     o.max_depth = int(max_depth)
+    o.timing = 1 if timing else 0
     return o
 
 

@@ -86,7 +86,7 @@ class PrtStats(C.Structure):
                 ("launches_extend", C.c_int64), ("shadow_node_visits", C.c_uint64),
                 ("shadow_tri_tests", C.c_uint64), ("bytes_extend", C.c_uint64),
                 ("bytes_shadow", C.c_uint64), ("ms_extend_primary", C.c_double),
-                ("ms_nee", C.c_double)]
+                ("ms_nee", C.c_double), ("mixed_scene_passes", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -101,10 +101,10 @@ class PrtRenderOpts(C.Structure):
 class PrtBatchOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("tile_samples", C.c_int32), ("inflight_tiles", C.c_int32),
                 ("stack_lds", C.c_int32), ("trav_blocks_per_cu", C.c_int32), ("use_graph", C.c_int32),
-                ("tile_stride", C.c_int32), ("max_depth", C.c_int32), ("pad_", C.c_int32)]
+                ("tile_stride", C.c_int32), ("max_depth", C.c_int32), ("timing", C.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 # ---------------------------------------------------------------- Python side
