@@ -89,7 +89,7 @@ struct StreamCtl {
   uint32_t status;         // 0 running, 1 finished, 2 inconsistent state, 3 queue capacity
   uint32_t iter;           // iterations run
   uint32_t cnt_sq;         // shadow queue count
-  uint32_t fetch[2];       // work-fetch counters of extend and shadow
+  uint32_t fetch[4];       // work-fetch counters of extend, shadow, shade and NEE
   uint32_t region_end[2];  // end of the scene regions of queue p (its primary region starts there)
   uint32_t work_width;     // padded index space of the current queue's work table
   uint32_t cont_width;     // ... of its scene regions alone (the first n_scenes entries)

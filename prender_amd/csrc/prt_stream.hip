@@ -147,7 +147,7 @@ __global__ void k_stream_init(Stream st) {
     c->status = 0;
     c->iter = 0;
     c->cnt_sq = 0;
-    c->fetch[0] = c->fetch[1] = 0;
+    c->fetch[0] = c->fetch[1] = c->fetch[2] = c->fetch[3] = 0;
     c->region_end[0] = c->region_end[1] = 0;
     c->work_width = c->cont_width = c->nee_width = c->n_work = 0;
     c->total_prim = c->total_cont = 0;
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_stream_refill(Stream st, int cur) {
     c->cont_width = cont_width;
     c->n_work = n_work;
     c->cnt_sq = 0;
-    c->fetch[0] = c->fetch[1] = 0;
+    c->fetch[0] = c->fetch[1] = c->fetch[2] = c->fetch[3] = 0;
     c->total_cont += n_cont;
     c->iter += 1;
     if (c->status < 2) {
@@ -324,31 +324,57 @@ __global__ __launch_bounds__(256) void k_stream_gen(Stream st, RayQ q, int cur) 
   if (lane_id() == 0 && rays) atomicAdd((unsigned long long*)&st.ctl->total_prim, (unsigned long long)rays);
 }
 
-// Closest hit for the current queue: waves fetch 64-wide chunks of the work
-// table's padded index space; a chunk is one scene's rays.
+// Waves take FETCH_CHUNKS 64-wide chunks of a padded index space per atomic
+// and walk them in runs of one table entry (one scene), so the scene's
+// parameters are loop-invariant within a run and their loads are hoisted.
+#define FETCH_CHUNKS 4u
+
+__device__ __forceinline__ uint32_t fetch_chunks(uint32_t* counter) {
+  uint32_t c = 0;
+  if (lane_id() == 0) c = atomicAdd(counter, FETCH_CHUNKS);
+  return (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)c, 0, 64));
+}
+
+// Static split of n_chunks chunks into one contiguous range per wave of the
+// grid (for kernels whose work per chunk is too short to pay for a fetch
+// atomic on one shared counter).
+__device__ __forceinline__ void wave_range(uint32_t n_chunks, uint32_t& c0, uint32_t& c1) {
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t per = (n_chunks + n_waves - 1) / n_waves;
+  c0 = min(wave * per, n_chunks);
+  c1 = min(c0 + per, n_chunks);
+}
+
+// Closest hit for the current queue through its work table.
 template <int NL>
 __global__ __launch_bounds__(256) void k_stream_extend(Stream st, RayQ q, double* __restrict__ hit_t,
                                                        int32_t* __restrict__ hit_id) {
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL);
-  const uint32_t n = st.ctl->work_width, n_work = st.ctl->n_work;
+  const uint32_t n_chunks = st.ctl->work_width >> 6, n_work = st.ctl->n_work;
   uint32_t* fetch = &st.ctl->fetch[0];
   TravStats ts{0, 0};
   while (true) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(fetch, 64u);
-    base = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)base, 0, 64));
-    if (base >= n) break;
-    const uint32_t e = find_entry(st.work_pref, n_work, base);
-    const WorkEntry we = st.work[e];
-    const uint32_t local = base - st.work_pref[e] + lane_id();
-    const Params& P = scene_params(st.scenes, we.scene);
-    if (local < we.cnt) {
-      const uint32_t i = we.off + local;
-      double t, u, v;
-      int32_t id;
-      closest_hit(P, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), t, id, u, v, S, ts);
-      hit_t[i] = t;
-      hit_id[i] = id;
+    uint32_t c = fetch_chunks(fetch);
+    if (c >= n_chunks) break;
+    const uint32_t c_end = min(c + FETCH_CHUNKS, n_chunks);
+    while (c < c_end) {
+      const uint32_t e = find_entry(st.work_pref, n_work, c << 6);
+      const WorkEntry we = st.work[e];
+      const uint32_t e_base = st.work_pref[e];
+      const uint32_t run_end = min(st.work_pref[e + 1] >> 6, c_end);
+      const Params& P = scene_params(st.scenes, we.scene);
+      for (; c < run_end; ++c) {
+        const uint32_t local = (c << 6) - e_base + lane_id();
+        if (local < we.cnt) {
+          const uint32_t i = we.off + local;
+          double t, u, v;
+          int32_t id;
+          closest_hit(P, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), t, id, u, v, S, ts);
+          hit_t[i] = t;
+          hit_id[i] = id;
+        }
+      }
     }
   }
   flush_stats(st.ctr, CTR_NODES, ts);
@@ -359,76 +385,89 @@ __global__ __launch_bounds__(256) void k_stream_extend(Stream st, RayQ q, double
 // tile); children go to their scene's region of queue nxt.
 __global__ __launch_bounds__(256) void k_stream_shade(Stream st, RayQ qin, const double* __restrict__ hit_t,
                                                       const int32_t* __restrict__ hit_id, RayQ qout, int nxt) {
-  const uint32_t cont_width = st.ctl->cont_width, n_new = st.ctl->n_new;
+  const uint32_t cont_chunks = st.ctl->cont_width >> 6, n_new = st.ctl->n_new;
   const uint32_t prim_base = st.ctl->region_end[nxt ^ 1];
   const uint32_t NS = (uint32_t)st.n_scenes;
   const uint32_t shift = (uint32_t)st.tile_shift, tmask = (1u << shift) - 1u;
-  const uint32_t total = cont_width + (n_new << shift);
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t n_chunks = cont_chunks + (n_new << (shift - 6));  // T >= 64: T/64 chunks per new tile
   ShadeCtx C;
   C.qout = qout;
   C.truncated = C.n_ext = C.pickups = C.dropped = 0;
-  for (uint32_t base = wave * 64u; base < total; base += n_waves * 64u) {
-    bool valid;
-    int scene;
-    uint32_t slot = 0, ray = 0, k = 0, branch = 1;
-    int depth = 0;
-    int64_t pixel = 0;
-    d3 T = mk(1.0, 1.0, 1.0), Pw = mk(0.0, 0.0, 0.0);
-    if (base < cont_width) {
-      const uint32_t e = find_entry(st.work_pref, NS, base);
-      const WorkEntry we = st.work[e];
-      const uint32_t local = base - st.work_pref[e] + lane_id();
-      scene = we.scene;
-      valid = local < we.cnt;
-      if (valid) {
-        ray = we.off + local;
-        slot = qin.slot[ray];
-        T = mk(qin.tr[ray], qin.tg[ray], qin.tb[ray]);
-        Pw = mk(qin.pr[ray], qin.pg[ray], qin.pb[ray]);
-        const uint32_t meta = qin.meta[ray];
-        depth = (int)(meta & 0xFFu);
-        branch = META_BRANCH(meta);
-        const SlotInfo si = st.slot_info[slot >> shift];
-        const uint32_t r = slot & tmask;
-        k = r % (uint32_t)si.spp;
-        pixel = si.pixel_base + r / (uint32_t)si.spp;
+  {
+    uint32_t c, c_end;
+    wave_range(n_chunks, c, c_end);
+    while (c < c_end) {
+      // a run: one scene region of the carried-over rays, or one new tile
+      const bool prim = c >= cont_chunks;
+      int scene;
+      uint32_t run_end, e_base = 0;
+      WorkEntry we{0, 0, 0, 0};
+      NewTile nt{};
+      if (!prim) {
+        const uint32_t e = find_entry(st.work_pref, NS, c << 6);
+        we = st.work[e];
+        e_base = st.work_pref[e];
+        run_end = min(st.work_pref[e + 1] >> 6, c_end);
+        scene = we.scene;
+      } else {
+        const uint32_t k = (c - cont_chunks) >> (shift - 6);
+        nt = st.new_tiles[k];
+        run_end = min(cont_chunks + ((k + 1) << (shift - 6)), c_end);
+        scene = nt.scene;
       }
-    } else {
-      // T >= 64 and chunks are 64-aligned: the chunk lies in one tile
-      const uint32_t j = base - cont_width;
-      const NewTile nt = st.new_tiles[j >> shift];
-      const uint32_t r = (j & tmask) + lane_id();
-      scene = nt.scene;
-      valid = r < nt.n_samples;
-      if (valid) {
-        slot = (nt.slot << shift) | r;
-        const uint32_t pl = r / nt.spp;
-        k = r % nt.spp;
-        ray = prim_base + nt.ray_base + pl * nt.ss2 + k % nt.ss2;
-        pixel = st.slot_info[nt.slot].pixel_base + pl;
+      const Params& P = scene_params(st.scenes, scene);
+      C.count_out = &st.seg_cnt[nxt * NS + scene];
+      C.off = st.seg_off[nxt * NS + scene];
+      C.cap = st.seg_cap[nxt * NS + scene];
+      for (; c < run_end; ++c) {
+        bool valid;
+        uint32_t slot = 0, ray = 0, k = 0, branch = 1;
+        int depth = 0;
+        int64_t pixel = 0;
+        d3 T = mk(1.0, 1.0, 1.0), Pw = mk(0.0, 0.0, 0.0);
+        if (!prim) {
+          const uint32_t local = (c << 6) - e_base + lane_id();
+          valid = local < we.cnt;
+          if (valid) {
+            ray = we.off + local;
+            slot = qin.slot[ray];
+            T = mk(qin.tr[ray], qin.tg[ray], qin.tb[ray]);
+            Pw = mk(qin.pr[ray], qin.pg[ray], qin.pb[ray]);
+            const uint32_t meta = qin.meta[ray];
+            depth = (int)(meta & 0xFFu);
+            branch = META_BRANCH(meta);
+            const SlotInfo si = st.slot_info[slot >> shift];
+            const uint32_t r = slot & tmask;
+            k = r % (uint32_t)si.spp;
+            pixel = si.pixel_base + r / (uint32_t)si.spp;
+          }
+        } else {
+          const uint32_t r = (((c - cont_chunks) << 6) & tmask) + lane_id();
+          valid = r < nt.n_samples;
+          if (valid) {
+            slot = (nt.slot << shift) | r;
+            const uint32_t pl = r / nt.spp;
+            k = r % nt.spp;
+            ray = prim_base + nt.ray_base + pl * nt.ss2 + k % nt.ss2;
+            pixel = st.slot_info[nt.slot].pixel_base + pl;
+          }
+        }
+        int delta = 0;
+        if (valid) {
+          C.slot = slot;
+          const uint32_t e0 = C.n_ext;
+          VertexAdj va;
+          const d3 add = shade_vertex(P, C, mk(qin.ox[ray], qin.oy[ray], qin.oz[ray]),
+                                      mk(qin.dx[ray], qin.dy[ray], qin.dz[ray]), hit_t[ray], hit_id[ray], T, Pw,
+                                      (uint32_t)pixel, k, depth, branch, va);
+          delta = (int)(C.n_ext - e0) - 1;
+          accum_add(&st.accum[3 * (size_t)slot + 0], add.x);
+          accum_add(&st.accum[3 * (size_t)slot + 1], add.y);
+          accum_add(&st.accum[3 * (size_t)slot + 2], add.z);
+        }
+        live_update(st, valid ? (int)(slot >> shift) : -1, delta);
       }
     }
-    scene = __builtin_amdgcn_readfirstlane(scene);
-    const Params& P = scene_params(st.scenes, scene);
-    C.count_out = &st.seg_cnt[nxt * NS + scene];
-    C.off = st.seg_off[nxt * NS + scene];
-    C.cap = st.seg_cap[nxt * NS + scene];
-    d3 add = mk(0.0, 0.0, 0.0);
-    int delta = 0;
-    if (valid) {
-      C.slot = slot;
-      const uint32_t e0 = C.n_ext;
-      VertexAdj va;
-      add = shade_vertex(P, C, mk(qin.ox[ray], qin.oy[ray], qin.oz[ray]), mk(qin.dx[ray], qin.dy[ray], qin.dz[ray]),
-                         hit_t[ray], hit_id[ray], T, Pw, (uint32_t)pixel, k, depth, branch, va);
-      delta = (int)(C.n_ext - e0) - 1;
-      accum_add(&st.accum[3 * (size_t)slot + 0], add.x);
-      accum_add(&st.accum[3 * (size_t)slot + 1], add.y);
-      accum_add(&st.accum[3 * (size_t)slot + 2], add.z);
-    }
-    live_update(st, valid ? (int)(slot >> shift) : -1, delta);
   }
   const uint64_t s_pick = wave_sum_u32(C.pickups), s_tr = wave_sum_u32(C.truncated);
   const uint64_t s_ext = wave_sum_u32(C.n_ext), s_drop = wave_sum_u32(C.dropped);
@@ -459,47 +498,54 @@ __global__ __launch_bounds__(256) void k_stream_scan(Stream st, int nxt) {
 
 // Light samples of the diffuse vertices behind the rays of queue nxt.
 __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, ShadowQ sq) {
-  const uint32_t width = st.ctl->nee_width;
+  const uint32_t n_chunks = st.ctl->nee_width >> 6;
   const uint32_t NS = (uint32_t)st.n_scenes;
   const uint32_t shift = (uint32_t)st.tile_shift, tmask = (1u << shift) - 1u;
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
   uint32_t emitted = 0, dropped = 0;
-  for (uint32_t base = wave * 64u; base < width; base += n_waves * 64u) {
-    const uint32_t s = find_entry(st.nee_pref, NS, base);
-    const uint32_t local = base - st.nee_pref[s] + lane_id();
-    const uint32_t cnt = min(st.seg_cnt[nxt * NS + s], st.seg_cap[nxt * NS + s]);
-    const Params& P = scene_params(st.scenes, (int)s);
-    ShadowRay s0, s1, s2;
-    bool v0 = false, v1 = false, v2 = false;
-    int32_t l0 = -1, l1 = -1, l2 = -1;
-    uint32_t slot = 0;
-    if (local < cnt) {
-      const uint32_t i = st.seg_off[nxt * NS + s] + local;
-      const uint32_t meta = q.meta[i];
-      if (meta & META_NEE) {
-        slot = q.slot[i];
-        const SlotInfo si = st.slot_info[slot >> shift];
-        const uint32_t r = slot & tmask;
-        const uint32_t k = r % (uint32_t)si.spp;
-        const int64_t pixel = si.pixel_base + r / (uint32_t)si.spp;
-        nee_vertex(P, (uint32_t)pixel, k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta), mk(q.ox[i], q.oy[i], q.oz[i]),
-                   mk(q.nx[i], q.ny[i], q.nz[i]), mk(q.tr[i], q.tg[i], q.tb[i]), s0, s1, s2, v0, v1, v2, l0, l1,
-                   l2);
+  {
+    uint32_t c, c_end;
+    wave_range(n_chunks, c, c_end);
+    while (c < c_end) {
+      const uint32_t s = find_entry(st.nee_pref, NS, c << 6);
+      const uint32_t s_base = st.nee_pref[s];
+      const uint32_t run_end = min(st.nee_pref[s + 1] >> 6, c_end);
+      const uint32_t cnt = min(st.seg_cnt[nxt * NS + s], st.seg_cap[nxt * NS + s]);
+      const uint32_t off = st.seg_off[nxt * NS + s];
+      const Params& P = scene_params(st.scenes, (int)s);
+      for (; c < run_end; ++c) {
+        const uint32_t local = (c << 6) - s_base + lane_id();
+        ShadowRay s0, s1, s2;
+        bool v0 = false, v1 = false, v2 = false;
+        int32_t l0 = -1, l1 = -1, l2 = -1;
+        uint32_t slot = 0;
+        if (local < cnt) {
+          const uint32_t i = off + local;
+          const uint32_t meta = q.meta[i];
+          if (meta & META_NEE) {
+            slot = q.slot[i];
+            const SlotInfo si = st.slot_info[slot >> shift];
+            const uint32_t r = slot & tmask;
+            const uint32_t k = r % (uint32_t)si.spp;
+            const int64_t pixel = si.pixel_base + r / (uint32_t)si.spp;
+            nee_vertex(P, (uint32_t)pixel, k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta),
+                       mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.nx[i], q.ny[i], q.nz[i]), mk(q.tr[i], q.tg[i], q.tb[i]),
+                       s0, s1, s2, v0, v1, v2, l0, l1, l2);
+          }
+        }
+        uint32_t n_sh = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
+        uint32_t j = wave_append_n(&st.ctl->cnt_sq, n_sh);
+        if (j + n_sh > sq.cap) {
+          dropped += n_sh;
+          n_sh = 0;
+        } else {
+          if (v0) put_shadow(sq, j++, s0, slot, l0);
+          if (v1) put_shadow(sq, j++, s1, slot, l1);
+          if (v2) put_shadow(sq, j++, s2, slot, l2);
+          emitted += n_sh;
+        }
+        live_update(st, n_sh > 0 ? (int)(slot >> shift) : -1, (int)n_sh);
       }
     }
-    uint32_t n_sh = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
-    uint32_t j = wave_append_n(&st.ctl->cnt_sq, n_sh);
-    if (j + n_sh > sq.cap) {
-      dropped += n_sh;
-      n_sh = 0;
-    } else {
-      if (v0) put_shadow(sq, j++, s0, slot, l0);
-      if (v1) put_shadow(sq, j++, s1, slot, l1);
-      if (v2) put_shadow(sq, j++, s2, slot, l2);
-      emitted += n_sh;
-    }
-    live_update(st, n_sh > 0 ? (int)(slot >> shift) : -1, (int)n_sh);
   }
   const uint64_t s_sh = wave_sum_u32(emitted), s_drop = wave_sum_u32(dropped);
   if (lane_id() == 0) {
