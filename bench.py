@@ -141,6 +141,12 @@ def main():
         "device": info["arch"],
         "scene_build_s": build_s,
     }
+    traffic = measured_traffic(dom, args.workload)
+    if traffic is not None:
+        out["roofline"]["traffic"] = traffic["traffic_bytes_per_launch"]
+        out["roofline"]["traffic_source"] = ("profiles/k_extend_cfg2_hbm_traffic.md: rocprofv3 FETCH_SIZE x %.3f + "
+                                             "WRITE_SIZE x %.3f per dispatch (scales from a 1 GiB copy / fill)"
+                                             % (traffic["fetch_scale"], traffic["write_scale"]))
     if not args.no_rmse:
         # RMSE of the measured image against a 16x-spp render of the same scene
         ref = torch.empty_like(rgb)
@@ -153,6 +159,18 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(desc, args, elapsed / args.steps, ref_np)
     print(json.dumps(out))
+
+
+def measured_traffic(kernel, workload_name):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary of the same workload (PMC counters need their own profiler run),
+    or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_name}_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d if d.get("kernel") == kernel and d.get("workload") == workload_name else None
 
 
 def workload(G, name):
