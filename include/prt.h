@@ -270,6 +270,42 @@ int prt_adjoint_device(prt_scene* scene, const prt_render_opts* opts,
                        const float* d_dimage, int entity, float* d_albedo_grad,
                        int light, double* d_emission_grad, prt_stats* stats);
 
+/* ---- scene files (replaces SceneEnv::Load + CreateEntList for the in-scope
+ *      keyword subset: scene_env.h:357-1990, Expr.cpp, ObjLoader.cpp) ------- */
+/* Parses a scene file: IMAGE, SCREEN, SAMPLING, SUPERSAMPLING, DEPTH,
+ * DEPTH_LIMIT, CAMERA_POS / _DIR / _UPVEC / _ANGLE / _MATRIX, TARGET_POS,
+ * BACKGROUND, ENVLIGHT, nextEventEstimation, luminance_cutoff,
+ * sensor_response, VAR, OUTPUT, gamma_offset; OBJECT sphere / uvplane /
+ * objfile with emission, color, roughness, refractive_index, reflection
+ * diffuse|specular|reflection, normal, rotation / translate / scale; the
+ * expression syntax of every numeric field and the environment overrides
+ * applied after the file.  Keywords that would change the image outside the
+ * subset fail with PRT_E_UNSUPPORTED (message in prt_last_error).  OBJ paths
+ * are tried as given, then relative to the scene file.  NULL on failure. */
+typedef struct prt_scene_file prt_scene_file;
+prt_scene_file* prt_scene_file_load(const char* path);
+/* The parsed description (valid until prt_scene_file_free); feed it to
+ * prt_scene_create. */
+const prt_scene_desc* prt_scene_file_desc(const prt_scene_file* file);
+/* OUTPUT (default "output.ppm") and gamma_offset of the file. */
+const char* prt_scene_file_output(const prt_scene_file* file, double* gamma_offset);
+void prt_scene_file_free(prt_scene_file* file);
+/* One numeric field as the scene-file reader evaluates it (Expr.cpp
+ * expr_eval: + - * / % ^, parentheses, pi, e, sqrt ... datan2, floor, ceil,
+ * "%NAME%" environment references); *status = PRT_OK or an error. */
+double prt_eval_expression(const char* text, int* status);
+
+/* ---- image output (ppm.h:14-35, render.h:1458-1560, bitmap.cpp) ---------- */
+/* rgb: W*H*3 floats in the reference's storage order (row 0 = top).
+ * 8-bit values int(pow(clamp(x,0,1), gamma) * 255 + 0.5); gamma <= 0 means
+ * 1/2.2. */
+int prt_image_to_rgb8(const float* rgb, int width, int height, double gamma, uint8_t* out);
+/* ASCII P3 file as Render::Output writes it. */
+int prt_write_ppm(const char* path, const float* rgb, int width, int height);
+/* 24-bit BMP as Render::OutputBmp writes it; |gamma_offset| > 0.01 replaces
+ * the 1/2.2 exponent. */
+int prt_write_bmp(const char* path, const float* rgb, int width, int height, double gamma_offset);
+
 #ifdef __cplusplus
 }
 #endif

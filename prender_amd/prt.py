@@ -65,6 +65,18 @@ def lib():
     L.prt_hold_rr_albedo.argtypes = [C.c_void_p, C.c_int]
     L.prt_adjoint_device.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_int, C.c_void_p,
                                      C.c_int, C.POINTER(C.c_double), C.POINTER(PrtStats)]
+    L.prt_scene_file_load.restype = C.c_void_p
+    L.prt_scene_file_load.argtypes = [C.c_char_p]
+    L.prt_scene_file_desc.restype = C.POINTER(PrtSceneDesc)
+    L.prt_scene_file_desc.argtypes = [C.c_void_p]
+    L.prt_scene_file_output.restype = C.c_char_p
+    L.prt_scene_file_output.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.prt_scene_file_free.argtypes = [C.c_void_p]
+    L.prt_eval_expression.restype = C.c_double
+    L.prt_eval_expression.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
+    L.prt_image_to_rgb8.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p]
+    L.prt_write_ppm.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_int]
+    L.prt_write_bmp.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_double]
     _lib = L
     return L
 
@@ -72,7 +84,9 @@ def lib():
 EXPORTED = ["prt_abi_version", "prt_last_error", "prt_device_info", "prt_scene_create", "prt_scene_destroy",
             "prt_scene_info", "prt_trace_closest", "prt_trace_closest_device", "prt_render", "prt_render_device",
             "prt_render_batch_device", "prt_tune_batch", "prt_set_face_albedo_device", "prt_set_emission",
-            "prt_hold_rr_albedo", "prt_adjoint_device"]
+            "prt_hold_rr_albedo", "prt_adjoint_device", "prt_scene_file_load", "prt_scene_file_desc",
+            "prt_scene_file_output", "prt_scene_file_free", "prt_eval_expression", "prt_image_to_rgb8",
+            "prt_write_ppm", "prt_write_bmp"]
 
 
 def check(rc: int, what: str):
@@ -243,3 +257,37 @@ def tune_batch(scenes, rgb_ptrs, var_ptrs=None, base: Optional[PrtBatchOpts] = N
     ns = C.c_double()
     check(lib().prt_tune_batch(hs, n, C.byref(b), rg, vr, C.byref(best), C.byref(ns)), "prt_tune_batch")
     return best, ns.value
+
+
+# ---- scene expressions and image files (host-side, no device needed)
+def eval_expression(text: str) -> float:
+    """A numeric scene-file field as the reader evaluates it (Expr.cpp)."""
+    st = C.c_int(0)
+    v = lib().prt_eval_expression(text.encode(), C.byref(st))
+    check(st.value, "prt_eval_expression")
+    return v
+
+
+def _image(rgb: np.ndarray):
+    a = np.ascontiguousarray(rgb, dtype=np.float32)
+    if a.ndim != 3 or a.shape[2] != 3:
+        raise PrtError("expected an (H, W, 3) image")
+    return a, a.shape[1], a.shape[0]
+
+
+def to_rgb8(rgb: np.ndarray, gamma: float = 0.0) -> np.ndarray:
+    """8-bit image with ppm.h's to_int (gamma 0 = 1/2.2)."""
+    a, w, h = _image(rgb)
+    out = np.empty((h, w, 3), dtype=np.uint8)
+    check(lib().prt_image_to_rgb8(a.ctypes.data, w, h, float(gamma), out.ctypes.data), "prt_image_to_rgb8")
+    return out
+
+
+def write_ppm(path: str, rgb: np.ndarray):
+    a, w, h = _image(rgb)
+    check(lib().prt_write_ppm(os.fsencode(path), a.ctypes.data, w, h), "prt_write_ppm")
+
+
+def write_bmp(path: str, rgb: np.ndarray, gamma_offset: float = 0.0):
+    a, w, h = _image(rgb)
+    check(lib().prt_write_bmp(os.fsencode(path), a.ctypes.data, w, h, float(gamma_offset)), "prt_write_bmp")

@@ -127,3 +127,23 @@ def test_render_parity_cfg3_common_random_numbers(prt):
     assert np.median(err) < 1e-3 * max(1e-6, float(np.median(ro)))
     assert sigma_test(rg, vg, ro, vo, 1.0) < 0.02
     print("cfg3 CRN: max err", err.max(), "means", rg.mean(), ro.mean(), "gpu", st, "cpu", so)
+
+
+def test_reference_scene_file_parity(prt):
+    # the reference's own proj/test/CornellBox.txt through the scene-file
+    # reader, at reduced size; it keeps the file's defaults, among them the
+    # luminance clamp (0.025) and DEPTH 8 / DEPTH_LIMIT 64
+    import os
+    from prender_amd import scenefile
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes", "CornellBox.txt")
+    d = scenefile.load(path)
+    d.camera.width, d.camera.height = 64, 48
+    d.sampling.samples = 2
+    assert d.sampling.luminance_cutoff == 0.025
+    print("render CornellBox.txt 64x48x8spp seed 17", flush=True)
+    rg, vg, st = prt.Scene(d).render(seed=17)
+    ro, vo, _ = OracleScene(d).render(seed=17)
+    err = np.abs(rg - ro)
+    assert np.median(err) < 1e-3 * max(1e-6, float(np.median(ro)))
+    assert sigma_test(rg, vg, ro, vo, 1.0) < 0.01
+    print("CornellBox.txt CRN: max err", err.max(), "mean", rg.mean(), ro.mean(), st)

@@ -1,0 +1,269 @@
+"""Scene-file reader (SceneEnv::Load subset, scene_env.h:357-1990), the
+expression evaluator of its numeric fields (Expr.h / Expr.cpp), the OBJ
+reader (ObjLoader.cpp) and the image writers (ppm.h, render.h OutputBmp),
+all on the host.  Expected values come from closed forms and from the
+reference's own scene file tests/golden/scenes/CornellBox.txt (proj/test)."""
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from prender_amd import prt
+from prender_amd import scenefile
+from prender_amd.scene import REFLECTION_DIFFUSE, REFLECTION_REFRACTION, REFLECTION_SPECULAR, Polygon, Sphere, UVPlane
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CORNELL = os.path.join(HERE, "golden", "scenes", "CornellBox.txt")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(prt.LIB_PATH), reason="libprt.so is not built")
+
+
+# ---------------------------------------------------------------- expressions
+@pytest.mark.parametrize("text,value", [
+    ("1+2*3", 7.0),
+    ("(1+2)*3", 9.0),
+    ("2^3^2", 512.0),          # right-associative
+    ("-2^2", 4.0),             # the sign binds tighter than ^ in this grammar
+    ("7%3", 1.0),
+    ("-7%3", 2.0),             # a - floor(a/b)*b
+    ("10/4", 2.5),
+    ("sqrt(16)+pow(2,10)", 1028.0),
+    ("floor(2.7)+ceil(2.1)", 5.0),
+    ("atan2(1,1)", math.pi / 4),
+    ("datan2(1,1)", 45.0),
+    ("2*pi", 2 * math.pi),
+    ("e", math.e),
+    ("012", 10.0),             # a leading 0 selects octal
+    ("0x1A", 26.0),
+    ("0b2:101", 5.0),
+    ("0t09", 9.0),
+    ("0.25", 0.25),
+    ("1.5e3", 1500.0),
+    ("2E-2", 0.02),
+    ("  3+ 4\n", 7.0),         # blanks are skipped before a number only
+])
+def test_expression_values(text, value):
+    assert prt.eval_expression(text) == pytest.approx(value, rel=1e-15, abs=1e-15)
+
+
+def test_expression_trig_in_degrees():
+    assert prt.eval_expression("sind(30)") == pytest.approx(0.5, abs=1e-15)
+    assert prt.eval_expression("cosd(60)") == pytest.approx(0.5, abs=1e-15)
+    assert prt.eval_expression("dacos(0.5)") == pytest.approx(60.0, abs=1e-12)
+
+
+def test_expression_environment_reference(monkeypatch):
+    monkeypatch.setenv("PRT_TEST_VALUE", "2.5")
+    assert prt.eval_expression("%PRT_TEST_VALUE%") == 2.5
+    monkeypatch.delenv("PRT_TEST_VALUE")
+    assert prt.eval_expression("%PRT_TEST_VALUE%") == 0.0
+
+
+@pytest.mark.parametrize("text", ["1+", "foo", "sqrt 4", "3 4\n", "(1",
+                                  # the reference rejects a blank just before the line end (its check
+                                  # looks at the second-to-last character, the one before '\n')
+                                  "1 \n", "3 + 4\n"])  # a blank ends a value: "3 + 4" is 3 then garbage
+def test_expression_errors(text):
+    with pytest.raises(prt.PrtError):
+        prt.eval_expression(text)
+
+
+# ---------------------------------------------------------------- the reference's Cornell box
+def test_reference_cornell_box_file():
+    d = scenefile.load(CORNELL)
+    cam = d.camera
+    assert (cam.width, cam.height) == (320, 240)
+    assert cam.position == (50.0, 52.0, 220.0)
+    assert cam.dir == (0.0, -0.04, -1.0)
+    assert cam.up == (0.0, 1.0, 0.0)
+    assert (cam.world_screen_width, cam.world_screen_height, cam.world_screen_dist) == (30.0, 30.0, 40.0)
+    assert cam.camera_angle is None
+    s = d.sampling
+    assert (s.samples, s.supersamples) == (100, 2)
+    # SceneEnv::Default for what the file leaves out
+    assert (s.depth, s.depth_limit, s.next_event_estimation) == (8, 64, 0)
+    assert s.luminance_cutoff == 0.025 and s.sensor_response == 1.0
+    assert d.output == "CornellBox.bmp"
+    assert len(d.entities) == 10 and all(isinstance(e, Sphere) for e in d.entities)
+    walls = d.entities[:6]
+    assert [w.radius for w in walls] == [100000.0] * 6
+    assert walls[0].position == (100001.0, 40.8, 81.6)
+    assert walls[5].position == (50.0, -99918.4, 81.6)
+    light = d.entities[8]
+    assert light.position == (50.0, 90.0, 81.6) and light.radius == 15.0
+    m_light = d.materials[light.material]
+    assert m_light.emission == (36.0, 36.0, 36.0) and m_light.is_light()
+    mirror = d.materials[d.entities[7].material]
+    assert mirror.resolved() == (REFLECTION_SPECULAR, 0.0) and mirror.color == (0.99, 0.99, 0.99)
+    glass = d.materials[d.entities[9].material]
+    assert glass.resolved()[0] == REFLECTION_REFRACTION and glass.refractive_index == 1.5
+    diffuse = d.materials[d.entities[0].material]
+    assert diffuse.resolved() == (REFLECTION_DIFFUSE, 1.0) and diffuse.color == (0.75, 0.25, 0.25)
+    assert d.light_entities() == [8]
+
+
+def test_reference_test_script_overrides(monkeypatch):
+    # proj/test/basic_test.bat sets these before running the renderer
+    monkeypatch.setenv("IMAGE_X", "640*0.25")
+    monkeypatch.setenv("IMAGE_Y", "480*0.25")
+    monkeypatch.setenv("SAMPLING", "400")
+    monkeypatch.setenv("NEXTEVENTESTIMATION", "1")
+    d = scenefile.load(CORNELL)
+    assert (d.camera.width, d.camera.height) == (160, 120)
+    assert d.sampling.samples == 400 and d.sampling.next_event_estimation == 1
+
+
+# ---------------------------------------------------------------- written scenes
+def write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_keywords_variables_and_comment_rule(tmp_path):
+    path = write(tmp_path, "s.txt", "\n".join([
+        "VAR w", "64*2",
+        "IMAGE", "128 96",
+        "#one comment line is skipped,",
+        "#and the next one is read as a keyword line (and ignored)",
+        "DEPTH", "#a comment before a value is skipped too", "w/32",
+        "DEPTH_LIMIT", "10",
+        "SAMPLING", "4",
+        "SUPERSAMPLING", "2",
+        "CAMERA_ANGLE", "45",
+        "TARGET_POS", "0 0 0",
+        "CAMERA_POS", "0 0 10",
+        "nextEventEstimation", "1",
+        "luminance_cutoff", "0",
+        "sensor_response", "2.5",
+        "BACKGROUND", "0.1 0.2 0.3",
+        "ENVLIGHT", "0.01 0.02 w/1000",
+        "OUTPUT", "out.ppm",
+        "",
+        "OBJECT", "uvplane 1 2 3", "V 0 0 0.5", "U 0.5 0 0",
+        "emission 10 10 10", "color 0 0 0", "reflection diffuse", "normal -1",
+        "",
+        "OBJECT", "sphere 0 1 0 2", "color 0.5 0.5 0.5", "roughness 0.25", "translate 1 2 3",
+        "",
+        "OBJECT", "sphere 0 0 0 1", "color 0.9 0.9 0.9", "refractive_index 1.33", "reflection reflection",
+        "", ""]))
+    d = scenefile.load(path)
+    assert (d.camera.width, d.camera.height) == (128, 96)
+    assert d.sampling.depth == 4 and d.sampling.depth_limit == 10
+    assert d.camera.camera_angle == 45.0
+    # TARGET_POS is applied after the file: dir = normalize(target - pos)
+    assert d.camera.dir == pytest.approx((0.0, 0.0, -1.0))
+    assert d.sampling.next_event_estimation == 1 and d.sampling.luminance_cutoff == 0.0
+    assert d.sampling.sensor_response == 2.5
+    assert d.sampling.background == (0.1, 0.2, 0.3)
+    assert d.sampling.environment_light == pytest.approx((0.01, 0.02, 0.128))
+    assert d.output == "out.ppm"
+    plane, sph, glass = d.entities
+    assert isinstance(plane, UVPlane) and plane.org == (1.0, 2.0, 3.0)
+    assert plane.U == pytest.approx((0.5, 0.0, 0.0)) and plane.V == pytest.approx((0.0, 0.0, 0.5))
+    assert plane.normal_vector_inverse == -1 and d.materials[plane.material].is_light()
+    assert isinstance(sph, Sphere) and sph.position == (1.0, 3.0, 3.0) and sph.radius == 2.0
+    # no reflection line: DIFFUSE with the given roughness
+    assert d.materials[sph.material].resolved() == (REFLECTION_DIFFUSE, 0.25)
+    assert d.materials[glass.material].resolved()[0] == REFLECTION_REFRACTION
+    assert d.materials[glass.material].refractive_index == 1.33
+
+
+def test_rotation_of_a_uvplane(tmp_path):
+    path = write(tmp_path, "r.txt", "\n".join([
+        "OBJECT", "uvplane 1 0 0", "U 2 0 0", "V 0 3 0", "color 0.5 0.5 0.5", "rotation 0 0 90", "", ""]))
+    (p,) = scenefile.load(path).entities
+    # a point (x, y) -> (-y, x) under Matrix4x4::RotationZ(90 deg) acting on row vectors
+    assert p.org == pytest.approx((0.0, 1.0, 0.0), abs=1e-12)
+    assert p.U == pytest.approx((0.0, 2.0, 0.0), abs=1e-12)
+    assert p.V == pytest.approx((-3.0, 0.0, 0.0), abs=1e-12)
+
+
+def test_obj_mesh_faces_transforms_and_colours(tmp_path):
+    write(tmp_path, "m.obj", "\n".join([
+        "# quad + triangle", "v 0 0 0 0.2 0.4 0.6", "v 1 0 0 0.2 0.4 0.6", "v 1 1 0 0.2 0.4 0.6",
+        "v 0 1 0 0.2 0.4 0.6", "v 0 0 1 0.9 0.1 0.1", "v 1 0 1 0.9 0.1 0.1", "v 0 1 1 0.9 0.1 0.1",
+        "vt 0 0", "vn 0 0 1", "f 1/1/1 2/1/1 3/1/1 4/1/1", "f 5//1 6//1 7//1", ""]))
+    obj2 = write(tmp_path, "n.obj", "\n".join(["v 0 0 0 1 0 0", "v 1 0 0 0 1 0", "v 0 1 0 0 0 1", "f 1 2 3", ""]))
+    path = write(tmp_path, "o.txt", "\n".join([
+        "OBJECT", "objfile m.obj", "color 0.8 0.8 0.8", "reflection diffuse", "scale 2 2 2", "translate 0 0 5", "",
+        ""]))
+    d = scenefile.load(path)  # the OBJ path resolves relative to the scene file
+    (m,) = d.entities
+    assert isinstance(m, Polygon)
+    # quads split (a,b,c) + (c,d,a) (ObjLoader.cpp); 1-based indices
+    np.testing.assert_array_equal(m.indices, [[0, 1, 2], [2, 3, 0], [4, 5, 6]])
+    # scale then translate, each point through the matrices in order
+    np.testing.assert_array_equal(m.vertices[2], np.float32([2, 2, 5]))
+    np.testing.assert_array_equal(m.vertices[4], np.float32([0, 0, 7]))
+    np.testing.assert_array_equal(m.face_albedo, np.float32([[0.2, 0.4, 0.6], [0.2, 0.4, 0.6], [0.9, 0.1, 0.1]]))
+    bad = write(tmp_path, "b.txt", "\n".join(["OBJECT", "objfile " + obj2, "color 1 1 1", "", ""]))
+    with pytest.raises(prt.PrtError, match="vary across a face"):
+        scenefile.load(bad)
+
+
+@pytest.mark.parametrize("text,word", [
+    ("metropolisTransport\n1\n", "metropolisTransport"),
+    ("participatingMedia\n1\n", "participatingMedia"),
+    ("LENS\n1 100\n", "LENS"),
+    ("OBJECT\nplane 0 0 0\nnormal 0 1 0\ncolor 1 1 1\n\n", "plane"),
+    ("OBJECT\nsphere 0 0 0 1\nreflection phong_brdf\n\n", "phong_brdf"),
+    ("OBJECT\nsphere 0 0 0 1\ntexture a.bmp\n\n", "texture"),
+])
+def test_unsupported_keywords_fail_loudly(tmp_path, text, word):
+    path = write(tmp_path, "u.txt", text)
+    with pytest.raises(prt.PrtError, match=word):
+        scenefile.load(path)
+
+
+def test_commented_object_blocks_are_skipped(tmp_path):
+    # '#OBJECT' hides the block: its lines are top-level lines that match no keyword
+    path = write(tmp_path, "c.txt", "\n".join([
+        "#metropolisTransport", "1", "#OBJECT", "sphere 65 20 20 20", "color 0.25 0.75 0.25",
+        "reflection diffuse", "", "OBJECT", "sphere 1 2 3 4", "color 1 1 1", "", ""]))
+    d = scenefile.load(path)
+    assert len(d.entities) == 1 and d.entities[0].position == (1.0, 2.0, 3.0)
+
+
+# ---------------------------------------------------------------- image output
+def to_int_ref(x):
+    """ppm.h to_int in numpy (double)."""
+    return (np.power(np.clip(x.astype(np.float64), 0.0, 1.0), 0.4545454545454545) * 255 + 0.5).astype(np.int64)
+
+
+def test_to_rgb8_matches_the_formula():
+    rng = np.random.default_rng(0)
+    img = rng.uniform(-0.2, 1.3, (5, 7, 3)).astype(np.float32)
+    img[0, 0] = [0.0, 1.0, 0.5]
+    out = prt.to_rgb8(img)
+    np.testing.assert_array_equal(out, to_int_ref(img))
+    assert out[0, 0].tolist() == [0, 255, int(0.5 ** 0.4545454545454545 * 255 + 0.5)]
+
+
+def test_ppm_file(tmp_path):
+    img = np.linspace(0, 1, 2 * 3 * 3, dtype=np.float32).reshape(2, 3, 3)
+    p = str(tmp_path / "a.ppm")
+    prt.write_ppm(p, img)
+    text = open(p).read()
+    head = "P3\n3 2\n255\n"
+    assert text.startswith(head)
+    vals = [int(v) for v in text[len(head):].split()]
+    assert vals == to_int_ref(img).reshape(-1).tolist()
+
+
+def test_bmp_file(tmp_path):
+    img = np.random.default_rng(1).uniform(0, 1, (3, 5, 3)).astype(np.float32)
+    p = str(tmp_path / "a.bmp")
+    prt.write_bmp(p, img)
+    b = open(p, "rb").read()
+    row = 5 * 3 + 5 % 4
+    assert b[:2] == b"BM" and struct.unpack("<I", b[2:6])[0] == 54 + 3 * row == len(b)
+    assert struct.unpack("<IIii", b[10:26]) == (54, 40, 5, 3)
+    assert struct.unpack("<HH", b[26:30]) == (1, 24)
+    q = to_int_ref(img)
+    for i in range(3):  # file row i = image row H-1-i, BGR
+        line = b[54 + i * row: 54 + (i + 1) * row]
+        expect = q[2 - i][:, ::-1].reshape(-1).tolist()
+        assert list(line[:15]) == expect and list(line[15:]) == [0]
