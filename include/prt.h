@@ -39,6 +39,7 @@ extern "C" {
 #define PRT_E_NOMEM        -3   /* device or host allocation failed */
 #define PRT_E_NODEVICE     -4   /* no gfx950 device visible */
 #define PRT_E_UNSUPPORTED  -5   /* feature outside the in-scope subset */
+#define PRT_E_INTERNAL     -6   /* a render failed its own consistency check (message in prt_last_error) */
 
 /* Material::reflection_type subset (proj/material.h:19-29). */
 #define PRT_REFLECTION_DIFFUSE     0

@@ -46,6 +46,8 @@ void Workspace::release() {
   hit_t.release();
   hit_id.release();
   for (auto& b : sq) b.release();
+  for (auto& b : desc) b.release();
+  slab_cnt.release();
   accum.release();
   accum2.release();
   counts.release();
@@ -54,6 +56,7 @@ void Workspace::release() {
   if (h_counts) (void)hipHostFree(h_counts);
   h_counts = nullptr;
   batch_samples = queue_cap = shadow_cap = 0;
+  slab_ray = slab_sh = 0;
 }
 
 int alloc_ray_queue(DevBuf* cols, int64_t cap) {
@@ -490,10 +493,19 @@ int prt_trace_closest(prt_scene* s, const prt_ray* rays, int64_t n, prt_hit* hit
 // ---------------------------------------------------------------- render
 static int ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum) {
   Workspace& w = s->ws;
-  const int64_t fan = s->has_glass ? 8 : 1;
-  const int64_t qcap = batch_samples * fan;
-  const int64_t scap = 3 * qcap;
-  if (w.batch_samples >= batch_samples && w.queue_cap >= qcap && w.h_counts && (!two_accum || w.accum2.p))
+  // a ray queue never holds more than batch_samples * grow path segments
+  // (glass splits in two up to depth 2); a vertex emits at most fan children
+  const int64_t grow = s->has_glass ? 8 : 1, fan = s->has_glass ? 2 : 1;
+  const int64_t W = PRT_SLAB_WAVES;
+  // chunks of any queue (entries / 64 + one padded chunk per slab) over the
+  // slab writers: the most input chunks one writer wave is given
+  const int64_t cw = ((batch_samples * grow + 63) / 64 + W + W - 1) / W;
+  const int64_t slab_ray = fan * 64 * cw, slab_sh = 3 * 64 * cw;
+  const int64_t qcap = std::max(W * slab_ray, batch_samples + 64);
+  const int64_t scap = W * slab_sh;
+  if (qcap >= ((int64_t)1 << 32) || scap >= ((int64_t)1 << 32)) return fail(PRT_E_INVALID, "batch too large");
+  if (w.batch_samples >= batch_samples && w.queue_cap >= qcap && w.shadow_cap >= scap && w.h_counts &&
+      (!two_accum || w.accum2.p))
     return PRT_OK;
   w.release();
   int rc;
@@ -502,15 +514,21 @@ static int ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum)
   if ((rc = dev_alloc(w.hit_t, (size_t)qcap * 8))) return rc;
   if ((rc = dev_alloc(w.hit_id, (size_t)qcap * 4))) return rc;
   if ((rc = alloc_shadow_queue(w.sq, scap))) return rc;
+  for (int k = 0; k < 3; ++k)
+    if ((rc = dev_alloc(w.desc[k], (size_t)((k < 2 ? qcap : scap) / 64 + 1) * sizeof(uint2)))) return rc;
+  if ((rc = dev_alloc(w.slab_cnt, (size_t)3 * W * sizeof(uint32_t)))) return rc;
   if ((rc = dev_alloc(w.accum, (size_t)batch_samples * 3 * sizeof(uint64_t)))) return rc;
   if (two_accum && (rc = dev_alloc(w.accum2, (size_t)batch_samples * 3 * sizeof(uint64_t)))) return rc;
   if ((rc = dev_alloc(w.counts, 16 * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(w.fetch, 1024 * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(w.fetch, (size_t)FETCH_LAUNCHES * PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE * sizeof(uint32_t))))
+    return rc;
   if ((rc = dev_alloc(w.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
   HIPCHK(hipHostMalloc((void**)&w.h_counts, 16 * sizeof(uint32_t), hipHostMallocDefault));
   w.batch_samples = batch_samples;
   w.queue_cap = qcap;
   w.shadow_cap = scap;
+  w.slab_ray = (uint32_t)slab_ray;
+  w.slab_sh = (uint32_t)slab_sh;
   return PRT_OK;
 }
 
@@ -536,8 +554,9 @@ static int setup_render(prt_scene* s, const prt_render_opts* o, RenderSetup& R) 
   R.n_pix = (int64_t)P.W * P.H;
   int64_t bp = opts.batch_pixels > 0 ? opts.batch_pixels : std::max<int64_t>(1, ((int64_t)1 << 21) / P.spp);
   bp = std::min(bp, R.n_pix);
-  // queues hold up to 8x (glass splits) and 24x (shadow rays) the batch
-  while (bp > 1 && bp * P.spp * 24 >= ((int64_t)1 << 32)) bp /= 2;
+  // queues hold up to 16x (glass splits into slabs) and 24x (shadow rays)
+  // the batch, indexed by uint32
+  while (bp > 1 && bp * P.spp * 32 >= ((int64_t)1 << 32)) bp /= 2;
   R.bp = bp;
   R.P = P;
   return PRT_OK;
@@ -551,67 +570,90 @@ struct CallTotals {
 
 // One batch through the wavefront: generate, then extend / shade / NEE /
 // shadow per path depth until no ray is left.  Sample radiance accumulates
-// into `accum` (fixed point, 3 per camera sample of the batch).
+// into `accum` (fixed point, 3 per camera sample of the batch).  Each
+// slab-writing kernel is followed by k_slab_index, which lists the chunks
+// of the queue it wrote for the kernels that read it.
 static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* accum, const Adjoint& A, Timer& tm,
                      CallTotals& tot) {
   Workspace& w = s->ws;
   hipStream_t st = s->stream;
   uint32_t* counts = (uint32_t*)w.counts.p;
   uint32_t* fetch = (uint32_t*)w.fetch.p;
+  uint32_t* slab_cnt = (uint32_t*)w.slab_cnt.p;
   uint64_t* ctr = (uint64_t*)w.ctr.p;
   RayQ q[2] = {ray_queue(w.q[0], w.queue_cap), ray_queue(w.q[1], w.queue_cap)};
   ShadowQ sq = shadow_queue(w.sq, w.shadow_cap);
   const int tb = s->max_trav_blocks;
   const bool nee = P.nee && P.n_lights > 0;
+  const uint32_t W = PRT_SLAB_WAVES;
+  const size_t fetch_launch = (size_t)PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE;
+  auto chunks = [&](int k) { return ChunkList{(const uint2*)w.desc[k].p, counts + k}; };
+  auto index = [&](int k, uint32_t n_slabs, uint32_t slab_cap, const uint32_t* cnt) {
+    const uint32_t max_chunks = (uint32_t)(w.desc[k].bytes / sizeof(uint2));
+    return launch_slab_index(cnt, n_slabs, slab_cap, (uint2*)w.desc[k].p, max_chunks, counts + k, counts + 4 + k, ctr,
+                             st);
+  };
+  int fi = 0;  // traversal launches so far; each takes fresh claim counters
+  auto next_fetch = [&](uint32_t** f) -> int {
+    if (fi >= FETCH_LAUNCHES) return fail(PRT_E_INVALID, "too many wavefront launches in one batch");
+    if (fi % FETCH_GROUP == 0) HIPCHK(launch_fill_u32(fetch + fi * fetch_launch, 0u, FETCH_GROUP * fetch_launch, st));
+    *f = fetch + fi++ * fetch_launch;
+    return PRT_OK;
+  };
   int rc;
+  uint32_t* f;
   const int64_t ns = B.n_pixels * P.spp;
   const int64_t np = B.n_pixels * P.SS * P.SS;
   HIPCHK(launch_fill_u64(accum, 0ull, 3 * ns, st));
-  HIPCHK(launch_fill_u32(fetch, 0u, 1024, st));
   HIPCHK(launch_fill_u32(counts, 0u, 16, st));
-  // counts[3] = number of primary rays (consumed by the extend kernel)
-  HIPCHK(launch_fill_u32(counts + 3, (uint32_t)np, 1, st));
+  // the primary rays: one slab of np rays at the start of q[0]
+  HIPCHK(launch_fill_u32(slab_cnt, (uint32_t)np, 1, st));
   HIPCHK(launch_gen_primary(P, B, q[0], st));
+  HIPCHK(index(0, 1, (uint32_t)w.queue_cap, slab_cnt));
   if ((rc = tm.mark(T_OTHER))) return rc;
-  int fi = 0;
-  HIPCHK(launch_extend(P, q[0], counts + 3, np, fetch + fi++, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb, st));
+  if ((rc = next_fetch(&f))) return rc;
+  HIPCHK(launch_extend(P, q[0], chunks(0), np, f, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb, st));
   if ((rc = tm.mark(T_EXTEND0))) return rc;
   tot.primary_rays += (uint64_t)np;
   tot.launches++;
   // depth 0: every camera sample against its sub-pixel's primary hit
-  HIPCHK(launch_shade(P, B, 1, q[0], counts + 3, ns, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1],
-                      counts + 1, accum, A, ctr, st));
+  HIPCHK(launch_shade(P, B, 1, q[0], chunks(0), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1],
+                      w.slab_ray, slab_cnt + W, accum, A, ctr, st));
+  HIPCHK(index(1, W, w.slab_ray, slab_cnt + W));
   if ((rc = tm.mark(T_SHADE))) return rc;
-  if (nee) HIPCHK(launch_nee(P, B, q[1], counts + 1, w.queue_cap, sq, counts + 2, ctr, st));
-  if ((rc = tm.mark(T_NEE))) return rc;
-  HIPCHK(launch_shadow(P, B, sq, counts + 2, w.shadow_cap, fetch + fi++, accum, A, ctr, tb, st));
-  if ((rc = tm.mark(T_SHADOW))) return rc;
+  if (nee) {
+    HIPCHK(launch_nee(P, B, q[1], chunks(1), sq, w.slab_sh, slab_cnt + 2 * W, ctr, st));
+    HIPCHK(index(2, W, w.slab_sh, slab_cnt + 2 * W));
+    if ((rc = tm.mark(T_NEE))) return rc;
+    if ((rc = next_fetch(&f))) return rc;
+    HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st));
+    if ((rc = tm.mark(T_SHADOW))) return rc;
+  }
   int cur = 1;
   while (true) {
     HIPCHK(hipMemcpyAsync(w.h_counts, counts, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    const uint32_t n = w.h_counts[cur];
+    const uint32_t n = w.h_counts[4 + cur];
     if (n == 0) break;
-    if ((int64_t)n > w.queue_cap || (int64_t)w.h_counts[2] > w.shadow_cap)
-      return fail(PRT_E_INVALID, "ray queue overflow");
-    if (fi + 2 >= 1024) return fail(PRT_E_INVALID, "too many wavefront launches in one batch");
     const int nxt = cur ^ 1;
-    // reset the output queue and the shadow queue for this depth
-    HIPCHK(launch_fill_u32(counts + nxt, 0u, 1, st));
-    HIPCHK(launch_fill_u32(counts + 2, 0u, 1, st));
+    if ((rc = next_fetch(&f))) return rc;
     if ((rc = tm.mark(T_OTHER))) return rc;
-    HIPCHK(launch_extend(P, q[cur], counts + cur, n, fetch + fi++, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb,
-                         st));
+    HIPCHK(launch_extend(P, q[cur], chunks(cur), n, f, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb, st));
     if ((rc = tm.mark(T_EXTEND))) return rc;
     tot.ext_rays += n;
     tot.launches++;
-    HIPCHK(launch_shade(P, B, 0, q[cur], counts + cur, n, (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[nxt],
-                        counts + nxt, accum, A, ctr, st));
+    HIPCHK(launch_shade(P, B, 0, q[cur], chunks(cur), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[nxt],
+                        w.slab_ray, slab_cnt + nxt * W, accum, A, ctr, st));
+    HIPCHK(index(nxt, W, w.slab_ray, slab_cnt + nxt * W));
     if ((rc = tm.mark(T_SHADE))) return rc;
-    if (nee) HIPCHK(launch_nee(P, B, q[nxt], counts + nxt, w.queue_cap, sq, counts + 2, ctr, st));
-    if ((rc = tm.mark(T_NEE))) return rc;
-    HIPCHK(launch_shadow(P, B, sq, counts + 2, w.shadow_cap, fetch + fi++, accum, A, ctr, tb, st));
-    if ((rc = tm.mark(T_SHADOW))) return rc;
+    if (nee) {
+      HIPCHK(launch_nee(P, B, q[nxt], chunks(nxt), sq, w.slab_sh, slab_cnt + 2 * W, ctr, st));
+      HIPCHK(index(2, W, w.slab_sh, slab_cnt + 2 * W));
+      if ((rc = tm.mark(T_NEE))) return rc;
+      if ((rc = next_fetch(&f))) return rc;
+      HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st));
+      if ((rc = tm.mark(T_SHADOW))) return rc;
+    }
     cur = nxt;
   }
   return PRT_OK;
@@ -623,6 +665,12 @@ static int finish_stats(prt_scene* s, const RenderSetup& R, Timer& tm, const Cal
   HIPCHK(hipMemcpyAsync(c, s->ws.ctr.p, CTR_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (c[CTR_DROP] != 0) return fail(PRT_E_INVALID, "ray queue overflow (" + std::to_string(c[CTR_DROP]) + " rays)");
+  // every queued ray must have been traced exactly once
+  if (c[CTR_TRACED] != tot.primary_rays + tot.ext_rays || c[CTR_STRACED] != c[CTR_SHADOW])
+    return fail(PRT_E_INTERNAL, "traversal finished " + std::to_string(c[CTR_TRACED]) + " of " +
+                                    std::to_string(tot.primary_rays + tot.ext_rays) + " rays and " +
+                                    std::to_string(c[CTR_STRACED]) + " of " + std::to_string(c[CTR_SHADOW]) +
+                                    " shadow rays");
   double cat[T_NCAT] = {0, 0, 0, 0, 0, 0};
   tm.collect(cat);
   if (stats) {

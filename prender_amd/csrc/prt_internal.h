@@ -57,21 +57,28 @@ int alloc_shadow_queue(DevBuf* cols, int64_t cap);
 RayQ ray_queue(DevBuf* b, int64_t cap);
 ShadowQ shadow_queue(DevBuf* b, int64_t cap);
 
-// Per-scene render workspace sized for a batch of samples.
+// Per-scene render workspace sized for a batch of samples.  The ray and
+// shadow queues are slab queues (prt_kernels.h): PRT_SLAB_WAVES slabs of
+// slab_ray / slab_sh entries, plus chunk descriptors for each.
 struct Workspace {
   int64_t batch_samples = 0;  // camera samples per batch
-  int64_t queue_cap = 0;      // rays per queue
-  int64_t shadow_cap = 0;
+  int64_t queue_cap = 0;      // entries allocated per ray queue
+  int64_t shadow_cap = 0;     // entries allocated in the shadow queue
+  uint32_t slab_ray = 0, slab_sh = 0;
   DevBuf q[2][RAYQ_COLS];     // two ray queues (SoA)
   DevBuf hit_t, hit_id;
   DevBuf sq[SHADOWQ_COLS];
+  DevBuf desc[3];             // chunk descriptors (uint2) of q[0], q[1], the shadow queue
+  DevBuf slab_cnt;            // uint32 [3][PRT_SLAB_WAVES] slab counts of the same three
   DevBuf accum, accum2;       // fixed-point sample radiance (accum2: adjoint primal)
-  DevBuf counts;              // uint32 [4]: q0 count, q1 count, shadow count, spare
-  DevBuf fetch;               // uint32 [1024] work-fetch counters, one per launch
+  DevBuf counts;              // uint32 [16]: chunks of q0, q1, sq at 0..2, entries at 4..6
+  DevBuf fetch;               // uint32 claim counters, PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE per launch
   DevBuf ctr;                 // uint64 [CTR_COUNT]
   uint32_t* h_counts = nullptr;
   void release();
 };
+// launches whose claim counters one fill clears
+enum { FETCH_GROUP = 64, FETCH_LAUNCHES = 1024 };
 
 // Device-time spans between successive marks, summed per category.
 struct Timer {

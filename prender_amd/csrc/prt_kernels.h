@@ -18,7 +18,9 @@ enum {
   CTR_SNODES = 8,  // BVH nodes fetched by shadow rays
   CTR_STRIS = 9,   // triangle tests of shadow rays
   CTR_REPLAY_MISMATCH = 10,  // adjoint: samples whose replayed radiance differs from the primal
-  CTR_COUNT = 12
+  CTR_TRACED = 11,   // closest-hit queries finished by k_extend (must equal the rays queued)
+  CTR_STRACED = 12,  // shadow rays finished by k_shadow (must equal CTR_SHADOW)
+  CTR_COUNT = 14
 };
 
 struct prt_ray_dev {
@@ -80,19 +82,39 @@ struct Batch {
   int64_t n_pixels;
 };
 
+// Slab queues of the per-scene renderer.  k_shade and k_nee run
+// PRT_SLAB_WAVES waves; wave w writes its outputs to entries
+// [w * slab_cap, w * slab_cap + cnt[w]) of its output queue, counting them in
+// LDS, and stores cnt[w] once at its end, so no counter is shared between
+// waves (same-address atomics serialise across the device).  k_slab_index
+// turns the counts into descriptors of 64-entry chunks, none spanning two
+// slabs, which the consuming kernels walk: k_shade and k_nee in static
+// per-wave ranges, k_extend and k_shadow through PRT_FETCH_COUNTERS claim
+// counters (each 128 bytes apart, one sub-range of the chunks each).
+#define PRT_SLAB_WAVES 4096
+#define PRT_SLAB_INDEX_MAX 8192
+#define PRT_FETCH_COUNTERS 64
+#define PRT_FETCH_STRIDE 32
+struct ChunkList {
+  const uint2* desc;   // [c] = (queue index of the chunk's first entry, its entries 1..64)
+  const uint32_t* n;   // number of chunks (device)
+};
+
 hipError_t launch_trace(const Params& P, const prt_ray_dev* rays, int64_t n, prt_hit_dev* hits, uint64_t* ctr,
                         int max_blocks, hipStream_t s);
 hipError_t launch_gen_primary(const Params& P, const Batch& B, const RayQ& q, hipStream_t s);
-hipError_t launch_extend(const Params& P, const RayQ& q, const uint32_t* count, int64_t n_host, uint32_t* fetch,
+hipError_t launch_slab_index(const uint32_t* slab_cnt, uint32_t n_slabs, uint32_t slab_cap, uint2* desc,
+                             uint32_t max_chunks, uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr,
+                             hipStream_t s);
+hipError_t launch_extend(const Params& P, const RayQ& q, const ChunkList& cl, int64_t n_host, uint32_t* fetch,
                          double* hit_t, int32_t* hit_id, uint64_t* ctr, int max_blocks, hipStream_t s);
-hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const uint32_t* count, int64_t n_max,
-                      const ShadowQ& sq, uint32_t* count_sq, uint64_t* ctr, hipStream_t s);
-hipError_t launch_shadow(const Params& P, const Batch& B, const ShadowQ& q, const uint32_t* count, int64_t n_max,
-                         uint32_t* fetch, uint64_t* accum, const Adjoint& A, uint64_t* ctr, int max_blocks,
-                         hipStream_t s);
-hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const uint32_t* count_in,
-                        int64_t n_host, const double* hit_t, const int32_t* hit_id, const RayQ& qout,
-                        uint32_t* count_out, uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s);
+hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const ChunkList& cl,
+                        const double* hit_t, const int32_t* hit_id, const RayQ& qout, uint32_t slab_cap,
+                        uint32_t* slab_cnt, uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s);
+hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const ChunkList& cl, const ShadowQ& sq,
+                      uint32_t slab_cap, uint32_t* slab_cnt, uint64_t* ctr, hipStream_t s);
+hipError_t launch_shadow(const Params& P, const Batch& B, const ShadowQ& q, const ChunkList& cl, uint32_t* fetch,
+                         uint64_t* accum, const Adjoint& A, uint64_t* ctr, int max_blocks, hipStream_t s);
 hipError_t launch_resolve(const Params& P, const Batch& B, const uint64_t* accum, void* rgb, void* var, int f64,
                           hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);

@@ -28,13 +28,18 @@
 #ifndef PRT_WAVES_NEE
 #define PRT_WAVES_NEE 0
 #endif
+// 1: k_extend / k_shadow refill lanes from a per-wave pool of claimed rays
+// (traverse_pool); 0: one ray per lane until the wave's 64 rays are done
+#ifndef PRT_TRAVERSE_POOL
+#define PRT_TRAVERSE_POOL 1
+#endif
 #define PRT_WAVES_ATTR(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1)))
 
 // ------------------------------------------------------------ trace API kernel
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_trace(
     Params P, const prt_ray_dev* __restrict__ rays, int64_t n, prt_hit_dev* __restrict__ hits, uint64_t* ctr) {
   PRT_DECLARE_STACK(S, P);
-  TravStats ts{0, 0};
+  TravStats ts{0, 0, 0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const prt_ray_dev r = rays[i];
     double t, u, v;
@@ -80,63 +85,61 @@ __device__ __forceinline__ size_t sample_image_index(const Params& P, const Batc
 }
 
 // ------------------------------------------------------------ extend / shadow
-// Closest hit for every ray of a queue; each wave fetches 64 rays at a time.
+// Closest hit for every ray of a chunked queue.
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_extend(
-    Params P, RayQ q, const uint32_t* __restrict__ count_ptr, uint32_t* fetch, double* __restrict__ hit_t,
-    int32_t* __restrict__ hit_id, uint64_t* ctr) {
+    Params P, RayQ q, ChunkList cl, uint32_t* fetch, double* __restrict__ hit_t, int32_t* __restrict__ hit_id,
+    uint64_t* ctr) {
   PRT_DECLARE_STACK(S, P);
-  const uint32_t n = *count_ptr;
-  TravStats ts{0, 0};
-  while (true) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(fetch, 64u);
-    base = (uint32_t)__shfl((int)base, 0, 64);
-    if (base >= n) break;
-    const uint32_t i = base + lane_id();
-    if (i < n) {
-      double t, u, v;
-      int32_t id;
-      closest_hit(P, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), t, id, u, v, S, ts);
-      hit_t[i] = t;
-      hit_id[i] = id;
-    }
-  }
-  flush_stats(ctr, CTR_NODES, ts);
+  const uint32_t n_chunks = *cl.n;
+  TravStats ts{0, 0, 0};
+  traverse_pool<false>(
+      P, n_chunks, cl.desc, fetch, S, ts,
+      [&](uint32_t j, d3& o, d3& d, double& tmax) {
+        o = mk(q.ox[j], q.oy[j], q.oz[j]);
+        d = mk(q.dx[j], q.dy[j], q.dz[j]);
+        tmax = PS_INF;
+      },
+      [&](uint32_t j, double t, int32_t id, bool) {
+        hit_t[j] = t;
+        hit_id[j] = id;
+      });
+  flush_stats(ctr, CTR_NODES, ts, CTR_TRACED);
 }
 
 // Any hit for NEE shadow rays; an unoccluded ray adds its contribution to the
 // camera sample's accumulator.
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_shadow(
-    Params P, Batch B, ShadowQ q, const uint32_t* __restrict__ count_ptr, uint32_t* fetch,
-    uint64_t* __restrict__ accum, Adjoint A, uint64_t* ctr) {
+    Params P, Batch B, ShadowQ q, ChunkList cl, uint32_t* fetch, uint64_t* __restrict__ accum, Adjoint A,
+    uint64_t* ctr) {
   PRT_DECLARE_STACK(S, P);
-  const uint32_t n = *count_ptr;
-  TravStats ts{0, 0};
+  const uint32_t n_chunks = *cl.n;
+  TravStats ts{0, 0, 0};
   double ge[3] = {0.0, 0.0, 0.0};
-  while (true) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(fetch, 64u);
-    base = (uint32_t)__shfl((int)base, 0, 64);
-    if (base >= n) break;
-    const uint32_t i = base + lane_id();
-    if (i < n) {
-      if (!occluded(P, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), q.tmax[i], S, ts)) {
-        const uint32_t s = q.slot[i];
-        const float c[3] = {q.cr[i], q.cg[i], q.cb[i]};
-        accum_add(&accum[3 * (size_t)s + 0], c[0]);
-        accum_add(&accum[3 * (size_t)s + 1], c[1]);
-        accum_add(&accum[3 * (size_t)s + 2], c[2]);
-        if (A.on && q.light[i] == A.light) {
-          // d(contribution)/d(emission) = contribution / emission
-          const double* E = P.mats[P.ents[A.light].material].emission;
-          const size_t px = sample_image_index(P, B, s);
-          for (int k = 0; k < 3; ++k)
-            if (E[k] != 0.0) ge[k] += (double)A.dimage[3 * px + k] * A.scale * (double)c[k] / E[k];
-        }
-      }
+  auto lit = [&](uint32_t i) {
+    const uint32_t s = q.slot[i];
+    const float c[3] = {q.cr[i], q.cg[i], q.cb[i]};
+    accum_add(&accum[3 * (size_t)s + 0], c[0]);
+    accum_add(&accum[3 * (size_t)s + 1], c[1]);
+    accum_add(&accum[3 * (size_t)s + 2], c[2]);
+    if (A.on && q.light[i] == A.light) {
+      // d(contribution)/d(emission) = contribution / emission
+      const double* E = P.mats[P.ents[A.light].material].emission;
+      const size_t px = sample_image_index(P, B, s);
+      for (int k = 0; k < 3; ++k)
+        if (E[k] != 0.0) ge[k] += (double)A.dimage[3 * px + k] * A.scale * (double)c[k] / E[k];
     }
-  }
-  flush_stats(ctr, CTR_SNODES, ts);
+  };
+  traverse_pool<true>(
+      P, n_chunks, cl.desc, fetch, S, ts,
+      [&](uint32_t j, d3& o, d3& d, double& tmax) {
+        o = mk(q.ox[j], q.oy[j], q.oz[j]);
+        d = mk(q.dx[j], q.dy[j], q.dz[j]);
+        tmax = q.tmax[j];
+      },
+      [&](uint32_t j, double, int32_t, bool blocked) {
+        if (!blocked) lit(j);
+      });
+  flush_stats(ctr, CTR_SNODES, ts, CTR_STRACED);
   if (A.on) {
     for (int k = 0; k < 3; ++k) {
       const double w = wave_sum_f64(ge[k]);
@@ -147,22 +150,40 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_shad
 
 // ------------------------------------------------------------ shade
 // Shade one queue (or, with primary = 1, every camera sample of the batch
-// against the hit of its sub-pixel's primary ray).
+// against the hit of its sub-pixel's primary ray).  ADJ: with the adjoint
+// bookkeeping of a path replay (a separate instance, so forward renders
+// carry none of its registers).
+template <bool ADJ>
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
-    Params P, Batch B, int primary, RayQ qin, const uint32_t* __restrict__ count_in, const double* __restrict__ hit_t,
-    const int32_t* __restrict__ hit_id, RayQ qout, uint32_t* count_out, uint64_t* __restrict__ accum, Adjoint A,
-    uint64_t* ctr) {
-  const uint32_t n = primary ? (uint32_t)(B.n_pixels * P.spp) : *count_in;
-  const uint32_t stride = gridDim.x * blockDim.x;
+    Params P, Batch B, int primary, RayQ qin, ChunkList cl, const double* __restrict__ hit_t,
+    const int32_t* __restrict__ hit_id, RayQ qout, uint32_t slab_cap, uint32_t* slab_cnt,
+    uint64_t* __restrict__ accum, Adjoint A, uint64_t* ctr) {
+  __shared__ uint32_t s_cnt[4];  // children appended by each wave of the block
+  const uint32_t wib = threadIdx.x >> 6;
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (lane_id() == 0) s_cnt[wib] = 0;
+  const uint32_t n_samples = (uint32_t)(B.n_pixels * P.spp);
+  const uint32_t n_chunks = primary ? (n_samples + 63u) >> 6 : *cl.n;
   const int ss2 = P.SS * P.SS;
   ShadeCtx C;
   C.qout = qout;
-  C.count_out = count_out;
-  C.off = 0;
-  C.cap = qout.cap;
+  C.count_out = &s_cnt[wib];
+  C.off = wave * slab_cap;
+  C.cap = slab_cap;
   C.truncated = C.n_ext = C.pickups = C.dropped = 0;
   double ge[3] = {0.0, 0.0, 0.0};
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  uint32_t c, c_end;
+  wave_range(n_chunks, c, c_end);
+  for (; c < c_end; ++c) {
+    uint32_t i;
+    if (primary) {
+      i = (c << 6) + lane_id();
+      if (i >= n_samples) continue;
+    } else {
+      const uint2 dc = cl.desc[c];
+      if (lane_id() >= dc.y) continue;
+      i = dc.x + lane_id();
+    }
     d3 o, d, T, Pw;
     double t;
     int32_t id;
@@ -197,7 +218,7 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
     const uint32_t pl = slot / P.spp, k = slot % P.spp;
     VertexAdj va;
     const d3 add = shade_vertex(P, C, o, d, t, id, T, Pw, (uint32_t)(B.pixel_base + pl), k, depth, branch, va);
-    if (A.on) {
+    if (ADJ && A.on) {
       const size_t px = sample_image_index(P, B, slot);
       const d3 g = mk(A.dimage[3 * px] * A.scale, A.dimage[3 * px + 1] * A.scale, A.dimage[3 * px + 2] * A.scale);
       if (va.albedo && va.entity == A.entity && va.face >= 0) {
@@ -223,7 +244,7 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
     accum_add(&accum[3 * (size_t)slot + 1], add.y);
     accum_add(&accum[3 * (size_t)slot + 2], add.z);
   }
-  if (A.on) {
+  if (ADJ && A.on) {
     for (int c = 0; c < 3; ++c) {
       const double w = wave_sum_f64(ge[c]);
       if (lane_id() == 0 && w != 0.0) atomicAdd(&A.grad_e[c], w);
@@ -232,6 +253,7 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
   const uint64_t s_pick = wave_sum_u32(C.pickups), s_tr = wave_sum_u32(C.truncated);
   const uint64_t s_ext = wave_sum_u32(C.n_ext), s_drop = wave_sum_u32(C.dropped);
   if (lane_id() == 0) {
+    slab_cnt[wave] = min(s_cnt[wib], slab_cap);
     if (s_pick) atomicAdd((unsigned long long*)&ctr[CTR_PICKUP], (unsigned long long)s_pick);
     if (s_tr) atomicAdd((unsigned long long*)&ctr[CTR_TRUNC], (unsigned long long)s_tr);
     if (s_ext) atomicAdd((unsigned long long*)&ctr[CTR_EXT], (unsigned long long)s_ext);
@@ -245,32 +267,45 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
 // vertex's random numbers.  Each wave appends its shadow rays with one prefix
 // sum and one atomic.
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
-    Params P, Batch B, RayQ q, const uint32_t* __restrict__ count_ptr, ShadowQ sq, uint32_t* count_sq, uint64_t* ctr) {
-  const uint32_t n = *count_ptr;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t n_round = (n + 63u) & ~63u;
+    Params P, Batch B, RayQ q, ChunkList cl, ShadowQ sq, uint32_t slab_cap, uint32_t* slab_cnt, uint64_t* ctr) {
+  __shared__ uint32_t s_cnt[4];  // shadow rays appended by each wave of the block
+  const uint32_t wib = threadIdx.x >> 6;
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (lane_id() == 0) s_cnt[wib] = 0;
+  const uint32_t off = wave * slab_cap;
   uint32_t emitted = 0, dropped = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
-    ShadowRay s0, s1, s2;
-    bool v0 = false, v1 = false, v2 = false;
-    int32_t l0 = -1, l1 = -1, l2 = -1;
-    uint32_t slot = 0;
-    if (i < n) {
+  uint32_t c, c_end;
+  wave_range(*cl.n, c, c_end);
+  for (; c < c_end; ++c) {
+    const uint2 dc = cl.desc[c];
+    bool have = false;
+    uint32_t slot = 0, w[9];
+    d3 x, nrm, ct;
+    if (lane_id() < dc.y) {
+      const uint32_t i = dc.x + lane_id();
       const uint32_t meta = q.meta[i];
       if (meta & META_NEE) {
+        have = true;
         slot = q.slot[i];
         const uint32_t pl = slot / P.spp, k = slot % P.spp;
-        nee_vertex(P, (uint32_t)(B.pixel_base + pl), k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta),
-                   mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.nx[i], q.ny[i], q.nz[i]), mk(q.tr[i], q.tg[i], q.tb[i]), s0,
-                   s1, s2, v0, v1, v2, l0, l1, l2);
+        nee_words(P, (uint32_t)(B.pixel_base + pl), k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta), w);
+        x = mk(q.ox[i], q.oy[i], q.oz[i]);
+        nrm = mk(q.nx[i], q.ny[i], q.nz[i]);
+        ct = mk(q.tr[i], q.tg[i], q.tb[i]);
       }
     }
+    ShadowRay s0, s1, s2;
+    int32_t l0 = -1, l1 = -1, l2 = -1;
+    const bool v0 = have && nee_one(P, w, 0, x, nrm, ct, s0, l0);
+    const bool v1 = have && nee_one(P, w, 1, x, nrm, ct, s1, l1);
+    const bool v2 = have && nee_one(P, w, 2, x, nrm, ct, s2, l2);
     const uint32_t cnt = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
-    uint32_t j = wave_append_n(count_sq, cnt);
-    if (j + cnt > sq.cap) {
+    uint32_t j = wave_append_n(&s_cnt[wib], cnt);
+    if (j + cnt > slab_cap) {
       dropped += cnt;
       continue;
     }
+    j += off;
     if (v0) put_shadow(sq, j++, s0, slot, l0);
     if (v1) put_shadow(sq, j++, s1, slot, l1);
     if (v2) put_shadow(sq, j++, s2, slot, l2);
@@ -278,8 +313,80 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
   }
   const uint64_t s_sh = wave_sum_u32(emitted), s_drop = wave_sum_u32(dropped);
   if (lane_id() == 0) {
+    slab_cnt[wave] = min(s_cnt[wib], slab_cap);
     if (s_sh) atomicAdd((unsigned long long*)&ctr[CTR_SHADOW], (unsigned long long)s_sh);
     if (s_drop) atomicAdd((unsigned long long*)&ctr[CTR_DROP], (unsigned long long)s_drop);
+  }
+}
+
+// ------------------------------------------------------------ slab index
+// Chunk descriptors of a slab queue (one block of 1024 threads): slab s holds
+// min(cnt[s], slab_cap) entries from s * slab_cap on and ceil(that / 64)
+// chunks; chunks are numbered slab by slab.  Also stores the number of chunks
+// and of entries.  More chunks than max_chunks (never, with the host's
+// sizing) are dropped and counted.
+__global__ __launch_bounds__(1024) void k_slab_index(const uint32_t* __restrict__ cnt, uint32_t n_slabs,
+                                                     uint32_t slab_cap, uint2* __restrict__ desc, uint32_t max_chunks,
+                                                     uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr) {
+  __shared__ uint32_t s_pref[PRT_SLAB_INDEX_MAX + 1];
+  __shared__ uint32_t s_wch[16], s_went[16];
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+  const uint32_t per = (n_slabs + 1023u) / 1024u;
+  const uint32_t b = min(tid * per, n_slabs), e = min(b + per, n_slabs);
+  uint32_t ch = 0, ent = 0;
+  for (uint32_t s = b; s < e; ++s) {
+    const uint32_t c = min(cnt[s], slab_cap);
+    ch += (c + 63u) >> 6;
+    ent += c;
+  }
+  // block exclusive scan of (ch, ent): within waves, then over the 16 waves
+  uint32_t ich = ch, ient = ent;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t a1 = (uint32_t)__shfl_up((int)ich, o, 64), a2 = (uint32_t)__shfl_up((int)ient, o, 64);
+    if (lane >= (uint32_t)o) {
+      ich += a1;
+      ient += a2;
+    }
+  }
+  if (lane == 63) {
+    s_wch[wv] = ich;
+    s_went[wv] = ient;
+  }
+  __syncthreads();
+  uint32_t wb_ch = 0, wb_ent = 0, tot_ch = 0, tot_ent = 0;
+  for (uint32_t k = 0; k < 16; ++k) {
+    if (k < wv) {
+      wb_ch += s_wch[k];
+      wb_ent += s_went[k];
+    }
+    tot_ch += s_wch[k];
+    tot_ent += s_went[k];
+  }
+  uint32_t run = wb_ch + ich - ch;
+  for (uint32_t s = b; s < e; ++s) {
+    s_pref[s] = run;
+    run += (min(cnt[s], slab_cap) + 63u) >> 6;
+  }
+  if (tid == 0) s_pref[n_slabs] = tot_ch;
+  __syncthreads();
+  const uint32_t total = min(tot_ch, max_chunks);
+  for (uint32_t c = tid; c < total; c += 1024u) {
+    uint32_t lo = 0, hi = n_slabs;  // the slab s with s_pref[s] <= c < s_pref[s + 1]
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (s_pref[mid] <= c) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t k = c - s_pref[lo];
+    const uint32_t cs = min(cnt[lo], slab_cap);
+    desc[c] = make_uint2(lo * slab_cap + (k << 6), min(64u, cs - (k << 6)));
+  }
+  if (tid == 0) {
+    *n_chunks = total;
+    *n_entries = tot_ent;
+    if (tot_ch > max_chunks)
+      atomicAdd((unsigned long long*)&ctr[CTR_DROP], (unsigned long long)(tot_ch - max_chunks) * 64ull);
   }
 }
 
@@ -355,29 +462,40 @@ hipError_t launch_gen_primary(const Params& P, const Batch& B, const RayQ& q, hi
   hipLaunchKernelGGL(k_gen_primary, dim3(grid_for(B.n_pixels * P.SS * P.SS)), dim3(256), 0, s, P, B, q);
   return hipGetLastError();
 }
-hipError_t launch_extend(const Params& P, const RayQ& q, const uint32_t* count, int64_t n_host, uint32_t* fetch,
+hipError_t launch_extend(const Params& P, const RayQ& q, const ChunkList& cl, int64_t n_host, uint32_t* fetch,
                          double* hit_t, int32_t* hit_id, uint64_t* ctr, int max_blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_extend, dim3(grid_for(n_host, 256, max_blocks)), dim3(256), 0, s, P, q, count, fetch, hit_t,
+  hipLaunchKernelGGL(k_extend, dim3(grid_for(n_host, 256, max_blocks)), dim3(256), 0, s, P, q, cl, fetch, hit_t,
                      hit_id, ctr);
   return hipGetLastError();
 }
-hipError_t launch_shadow(const Params& P, const Batch& B, const ShadowQ& q, const uint32_t* count, int64_t n_max,
-                         uint32_t* fetch, uint64_t* accum, const Adjoint& A, uint64_t* ctr, int max_blocks,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid_for(n_max, 256, max_blocks)), dim3(256), 0, s, P, B, q, count, fetch, accum,
-                     A, ctr);
+hipError_t launch_shadow(const Params& P, const Batch& B, const ShadowQ& q, const ChunkList& cl, uint32_t* fetch,
+                         uint64_t* accum, const Adjoint& A, uint64_t* ctr, int max_blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_shadow, dim3(max_blocks), dim3(256), 0, s, P, B, q, cl, fetch, accum, A, ctr);
   return hipGetLastError();
 }
-hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const uint32_t* count_in,
-                        int64_t n_host, const double* hit_t, const int32_t* hit_id, const RayQ& qout,
-                        uint32_t* count_out, uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s) {
-  hipLaunchKernelGGL(k_shade, dim3(grid_for(n_host)), dim3(256), 0, s, P, B, primary, qin, count_in, hit_t, hit_id,
-                     qout, count_out, accum, A, ctr);
+hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const ChunkList& cl,
+                        const double* hit_t, const int32_t* hit_id, const RayQ& qout, uint32_t slab_cap,
+                        uint32_t* slab_cnt, uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s) {
+  const dim3 grid(PRT_SLAB_WAVES / 4);
+  if (A.on)
+    hipLaunchKernelGGL(k_shade<true>, grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, slab_cap,
+                       slab_cnt, accum, A, ctr);
+  else
+    hipLaunchKernelGGL(k_shade<false>, grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, slab_cap,
+                       slab_cnt, accum, A, ctr);
   return hipGetLastError();
 }
-hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const uint32_t* count, int64_t n_max,
-                      const ShadowQ& sq, uint32_t* count_sq, uint64_t* ctr, hipStream_t s) {
-  hipLaunchKernelGGL(k_nee, dim3(grid_for(n_max, 256, 8192)), dim3(256), 0, s, P, B, q, count, sq, count_sq, ctr);
+hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const ChunkList& cl, const ShadowQ& sq,
+                      uint32_t slab_cap, uint32_t* slab_cnt, uint64_t* ctr, hipStream_t s) {
+  hipLaunchKernelGGL(k_nee, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slab_cap, slab_cnt, ctr);
+  return hipGetLastError();
+}
+hipError_t launch_slab_index(const uint32_t* slab_cnt, uint32_t n_slabs, uint32_t slab_cap, uint2* desc,
+                             uint32_t max_chunks, uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr,
+                             hipStream_t s) {
+  if (n_slabs == 0 || n_slabs > PRT_SLAB_INDEX_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_slab_index, dim3(1), dim3(1024), 0, s, slab_cnt, n_slabs, slab_cap, desc, max_chunks, n_chunks,
+                     n_entries, ctr);
   return hipGetLastError();
 }
 hipError_t launch_resolve(const Params& P, const Batch& B, const uint64_t* accum, void* rgb, void* var, int f64,

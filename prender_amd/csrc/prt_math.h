@@ -101,6 +101,37 @@ __device__ __forceinline__ bool mt_test(const float4 a, const float4 b, const fl
   return true;
 }
 
+// sin(2 pi u) and cos(2 pi u) for u in [0, 1].  The quadrant q = rint(4u) is
+// split off exactly (u is a multiple of 2^-32, so u - q/4 is exact), leaving
+// y = 2 pi (u - q/4) in [-pi/4, pi/4], where the Taylor series cut after
+// y^15 / 15! (sine) and y^16 / 16! (cosine) are accurate to below half an
+// ulp; the quadrant then rotates (sin y, cos y).  Differs from sincos(2 pi u)
+// by a few ulps at most, with no large-argument reduction path (which holds
+// most of a general sincos's registers).
+__device__ __forceinline__ void sincos_2pi(double u, double& s, double& c) {
+  const double q = rint(4.0 * u);
+  const double y = PS_TWOPI * (u - 0.25 * q);
+  const double z = y * y;
+  const double sy =
+      y + y * z *
+              (-0.16666666666666666 +
+               z * (0.008333333333333333 +
+                    z * (-0.0001984126984126984 +
+                         z * (2.7557319223985893e-06 +
+                              z * (-2.505210838544172e-08 + z * (1.6059043836821613e-10 + z * -7.647163731819816e-13))))));
+  const double cy =
+      1.0 + z * (-0.5 +
+                 z * (0.041666666666666664 +
+                      z * (-0.001388888888888889 +
+                           z * (2.48015873015873e-05 +
+                                z * (-2.755731922398589e-07 +
+                                     z * (2.08767569878681e-09 +
+                                          z * (-1.1470745597729725e-11 + z * 4.779477332387385e-14)))))));
+  const int k = (int)q & 3;
+  s = (k == 0) ? sy : (k == 1) ? cy : (k == 2) ? -sy : -cy;
+  c = (k == 0) ? cy : (k == 1) ? -sy : (k == 2) ? -cy : sy;
+}
+
 // Sphere::intersect, sphere.h:68-87
 __device__ __forceinline__ bool sphere_t(const DevEntity& e, d3 o, d3 d, double& t) {
   const d3 p_o = ld3(e.p0) - o;
@@ -181,6 +212,30 @@ __device__ __forceinline__ uint32_t wave_append_n(uint32_t* counter, uint32_t n)
     base = (uint32_t)__shfl((int)base, 63, 64);
   }
   return base + incl - n;
+}
+
+// Append one entry per lane with v set (the wave converged): one atomic per
+// wave, rank among the set lanes.
+__device__ __forceinline__ uint32_t wave_append_bit(uint32_t* counter, bool v) {
+  const uint64_t m = __ballot(v);
+  const uint32_t total = (uint32_t)__popcll(m);
+  uint32_t base = 0;
+  if (total > 0) {
+    if (lane_id() == 0) base = atomicAdd(counter, total);
+    base = (uint32_t)__shfl((int)base, 0, 64);
+  }
+  return base + (uint32_t)__popcll(m & ((1ull << lane_id()) - 1ull));
+}
+
+// Static contiguous range [c0, c1) of n_chunks chunks for this wave of the
+// grid, for kernels whose work per chunk is too short to pay for a fetch
+// atomic.
+__device__ __forceinline__ void wave_range(uint32_t n_chunks, uint32_t& c0, uint32_t& c1) {
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t per = (n_chunks + n_waves - 1) / n_waves;
+  c0 = min(wave * per, n_chunks);
+  c1 = min(c0 + per, n_chunks);
 }
 
 // Append from divergent code: the active lanes share one atomic (ballot,

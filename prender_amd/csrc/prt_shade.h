@@ -171,10 +171,10 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
     const d3 w = n_orient;
     const d3 uu = (fabs(w.x) > PS_EPS) ? normalize(cross(mk(0.0, 1.0, 0.0), w)) : normalize(cross(mk(1.0, 0.0, 0.0), w));
     const d3 vv = normalize(cross(w, uu));
-    const double r1 = PS_TWOPI * u01(b0[2]);
+    // r1 = 2 pi u (brdf.h:498); sin r1, cos r1 without a general reduction
     const double r2 = u01(b0[3]), r2s = sqrt(r2);
     double s1, c1;
-    sincos(r1, &s1, &c1);
+    sincos_2pi(u01(b0[2]), s1, c1);
     const d3 dir = normalize((uu * c1 * r2s + vv * s1 * r2s + w * sqrt(1.0 - r2)));
     const d3 ct = vmul(T, color / rr / (1.0 - 0.0));
     const bool nee = P.nee && P.n_lights > 0;
@@ -209,7 +209,8 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
   const double a = nt - nc, b = nt + nc;
   const double R0 = (a * a) / (b * b);
   const double c = 1.0 - (into ? -ddn : dot(tdir, -1.0 * n_orient));
-  const double Re = R0 + (1.0 - R0) * pow(c, 5.0);
+  const double c_2 = c * c;
+  const double Re = R0 + (1.0 - R0) * (c_2 * c_2 * c);  // pow(c, 5.0) to an ulp or two
   const double q = into ? nc / nt : nt / nc;
   const double nnt2 = q * q;  // pow(q, 2.0), exact
   const double Tr = (1.0 - Re) * nnt2;
@@ -248,11 +249,11 @@ __device__ __forceinline__ bool nee_sample(const Params& P, int li, double ra, d
   const DevEntity& le = P.ents[li];
   d3 light_pos, light_normal;
   if (le.type == ENT_SPHERE) {
-    const double r1 = PS_TWOPI * ra;
+    // r1 = 2 pi ra
     const double r2 = 1.0 - 2.0 * rb;
     const double s22 = sqrt(1.0 - r2 * r2);
     double sn, cs;
-    sincos(r1, &sn, &cs);
+    sincos_2pi(ra, sn, cs);
     light_pos = ld3(le.p0) + ((le.radius + PS_EPS) * mk(s22 * cs, s22 * sn, r2));
     light_normal = normalize(light_pos - ld3(le.p0));
   } else {
@@ -295,34 +296,32 @@ __device__ __forceinline__ void put_shadow(ShadowQ& q, uint32_t j, const ShadowR
   q.slot[j] = slot;
 }
 
-// The three light samples of a diffuse vertex at x (orienting normal nrm,
-// throughput ct = T*colour/rr), with the parent vertex's random numbers
-// (dimensions 4..12).  Returns how many shadow rays it produced.
-__device__ __forceinline__ uint32_t nee_vertex(const Params& P, uint32_t pixel, uint32_t sample, int depth,
-                                               uint32_t branch, d3 x, d3 nrm, d3 ct, ShadowRay& s0, ShadowRay& s1,
-                                               ShadowRay& s2, bool& v0, bool& v1, bool& v2, int32_t& l0, int32_t& l1,
-                                               int32_t& l2) {
-  const int NL = P.n_lights;
+// Random words of the three light samples of a diffuse vertex (dimensions
+// 4..12, drawn with the parent vertex's counter): sample m picks its light
+// with w[3m] and its point with w[3m+1], w[3m+2].
+__device__ __forceinline__ void nee_words(const Params& P, uint32_t pixel, uint32_t sample, int depth,
+                                          uint32_t branch, uint32_t w[9]) {
   const uint32_t c2 = rng_c2(depth, branch);
   uint32_t b1[4], b2[4], b3[4];
   philox(pixel, sample, c2, 1u, P.key0, P.key1, b1);
   philox(pixel, sample, c2, 2u, P.key0, P.key1, b2);
   philox(pixel, sample, c2, 3u, P.key0, P.key1, b3);
-  // LightList::randomLight (light.h:146-151): index = (int)(u*(N-1)+0.5),
-  // probability 1; dimensions 4+3i (pick), 5+3i and 6+3i (point)
-  int i0 = (int)(u01(b1[0]) * (double)(NL - 1) + 0.5);
-  int i1 = (int)(u01(b1[3]) * (double)(NL - 1) + 0.5);
-  int i2 = (int)(u01(b2[2]) * (double)(NL - 1) + 0.5);
-  i0 = i0 >= NL ? NL - 1 : i0;
-  i1 = i1 >= NL ? NL - 1 : i1;
-  i2 = i2 >= NL ? NL - 1 : i2;
-  l0 = P.lights[i0];
-  l1 = P.lights[i1];
-  l2 = P.lights[i2];
-  v0 = nee_sample(P, l0, u01(b1[1]), u01(b1[2]), x, nrm, ct, s0);
-  v1 = nee_sample(P, l1, u01(b2[0]), u01(b2[1]), x, nrm, ct, s1);
-  v2 = nee_sample(P, l2, u01(b2[3]), u01(b3[0]), x, nrm, ct, s2);
-  return (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
+  w[0] = b1[0]; w[1] = b1[1]; w[2] = b1[2]; w[3] = b1[3];
+  w[4] = b2[0]; w[5] = b2[1]; w[6] = b2[2]; w[7] = b2[3];
+  w[8] = b3[0];
+}
+
+// Light sample m (0..2) of a diffuse vertex at x (orienting normal nrm,
+// throughput ct = T*colour/rr): LightList::randomLight (light.h:146-151),
+// index (int)(u*(N-1)+0.5) with probability 1, then nee_sample.  l receives
+// the light's entity index.
+__device__ __forceinline__ bool nee_one(const Params& P, const uint32_t w[9], int m, d3 x, d3 nrm, d3 ct,
+                                        ShadowRay& s, int32_t& l) {
+  const int NL = P.n_lights;
+  int i = (int)(u01(w[3 * m]) * (double)(NL - 1) + 0.5);
+  i = i >= NL ? NL - 1 : i;
+  l = P.lights[i];
+  return nee_sample(P, l, u01(w[3 * m + 1]), u01(w[3 * m + 2]), x, nrm, ct, s);
 }
 
 // Luminance of mlt.h:171-174 with the Vector3d dot pairing.

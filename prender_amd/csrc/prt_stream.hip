@@ -335,17 +335,6 @@ __device__ __forceinline__ uint32_t fetch_chunks(uint32_t* counter) {
   return (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)c, 0, 64));
 }
 
-// Static split of n_chunks chunks into one contiguous range per wave of the
-// grid (for kernels whose work per chunk is too short to pay for a fetch
-// atomic on one shared counter).
-__device__ __forceinline__ void wave_range(uint32_t n_chunks, uint32_t& c0, uint32_t& c1) {
-  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t per = (n_chunks + n_waves - 1) / n_waves;
-  c0 = min(wave * per, n_chunks);
-  c1 = min(c0 + per, n_chunks);
-}
-
 // Closest hit for the current queue through its work table.
 template <int NL>
 __global__ __launch_bounds__(256) void k_stream_extend(Stream st, RayQ q, double* __restrict__ hit_t,
@@ -353,7 +342,7 @@ __global__ __launch_bounds__(256) void k_stream_extend(Stream st, RayQ q, double
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL);
   const uint32_t n_chunks = st.ctl->work_width >> 6, n_work = st.ctl->n_work;
   uint32_t* fetch = &st.ctl->fetch[0];
-  TravStats ts{0, 0};
+  TravStats ts{0, 0, 0};
   while (true) {
     uint32_t c = fetch_chunks(fetch);
     if (c >= n_chunks) break;
@@ -514,24 +503,30 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
       const Params& P = scene_params(st.scenes, (int)s);
       for (; c < run_end; ++c) {
         const uint32_t local = (c << 6) - s_base + lane_id();
-        ShadowRay s0, s1, s2;
-        bool v0 = false, v1 = false, v2 = false;
-        int32_t l0 = -1, l1 = -1, l2 = -1;
-        uint32_t slot = 0;
+        bool have = false;
+        uint32_t slot = 0, w[9];
+        d3 x, nrm, ct;
         if (local < cnt) {
           const uint32_t i = off + local;
           const uint32_t meta = q.meta[i];
           if (meta & META_NEE) {
+            have = true;
             slot = q.slot[i];
             const SlotInfo si = st.slot_info[slot >> shift];
             const uint32_t r = slot & tmask;
             const uint32_t k = r % (uint32_t)si.spp;
             const int64_t pixel = si.pixel_base + r / (uint32_t)si.spp;
-            nee_vertex(P, (uint32_t)pixel, k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta),
-                       mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.nx[i], q.ny[i], q.nz[i]), mk(q.tr[i], q.tg[i], q.tb[i]),
-                       s0, s1, s2, v0, v1, v2, l0, l1, l2);
+            nee_words(P, (uint32_t)pixel, k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta), w);
+            x = mk(q.ox[i], q.oy[i], q.oz[i]);
+            nrm = mk(q.nx[i], q.ny[i], q.nz[i]);
+            ct = mk(q.tr[i], q.tg[i], q.tb[i]);
           }
         }
+        ShadowRay s0, s1, s2;
+        int32_t l0 = -1, l1 = -1, l2 = -1;
+        const bool v0 = have && nee_one(P, w, 0, x, nrm, ct, s0, l0);
+        const bool v1 = have && nee_one(P, w, 1, x, nrm, ct, s1, l1);
+        const bool v2 = have && nee_one(P, w, 2, x, nrm, ct, s2, l2);
         uint32_t n_sh = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
         uint32_t j = wave_append_n(&st.ctl->cnt_sq, n_sh);
         if (j + n_sh > sq.cap) {
@@ -541,8 +536,8 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
           if (v0) put_shadow(sq, j++, s0, slot, l0);
           if (v1) put_shadow(sq, j++, s1, slot, l1);
           if (v2) put_shadow(sq, j++, s2, slot, l2);
-          emitted += n_sh;
         }
+        emitted += n_sh;
         live_update(st, n_sh > 0 ? (int)(slot >> shift) : -1, (int)n_sh);
       }
     }
@@ -562,7 +557,7 @@ __global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq) {
   const uint32_t n = min(st.ctl->cnt_sq, sq.cap);
   uint32_t* fetch = &st.ctl->fetch[1];
   const uint32_t shift = (uint32_t)st.tile_shift;
-  TravStats ts{0, 0};
+  TravStats ts{0, 0, 0};
   uint32_t extra = 0;
   while (true) {
     uint32_t base = 0;

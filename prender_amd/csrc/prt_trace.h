@@ -11,6 +11,7 @@
 
 struct TravStats {
   uint32_t nodes, tris;
+  uint32_t rays;  // queries finished (traverse_pool)
 };
 
 // Per-lane traversal stack: the first NL entries in LDS (lane interleaved,
@@ -60,11 +61,139 @@ __device__ __forceinline__ float fup(double t) {
   return (double)f < t ? __uint_as_float(__float_as_uint(f) + 1u) : f;
 }
 
+// 1: while-while traversal (traverse_ww); 0: one node or leaf per loop trip
+#ifndef PRT_TRAVERSE_WW
+#define PRT_TRAVERSE_WW 1
+#endif
+
+// Slab test of the 4 children of node `cur` against the ray's float slabs:
+// entry distances sorted ascending with their child references (+inf: miss).
+__device__ __forceinline__ void node_children(const DevNode* nodes, int32_t cur, float ix, float iy, float iz,
+                                              float oix, float oiy, float oiz, float tb, float tn[4],
+                                              int32_t cc[4]) {
+  const float4* nd = reinterpret_cast<const float4*>(nodes + cur);
+  const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+  const int4 ch = *reinterpret_cast<const int4*>(nd + 6);
+  const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
+  const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+  const int32_t cha[4] = {ch.x, ch.y, ch.z, ch.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const float x0 = __builtin_fmaf(lxa[k], ix, -oix), x1 = __builtin_fmaf(hxa[k], ix, -oix);
+    const float y0 = __builtin_fmaf(lya[k], iy, -oiy), y1 = __builtin_fmaf(hya[k], iy, -oiy);
+    const float z0 = __builtin_fmaf(lza[k], iz, -oiz), z1 = __builtin_fmaf(hza[k], iz, -oiz);
+    const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+    const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+    tn[k] = (tnear <= tfar) ? tnear : __builtin_inff();
+    cc[k] = cha[k];
+  }
+#define PRT_CSWAP(a, b)                  \
+  if (tn[b] < tn[a]) {                   \
+    const float tt = tn[a];              \
+    tn[a] = tn[b];                       \
+    tn[b] = tt;                          \
+    const int32_t c_ = cc[a];            \
+    cc[a] = cc[b];                       \
+    cc[b] = c_;                          \
+  }
+  PRT_CSWAP(0, 1) PRT_CSWAP(2, 3) PRT_CSWAP(0, 2) PRT_CSWAP(1, 3) PRT_CSWAP(1, 2)
+#undef PRT_CSWAP
+}
+
+// While-while traversal with speculative leaf postponement (Aila & Laine,
+// "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009): a lane
+// walks internal nodes until it meets a leaf, parks that leaf and keeps
+// walking until every lane of the wave holds a parked leaf (or is done);
+// then the whole wave tests triangles together.  Node visits and triangle
+// tests thus run with most lanes active instead of interleaving a node
+// step of some lanes with a leaf loop of others in every trip.
+//
+// Every triangle a ray tests is one whose box the conservative culling
+// leaves open, and a hit is kept only when strictly closer, so the closest
+// hit equals traverse()'s except which of several triangles at exactly the
+// same t is reported (the test order differs).
+#define PRT_TRAV_DONE 0x7FFFFFFF
+template <bool ANY, class Stk>
+__device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double& t_best, int32_t& id_best, double& ub,
+                                            double& vb, Stk& S, TravStats& ts) {
+  if (P.n_nodes <= 0) return false;
+  const float ox = (float)o.x, oy = (float)o.y, oz = (float)o.z;
+  float ix = (float)(1.0 / d.x), iy = (float)(1.0 / d.y), iz = (float)(1.0 / d.z);
+  if (__builtin_isinf(ix)) ix = __builtin_copysignf(1.0e30f, ix);
+  if (__builtin_isinf(iy)) iy = __builtin_copysignf(1.0e30f, iy);
+  if (__builtin_isinf(iz)) iz = __builtin_copysignf(1.0e30f, iz);
+  const float oix = ox * ix, oiy = oy * iy, oiz = oz * iz;
+  float tb = t_best < 1.0e30 ? fup(t_best) : __builtin_inff();
+  bool found = false;
+  int sp = 0;
+  // cur: internal node (0 <= cur < PRT_TRAV_DONE), leaf code (< 0) or
+  // PRT_TRAV_DONE; lf: parked leaf (< 0) or none (>= 0)
+  int32_t cur = 0, lf = 0;
+  const float4* tris4 = reinterpret_cast<const float4*>(P.tris);
+  // the nearest pending entry that can still hold a closer hit
+  auto pop_next = [&]() -> int32_t {
+    while (sp > 0) {
+      int32_t node;
+      float t;
+      S.pop(--sp, node, t);
+      if (t <= tb) return node;
+    }
+    return PRT_TRAV_DONE;
+  };
+  while (true) {
+    while ((uint32_t)cur < (uint32_t)PRT_TRAV_DONE) {
+      float tn[4];
+      int32_t cc[4];
+      node_children(P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
+      ts.nodes++;
+      if (tn[3] != __builtin_inff()) S.push(sp++, cc[3], tn[3]);
+      if (tn[2] != __builtin_inff()) S.push(sp++, cc[2], tn[2]);
+      if (tn[1] != __builtin_inff()) S.push(sp++, cc[1], tn[1]);
+      cur = (tn[0] != __builtin_inff()) ? cc[0] : pop_next();
+      if (cur < 0 && lf >= 0) {  // park the first leaf and keep walking
+        lf = cur;
+        cur = pop_next();
+      }
+      if (!__any(lf >= 0)) break;  // every lane still walking holds a leaf
+    }
+    while (lf < 0) {
+      const uint32_t code = ~(uint32_t)lf;
+      const uint32_t first = code & 0x07FFFFFFu;
+      const uint32_t count = (code >> 27) + 1u;
+      for (uint32_t k = 0; k < count; ++k) {
+        const float4* tr = tris4 + (size_t)(first + k) * 3;
+        const float4 a = tr[0], b = tr[1], c = tr[2];
+        ts.tris++;
+        double t, u, v;
+        if (mt_test(a, b, c, o, d, t, u, v) && t < t_best) {
+          if (ANY) {
+            id_best = (int32_t)(first + k);
+            return true;
+          }
+          t_best = t;
+          ub = u;
+          vb = v;
+          id_best = (int32_t)(first + k);
+          tb = fup(t);
+          found = true;
+        }
+      }
+      lf = cur;  // a second leaf met while walking is tested next
+      if (cur < 0) cur = pop_next();
+    }
+    if (cur == PRT_TRAV_DONE) break;
+  }
+  return found;
+}
+
 // Closest (ANY = false) or any (ANY = true) hit over the BVH triangles with
 // t in (1e-6, t_best).  o is the already-nudged origin.
 template <bool ANY, class Stk>
 __device__ __forceinline__ bool traverse(const Params& P, d3 o, d3 d, double& t_best, int32_t& id_best, double& ub,
                                          double& vb, Stk& S, TravStats& ts) {
+#if PRT_TRAVERSE_WW
+  return traverse_ww<ANY>(P, o, d, t_best, id_best, ub, vb, S, ts);
+#else
   if (P.n_nodes <= 0) return false;
   const float ox = (float)o.x, oy = (float)o.y, oz = (float)o.z;
   float ix = (float)(1.0 / d.x), iy = (float)(1.0 / d.y), iz = (float)(1.0 / d.z);
@@ -76,43 +205,13 @@ __device__ __forceinline__ bool traverse(const Params& P, d3 o, d3 d, double& t_
   bool found = false;
   int sp = 0;
   int32_t cur = 0;
-  const float4* nodes4 = reinterpret_cast<const float4*>(P.nodes);
   const float4* tris4 = reinterpret_cast<const float4*>(P.tris);
   while (true) {
     if (cur >= 0) {
-      const float4* nd = nodes4 + (size_t)cur * 8;
-      const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
-      const int4 ch = *reinterpret_cast<const int4*>(nd + 6);
-      ts.nodes++;
       float tn[4];
       int32_t cc[4];
-      const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w},
-                  lza[4] = {lz.x, lz.y, lz.z, lz.w};
-      const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w},
-                  hza[4] = {hz.x, hz.y, hz.z, hz.w};
-      const int32_t cha[4] = {ch.x, ch.y, ch.z, ch.w};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const float x0 = __builtin_fmaf(lxa[k], ix, -oix), x1 = __builtin_fmaf(hxa[k], ix, -oix);
-        const float y0 = __builtin_fmaf(lya[k], iy, -oiy), y1 = __builtin_fmaf(hya[k], iy, -oiy);
-        const float z0 = __builtin_fmaf(lza[k], iz, -oiz), z1 = __builtin_fmaf(hza[k], iz, -oiz);
-        const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-        const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
-        tn[k] = (tnear <= tfar) ? tnear : __builtin_inff();
-        cc[k] = cha[k];
-      }
-      // sort the four (t, child) pairs ascending
-#define PRT_CSWAP(a, b)                  \
-  if (tn[b] < tn[a]) {                   \
-    const float tt = tn[a];              \
-    tn[a] = tn[b];                       \
-    tn[b] = tt;                          \
-    const int32_t c_ = cc[a];            \
-    cc[a] = cc[b];                       \
-    cc[b] = c_;                          \
-  }
-      PRT_CSWAP(0, 1) PRT_CSWAP(2, 3) PRT_CSWAP(0, 2) PRT_CSWAP(1, 3) PRT_CSWAP(1, 2)
-#undef PRT_CSWAP
+      node_children(P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
+      ts.nodes++;
       if (tn[3] != __builtin_inff()) S.push(sp++, cc[3], tn[3]);
       if (tn[2] != __builtin_inff()) S.push(sp++, cc[2], tn[2]);
       if (tn[1] != __builtin_inff()) S.push(sp++, cc[1], tn[1]);
@@ -156,6 +255,7 @@ __device__ __forceinline__ bool traverse(const Params& P, d3 o, d3 d, double& t_
     if (!got) break;
   }
   return found;
+#endif
 }
 
 // Closest hit over analytic entities then the BVH; o_in is the ray origin
@@ -191,10 +291,204 @@ __device__ __forceinline__ bool occluded(const Params& P, d3 o, d3 d, double tma
   return traverse<true>(P, o, d, tmax, id, u, v, S, ts);
 }
 
-__device__ __forceinline__ void flush_stats(uint64_t* ctr, int idx_nodes, TravStats ts) {
+// ------------------------------------------------ queue traversal with refill
+//
+// Closest or any hit for the rays of a chunked queue (chunk c holds desc[c].y
+// rays from queue index desc[c].x on), the while-while loop of traverse_ww
+// with lanes refilled as their rays finish.  The chunks are split into
+// PRT_FETCH_COUNTERS sub-ranges (fewer when the grid has fewer waves), each
+// with its own claim counter on its own
+// 128-byte line; a wave claims PRT_FETCH_RAYS / 64 chunks per atomic from
+// the counter of its sub-range (so no counter is shared by the whole
+// device), keeps the claimed entries not yet started as a pool, and hands
+// them to idle lanes at the top of each round once at least PRT_REFILL_MIN
+// lanes are idle.  Without refill a wave lives as long as its slowest ray
+// and the other lanes idle meanwhile.  Each ray's own traversal is
+// traverse_ww's, so its hit is the same.
+//
+// Load(j, o, d, tmax): ray j's origin (before the 1e-6 nudge for a closest
+// hit, already nudged for an any hit), direction and t bound.
+// Done(j, t, id, blocked): the finished ray (closest hit t / id as
+// closest_hit; any hit: blocked).
+#ifndef PRT_REFILL_MIN
+#define PRT_REFILL_MIN 16
+#endif
+#ifndef PRT_FETCH_RAYS
+#define PRT_FETCH_RAYS 128
+#endif
+template <bool ANY, class Stk, class Load, class Done>
+__device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks, const uint2* __restrict__ desc,
+                                              uint32_t* fetch, Stk& S, TravStats& ts, Load&& load, Done&& done) {
+  const float4* tris4 = reinterpret_cast<const float4*>(P.tris);
+  const uint32_t lane = lane_id();
+  const uint64_t below = (1ull << lane) - 1ull;
+  constexpr uint32_t kClaim = (uint32_t)PRT_FETCH_RAYS / 64u;  // chunks per claim
+  // this wave's sub-range of chunks and its counter
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  // as many sub-ranges as counters, but no more than waves: every sub-range
+  // needs a wave that drains it
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t n_sub = n_waves < (uint32_t)PRT_FETCH_COUNTERS ? n_waves : (uint32_t)PRT_FETCH_COUNTERS;
+  const uint32_t sub = wave % n_sub;
+  const uint32_t sub_b = (uint32_t)((uint64_t)sub * n_chunks / n_sub);
+  const uint32_t sub_e = (uint32_t)((uint64_t)(sub + 1) * n_chunks / n_sub);
+  uint32_t* counter = fetch + (size_t)sub * PRT_FETCH_STRIDE;
+  d3 o = mk(0.0, 0.0, 0.0), d = o;
+  float ix = 0.0f, iy = 0.0f, iz = 0.0f, oix = 0.0f, oiy = 0.0f, oiz = 0.0f, tb = 0.0f;
+  double t_best = 0.0;
+  int32_t id_best = -1, cur = PRT_TRAV_DONE, lf = 0;
+  int sp = 0;
+  uint32_t ri = 0;
+  bool act = false, blocked = false;
+  // wave-uniform: claimed entries pool_base .. pool_base + pool_n - 1 (entry
+  // g = chunk * 64 + lane slot) not yet started; more: the sub-range may hold
+  // unclaimed chunks
+  uint32_t pool_base = 0, pool_n = 0;
+  bool more = sub_b < sub_e;
+  auto pop_next = [&]() -> int32_t {
+    while (sp > 0) {
+      int32_t node;
+      float t;
+      S.pop(--sp, node, t);
+      if (t <= tb) return node;
+    }
+    return PRT_TRAV_DONE;
+  };
+  while (true) {
+    // ---- refill idle lanes
+    const uint64_t idle = __ballot(!act);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (n_idle >= (uint32_t)PRT_REFILL_MIN && (pool_n > 0 || more)) {
+      const uint32_t rank = (uint32_t)__popcll(idle & below);
+      const uint32_t take = n_idle < pool_n ? n_idle : pool_n;
+      int64_t g = -1;
+      if (!act && rank < take) g = pool_base + rank;
+      pool_base += take;
+      pool_n -= take;
+      if (n_idle > take && more) {
+        uint32_t x = 0;
+        if (lane == 0) x = atomicAdd(counter, kClaim);
+        x = (uint32_t)__shfl((int)x, 0, 64);
+        const uint32_t c0 = sub_b + x;
+        if (c0 >= sub_e) {
+          more = false;
+        } else {
+          const uint32_t nc = (sub_e - c0) < kClaim ? (sub_e - c0) : kClaim;
+          const uint32_t ent = nc * 64u;
+          const uint32_t need = n_idle - take, used = need < ent ? need : ent;
+          if (!act && rank >= take && rank - take < ent) g = c0 * 64u + (rank - take);
+          pool_base = c0 * 64u + used;
+          pool_n = ent - used;
+          if (c0 + nc >= sub_e) more = false;
+        }
+      }
+      if (g >= 0) {
+        const uint2 dc = desc[(uint32_t)g >> 6];
+        if (((uint32_t)g & 63u) < dc.y) {
+          // start ray ri: analytic entities first, as closest_hit / occluded
+          ri = dc.x + ((uint32_t)g & 63u);
+          act = true;
+          blocked = false;
+          d3 oo;
+          double tmax;
+          load(ri, oo, d, tmax);
+          o = ANY ? oo : oo + d * 0.000001;
+          t_best = tmax;
+          id_best = -1;
+          for (int k = 0; k < P.n_analytic; ++k) {
+            const int32_t ei = P.analytic[k];
+            double tt;
+            if (analytic_t(P.ents[ei], o, d, tt) && tt < t_best) {
+              if (ANY) {
+                blocked = true;
+                break;
+              }
+              t_best = tt;
+              id_best = -2 - ei;
+            }
+          }
+          lf = 0;
+          sp = 0;
+          cur = PRT_TRAV_DONE;
+          if (!blocked && P.n_nodes > 0) {
+            ix = (float)(1.0 / d.x);
+            iy = (float)(1.0 / d.y);
+            iz = (float)(1.0 / d.z);
+            if (__builtin_isinf(ix)) ix = __builtin_copysignf(1.0e30f, ix);
+            if (__builtin_isinf(iy)) iy = __builtin_copysignf(1.0e30f, iy);
+            if (__builtin_isinf(iz)) iz = __builtin_copysignf(1.0e30f, iz);
+            oix = (float)o.x * ix;
+            oiy = (float)o.y * iy;
+            oiz = (float)o.z * iz;
+            tb = t_best < 1.0e30 ? fup(t_best) : __builtin_inff();
+            cur = 0;
+          }
+        }
+      }
+    }
+    if (!__any(act)) {
+      if (pool_n == 0 && !more) break;  // nothing running and nothing left to start
+      continue;                         // the claimed entries were padding
+    }
+    // ---- walk internal nodes until every walking lane holds a leaf
+    while ((uint32_t)cur < (uint32_t)PRT_TRAV_DONE) {
+      float tn[4];
+      int32_t cc[4];
+      node_children(P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
+      ts.nodes++;
+      if (tn[3] != __builtin_inff()) S.push(sp++, cc[3], tn[3]);
+      if (tn[2] != __builtin_inff()) S.push(sp++, cc[2], tn[2]);
+      if (tn[1] != __builtin_inff()) S.push(sp++, cc[1], tn[1]);
+      cur = (tn[0] != __builtin_inff()) ? cc[0] : pop_next();
+      if (cur < 0 && lf >= 0) {
+        lf = cur;
+        cur = pop_next();
+      }
+      if (!__any(lf >= 0)) break;
+    }
+    // ---- test the parked leaves
+    while (lf < 0) {
+      const uint32_t code = ~(uint32_t)lf;
+      const uint32_t first = code & 0x07FFFFFFu;
+      const uint32_t count = (code >> 27) + 1u;
+      for (uint32_t k = 0; k < count; ++k) {
+        const float4* tr = tris4 + (size_t)(first + k) * 3;
+        const float4 a = tr[0], b = tr[1], c = tr[2];
+        ts.tris++;
+        double t, u, v;
+        if (mt_test(a, b, c, o, d, t, u, v) && t < t_best) {
+          if (ANY) {
+            blocked = true;
+            break;
+          }
+          t_best = t;
+          id_best = (int32_t)(first + k);
+          tb = fup(t);
+        }
+      }
+      if (ANY && blocked) {
+        lf = 0;
+        cur = PRT_TRAV_DONE;
+      } else {
+        lf = cur;
+        if (cur < 0) cur = pop_next();
+      }
+    }
+    // ---- hand over finished rays
+    if (act && cur == PRT_TRAV_DONE && lf >= 0) {
+      done(ri, t_best, id_best, blocked);
+      act = false;
+      ts.rays++;
+    }
+  }
+}
+
+__device__ __forceinline__ void flush_stats(uint64_t* ctr, int idx_nodes, TravStats ts, int idx_rays = -1) {
   const uint64_t n = wave_sum_u32(ts.nodes), t = wave_sum_u32(ts.tris);
+  const uint64_t r = idx_rays >= 0 ? wave_sum_u32(ts.rays) : 0;
   if (lane_id() == 0) {
     if (n) atomicAdd((unsigned long long*)&ctr[idx_nodes], (unsigned long long)n);
     if (t) atomicAdd((unsigned long long*)&ctr[idx_nodes + 1], (unsigned long long)t);
+    if (r) atomicAdd((unsigned long long*)&ctr[idx_rays], (unsigned long long)r);
   }
 }
