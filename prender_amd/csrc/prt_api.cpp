@@ -47,7 +47,7 @@ void Workspace::release() {
   hit_id.release();
   for (auto& b : sq) b.release();
   for (auto& b : desc) b.release();
-  slab_cnt.release();
+  slabs.release();
   accum.release();
   accum2.release();
   counts.release();
@@ -56,7 +56,6 @@ void Workspace::release() {
   if (h_counts) (void)hipHostFree(h_counts);
   h_counts = nullptr;
   batch_samples = queue_cap = shadow_cap = 0;
-  slab_ray = slab_sh = 0;
 }
 
 int alloc_ray_queue(DevBuf* cols, int64_t cap) {
@@ -493,16 +492,16 @@ int prt_trace_closest(prt_scene* s, const prt_ray* rays, int64_t n, prt_hit* hit
 // ---------------------------------------------------------------- render
 static int ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum) {
   Workspace& w = s->ws;
-  // a ray queue never holds more than batch_samples * grow path segments
-  // (glass splits in two up to depth 2); a vertex emits at most fan children
-  const int64_t grow = s->has_glass ? 8 : 1, fan = s->has_glass ? 2 : 1;
+  // A slab writer over input chunks [c0, c1) owns fan * 64 * (c1 - c0)
+  // entries from fan * 64 * c0 on, so a launch's outputs span fan * 64 *
+  // (input chunks) <= fan * (input rays + 64 * PRT_SLAB_WAVES).  With glass a
+  // vertex at depth <= 2 spawns two children (fan 2), deeper ones one, so
+  // fan * input rays <= 8x the samples (2 * 4x at depth 2); without glass 1x.
+  // Shadow rays: three per ray of the NEE input.
+  const int64_t grow = s->has_glass ? 8 : 1;
   const int64_t W = PRT_SLAB_WAVES;
-  // chunks of any queue (entries / 64 + one padded chunk per slab) over the
-  // slab writers: the most input chunks one writer wave is given
-  const int64_t cw = ((batch_samples * grow + 63) / 64 + W + W - 1) / W;
-  const int64_t slab_ray = fan * 64 * cw, slab_sh = 3 * 64 * cw;
-  const int64_t qcap = std::max(W * slab_ray, batch_samples + 64);
-  const int64_t scap = W * slab_sh;
+  const int64_t qcap = grow * batch_samples + 2 * 64 * W + 64;
+  const int64_t scap = 3 * (grow * batch_samples + 64 * W);
   if (qcap >= ((int64_t)1 << 32) || scap >= ((int64_t)1 << 32)) return fail(PRT_E_INVALID, "batch too large");
   if (w.batch_samples >= batch_samples && w.queue_cap >= qcap && w.shadow_cap >= scap && w.h_counts &&
       (!two_accum || w.accum2.p))
@@ -514,9 +513,10 @@ static int ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum)
   if ((rc = dev_alloc(w.hit_t, (size_t)qcap * 8))) return rc;
   if ((rc = dev_alloc(w.hit_id, (size_t)qcap * 4))) return rc;
   if ((rc = alloc_shadow_queue(w.sq, scap))) return rc;
+  // chunks: a slab's last chunk may be partly filled
   for (int k = 0; k < 3; ++k)
-    if ((rc = dev_alloc(w.desc[k], (size_t)((k < 2 ? qcap : scap) / 64 + 1) * sizeof(uint2)))) return rc;
-  if ((rc = dev_alloc(w.slab_cnt, (size_t)3 * W * sizeof(uint32_t)))) return rc;
+    if ((rc = dev_alloc(w.desc[k], (size_t)((k < 2 ? qcap : scap) / 64 + W + 1) * sizeof(uint2)))) return rc;
+  if ((rc = dev_alloc(w.slabs, (size_t)3 * W * sizeof(uint2)))) return rc;
   if ((rc = dev_alloc(w.accum, (size_t)batch_samples * 3 * sizeof(uint64_t)))) return rc;
   if (two_accum && (rc = dev_alloc(w.accum2, (size_t)batch_samples * 3 * sizeof(uint64_t)))) return rc;
   if ((rc = dev_alloc(w.counts, 16 * sizeof(uint32_t)))) return rc;
@@ -527,8 +527,6 @@ static int ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum)
   w.batch_samples = batch_samples;
   w.queue_cap = qcap;
   w.shadow_cap = scap;
-  w.slab_ray = (uint32_t)slab_ray;
-  w.slab_sh = (uint32_t)slab_sh;
   return PRT_OK;
 }
 
@@ -552,7 +550,10 @@ static int setup_render(prt_scene* s, const prt_render_opts* o, RenderSetup& R) 
   P.key0 = (uint32_t)opts.seed;
   P.key1 = (uint32_t)(opts.seed >> 32);
   R.n_pix = (int64_t)P.W * P.H;
-  int64_t bp = opts.batch_pixels > 0 ? opts.batch_pixels : std::max<int64_t>(1, ((int64_t)1 << 21) / P.spp);
+  // 2^21 camera samples per batch; 2^20 with glass, whose queues are sized
+  // for up to 8 path segments per sample
+  const int64_t batch_samples = (int64_t)1 << (s->has_glass ? 20 : 21);
+  int64_t bp = opts.batch_pixels > 0 ? opts.batch_pixels : std::max<int64_t>(1, batch_samples / P.spp);
   bp = std::min(bp, R.n_pix);
   // queues hold up to 16x (glass splits into slabs) and 24x (shadow rays)
   // the batch, indexed by uint32
@@ -579,7 +580,7 @@ static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* ac
   hipStream_t st = s->stream;
   uint32_t* counts = (uint32_t*)w.counts.p;
   uint32_t* fetch = (uint32_t*)w.fetch.p;
-  uint32_t* slab_cnt = (uint32_t*)w.slab_cnt.p;
+  uint2* slabs = (uint2*)w.slabs.p;
   uint64_t* ctr = (uint64_t*)w.ctr.p;
   RayQ q[2] = {ray_queue(w.q[0], w.queue_cap), ray_queue(w.q[1], w.queue_cap)};
   ShadowQ sq = shadow_queue(w.sq, w.shadow_cap);
@@ -587,11 +588,12 @@ static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* ac
   const bool nee = P.nee && P.n_lights > 0;
   const uint32_t W = PRT_SLAB_WAVES;
   const size_t fetch_launch = (size_t)PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE;
+  // children per shaded vertex: glass splits in two up to depth 2
+  auto fan = [&](int depth) { return (uint32_t)(s->has_glass && depth <= 2 ? 2 : 1); };
   auto chunks = [&](int k) { return ChunkList{(const uint2*)w.desc[k].p, counts + k}; };
-  auto index = [&](int k, uint32_t n_slabs, uint32_t slab_cap, const uint32_t* cnt) {
+  auto index = [&](int k, uint32_t n_slabs, const uint2* sl) {
     const uint32_t max_chunks = (uint32_t)(w.desc[k].bytes / sizeof(uint2));
-    return launch_slab_index(cnt, n_slabs, slab_cap, (uint2*)w.desc[k].p, max_chunks, counts + k, counts + 4 + k, ctr,
-                             st);
+    return launch_slab_index(sl, n_slabs, (uint2*)w.desc[k].p, max_chunks, counts + k, counts + 4 + k, ctr, st);
   };
   int fi = 0;  // traversal launches so far; each takes fresh claim counters
   auto next_fetch = [&](uint32_t** f) -> int {
@@ -607,9 +609,10 @@ static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* ac
   HIPCHK(launch_fill_u64(accum, 0ull, 3 * ns, st));
   HIPCHK(launch_fill_u32(counts, 0u, 16, st));
   // the primary rays: one slab of np rays at the start of q[0]
-  HIPCHK(launch_fill_u32(slab_cnt, (uint32_t)np, 1, st));
+  HIPCHK(launch_fill_u32((uint32_t*)slabs, 0u, 1, st));
+  HIPCHK(launch_fill_u32((uint32_t*)slabs + 1, (uint32_t)np, 1, st));
   HIPCHK(launch_gen_primary(P, B, q[0], st));
-  HIPCHK(index(0, 1, (uint32_t)w.queue_cap, slab_cnt));
+  HIPCHK(index(0, 1, slabs));
   if ((rc = tm.mark(T_OTHER))) return rc;
   if ((rc = next_fetch(&f))) return rc;
   HIPCHK(launch_extend(P, q[0], chunks(0), np, f, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb, st));
@@ -617,20 +620,20 @@ static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* ac
   tot.primary_rays += (uint64_t)np;
   tot.launches++;
   // depth 0: every camera sample against its sub-pixel's primary hit
-  HIPCHK(launch_shade(P, B, 1, q[0], chunks(0), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1],
-                      w.slab_ray, slab_cnt + W, accum, A, ctr, st));
-  HIPCHK(index(1, W, w.slab_ray, slab_cnt + W));
+  HIPCHK(launch_shade(P, B, 1, q[0], chunks(0), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1], fan(0),
+                      slabs + W, accum, A, ctr, st));
+  HIPCHK(index(1, W, slabs + W));
   if ((rc = tm.mark(T_SHADE))) return rc;
   if (nee) {
-    HIPCHK(launch_nee(P, B, q[1], chunks(1), sq, w.slab_sh, slab_cnt + 2 * W, ctr, st));
-    HIPCHK(index(2, W, w.slab_sh, slab_cnt + 2 * W));
+    HIPCHK(launch_nee(P, B, q[1], chunks(1), sq, slabs + 2 * W, ctr, st));
+    HIPCHK(index(2, W, slabs + 2 * W));
     if ((rc = tm.mark(T_NEE))) return rc;
     if ((rc = next_fetch(&f))) return rc;
     HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st));
     if ((rc = tm.mark(T_SHADOW))) return rc;
   }
   int cur = 1;
-  while (true) {
+  for (int depth = 1;; ++depth) {  // depth of the rays in q[cur]
     HIPCHK(hipMemcpyAsync(w.h_counts, counts, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     const uint32_t n = w.h_counts[4 + cur];
@@ -643,12 +646,12 @@ static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* ac
     tot.ext_rays += n;
     tot.launches++;
     HIPCHK(launch_shade(P, B, 0, q[cur], chunks(cur), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[nxt],
-                        w.slab_ray, slab_cnt + nxt * W, accum, A, ctr, st));
-    HIPCHK(index(nxt, W, w.slab_ray, slab_cnt + nxt * W));
+                        fan(depth), slabs + nxt * W, accum, A, ctr, st));
+    HIPCHK(index(nxt, W, slabs + nxt * W));
     if ((rc = tm.mark(T_SHADE))) return rc;
     if (nee) {
-      HIPCHK(launch_nee(P, B, q[nxt], chunks(nxt), sq, w.slab_sh, slab_cnt + 2 * W, ctr, st));
-      HIPCHK(index(2, W, w.slab_sh, slab_cnt + 2 * W));
+      HIPCHK(launch_nee(P, B, q[nxt], chunks(nxt), sq, slabs + 2 * W, ctr, st));
+      HIPCHK(index(2, W, slabs + 2 * W));
       if ((rc = tm.mark(T_NEE))) return rc;
       if ((rc = next_fetch(&f))) return rc;
       HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st));

@@ -23,20 +23,29 @@ struct StreamWorkspace {
   int64_t n_slots = 0, tile_shift = 0, q_cap = 0, sq_cap = 0;
   DevBuf q[2][RAYQ_COLS], hit_t, hit_id, sq[SHADOWQ_COLS];
   DevBuf accum, ctr, ctl, slot_info, live, free_list, new_tiles;
-  DevBuf scenes, tiles, outs, ovf, scene_fan, seg, new_samp, work, work_pref, nee_pref;
+  DevBuf scenes, tiles, outs, ovf, runs, nruns, desc[2], sh_desc, fetch;
   int64_t ovf_threads = 0;
   StreamCtl* h_ctl = nullptr;  // pinned, two copies (double-buffered status reads)
   void release() {
     for (auto& a : q)
       for (auto& b : a) b.release();
     for (auto& b : sq) b.release();
-    DevBuf* all[] = {&hit_t, &hit_id, &accum, &ctr, &ctl, &slot_info, &live, &free_list, &new_tiles, &scenes,
-                     &tiles, &outs, &ovf, &scene_fan, &seg, &new_samp, &work, &work_pref, &nee_pref};
+    DevBuf* all[] = {&hit_t,  &hit_id, &accum, &ctr,     &ctl,     &slot_info, &live,    &free_list, &new_tiles,
+                     &scenes, &tiles,  &outs,  &ovf,     &runs,    &nruns,     &desc[0], &desc[1],   &sh_desc,
+                     &fetch};
     for (DevBuf* b : all) b->release();
     if (h_ctl) (void)hipHostFree(h_ctl);
     h_ctl = nullptr;
     n_slots = tile_shift = q_cap = sq_cap = ovf_threads = 0;
   }
+};
+
+// Slab geometry of a batch (prt_stream.h): PRT_SLAB_WAVES slab writers,
+// slab_ray / slab_sh entries per shade / NEE slab, max_runs run records per
+// writer, the primary region after the slabs.
+struct StreamGeometry {
+  int64_t slab_ray = 0, slab_sh = 0, max_runs = 0, prim_base = 0, prim_cap = 0, q_cap = 0, sq_cap = 0;
+  int64_t max_desc = 0, max_sh_desc = 0;
 };
 
 void release_stream_workspace(StreamWorkspace* w) {
@@ -104,34 +113,57 @@ int resolve_config(prt_scene* const* scenes, int n, const prt_batch_opts* o, Bat
   return PRT_OK;
 }
 
-// Queue capacities.  A sample has at most 8 rays in flight with glass (splits
-// at depths 0..2), else 1; a queue's scene regions are bounded by 2 (glass) or
-// 1 child per shaded vertex, i.e. by 16x or 1x the samples in flight, plus
-// 64-entry padding per region; the primary region by the samples in flight
-// plus padding per tile.  Shadow rays: 3 per child of an NEE scene.
-int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, bool glass, bool nee_glass, int n_scenes,
+// Slab sizing.  A camera sample has at most 8 path segments in flight with
+// glass (splits at depths 0..2), else 1, so a queue holds at most rays =
+// 8x / 1x the samples in flight; shade's input is those plus one vertex per
+// new camera sample (at most the samples in flight), in 64-entry chunks plus
+// at most one partly filled chunk per run (a run per scene change in a
+// writer's range: a few per writer with the scene-major work queue).  A
+// writer gets at most cw chunks and writes at most fan (2 with glass, else
+// 1) children or 3 shadow rays per entry.  Anything beyond is dropped and
+// reported, never written out of bounds.
+StreamGeometry stream_geometry(const BatchConfig& c, bool glass) {
+  StreamGeometry g;
+  const int64_t W = PRT_SLAB_WAVES;
+  const int64_t T = (int64_t)1 << c.tile_shift;
+  const int64_t inflight = (int64_t)c.n_slots * T;
+  const int64_t rays = (glass ? 8 : 1) * inflight;
+  const int64_t cw = ((rays + inflight) / 64 + W - 1) / W + 4;
+  g.slab_ray = (glass ? 2 : 1) * 64 * cw;
+  g.slab_sh = 3 * 64 * cw;
+  g.max_runs = cw;
+  g.prim_base = W * g.slab_ray;
+  g.prim_cap = inflight;  // a tile's camera rays (n_pixels * SS^2 <= T) padded to 64 (T is a multiple of 64)
+  g.q_cap = g.prim_base + g.prim_cap;
+  g.sq_cap = W * g.slab_sh;
+  // a slab's runs each end in at most one partly filled chunk
+  g.max_desc = W * (g.slab_ray / 64 + g.max_runs) + g.prim_cap / 64;
+  g.max_sh_desc = W * (g.slab_sh / 64 + g.max_runs);
+  return g;
+}
+
+int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, const StreamGeometry& g, int n_scenes,
                             int64_t n_tiles) {
   if (!lead->sws) lead->sws = new (std::nothrow) StreamWorkspace();
   if (!lead->sws) return fail(PRT_E_NOMEM, "out of host memory");
   StreamWorkspace& w = *lead->sws;
   const int64_t T = (int64_t)1 << c.tile_shift;
   const int64_t inflight = (int64_t)c.n_slots * T;
-  const int64_t pad = 64 * ((int64_t)n_scenes + c.n_slots);
-  const int64_t qcap = (glass ? 17 : 2) * inflight + pad;
-  const int64_t scap = 3 * (nee_glass ? 16 : 1) * inflight + 64 * (int64_t)n_scenes;
-  if (qcap >= ((int64_t)1 << 32) || scap >= ((int64_t)1 << 32)) return fail(PRT_E_INVALID, "queue capacity above 2^32");
+  if (g.q_cap >= ((int64_t)1 << 32) || g.sq_cap >= ((int64_t)1 << 32))
+    return fail(PRT_E_INVALID, "queue capacity above 2^32");
   int rc;
-  if (w.q_cap < qcap) {
+  if (w.q_cap < g.q_cap) {
     for (int qi = 0; qi < 2; ++qi)
-      if ((rc = alloc_ray_queue(w.q[qi], qcap))) return rc;
-    if ((rc = dev_alloc(w.hit_t, (size_t)qcap * 8))) return rc;
-    if ((rc = dev_alloc(w.hit_id, (size_t)qcap * 4))) return rc;
-    w.q_cap = qcap;
+      if ((rc = alloc_ray_queue(w.q[qi], g.q_cap))) return rc;
+    if ((rc = dev_alloc(w.hit_t, (size_t)g.q_cap * 8))) return rc;
+    if ((rc = dev_alloc(w.hit_id, (size_t)g.q_cap * 4))) return rc;
+    w.q_cap = g.q_cap;
   }
-  if (w.sq_cap < scap) {
-    if ((rc = alloc_shadow_queue(w.sq, scap))) return rc;
-    w.sq_cap = scap;
+  if (w.sq_cap < g.sq_cap) {
+    if ((rc = alloc_shadow_queue(w.sq, g.sq_cap))) return rc;
+    w.sq_cap = g.sq_cap;
   }
+  const size_t W = PRT_SLAB_WAVES;
   if ((rc = dev_alloc(w.accum, (size_t)inflight * 3 * sizeof(uint64_t)))) return rc;
   if ((rc = dev_alloc(w.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
   if ((rc = dev_alloc(w.ctl, sizeof(StreamCtl)))) return rc;
@@ -142,12 +174,12 @@ int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, bool glass, b
   if ((rc = dev_alloc(w.scenes, (size_t)n_scenes * sizeof(Params)))) return rc;
   if ((rc = dev_alloc(w.tiles, (size_t)n_tiles * sizeof(TileDesc)))) return rc;
   if ((rc = dev_alloc(w.outs, (size_t)n_scenes * 2 * sizeof(float*)))) return rc;
-  if ((rc = dev_alloc(w.scene_fan, (size_t)n_scenes * sizeof(int32_t)))) return rc;
-  if ((rc = dev_alloc(w.seg, (size_t)n_scenes * 6 * sizeof(uint32_t)))) return rc;  // off, cap, cnt x 2 queues
-  if ((rc = dev_alloc(w.new_samp, (size_t)n_scenes * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(w.work, ((size_t)n_scenes + c.n_slots) * sizeof(WorkEntry)))) return rc;
-  if ((rc = dev_alloc(w.work_pref, ((size_t)n_scenes + c.n_slots + 1) * sizeof(uint32_t)))) return rc;
-  if ((rc = dev_alloc(w.nee_pref, ((size_t)n_scenes + 1) * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(w.runs, 2 * W * (size_t)g.max_runs * sizeof(RunRec)))) return rc;  // shade's, NEE's
+  if ((rc = dev_alloc(w.nruns, 2 * W * sizeof(uint32_t)))) return rc;
+  for (int p = 0; p < 2; ++p)
+    if ((rc = dev_alloc(w.desc[p], (size_t)g.max_desc * sizeof(ChunkDesc)))) return rc;
+  if ((rc = dev_alloc(w.sh_desc, (size_t)g.max_sh_desc * sizeof(ChunkDesc)))) return rc;
+  if ((rc = dev_alloc(w.fetch, (size_t)2 * PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE * sizeof(uint32_t)))) return rc;
   const int64_t ovf_threads = (int64_t)std::max(1, lead->cu_count) * kMaxTravBlocksPerCu * 256;
   if ((rc = dev_alloc(w.ovf, (size_t)ovf_threads * 2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * sizeof(uint32_t))))
     return rc;
@@ -171,8 +203,7 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   const int64_t T = (int64_t)1 << c.tile_shift;
   // work queue: scene-major runs of T / spp pixels; every tile_stride-th tile
   std::vector<TileDesc> tiles;
-  std::vector<int32_t> fan(n);
-  bool glass = false, nee = false, nee_glass = false;
+  bool glass = false, nee = false;
   uint64_t samples = 0;
   int64_t tile_counter = 0;
   for (int i = 0; i < n; ++i) {
@@ -180,8 +211,6 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
     const bool scene_nee = P.nee && P.n_lights > 0;
     glass |= scenes[i]->has_glass;
     nee |= scene_nee;
-    nee_glass |= scene_nee && scenes[i]->has_glass;
-    fan[i] = scenes[i]->has_glass ? 2 : 1;
     const int64_t npix = (int64_t)P.W * P.H;
     const int64_t per = std::max<int64_t>(1, T / P.spp);
     for (int64_t p = 0; p < npix; p += per, ++tile_counter) {
@@ -196,7 +225,8 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   }
   if (tiles.size() >= ((size_t)1 << 31)) return fail(PRT_E_INVALID, "too many tiles");
   int rc;
-  if ((rc = ensure_stream_workspace(lead, c, glass, nee_glass, n, (int64_t)tiles.size()))) return rc;
+  const StreamGeometry geo = stream_geometry(c, glass);
+  if ((rc = ensure_stream_workspace(lead, c, geo, n, (int64_t)tiles.size()))) return rc;
   StreamWorkspace& w = *lead->sws;
   hipStream_t s = lead->stream;
 
@@ -219,7 +249,6 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   HIPCHK(hipMemcpyAsync(w.scenes.p, params.data(), params.size() * sizeof(Params), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(w.tiles.p, tiles.data(), tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(w.outs.p, outs.data(), outs.size() * sizeof(float*), hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemcpyAsync(w.scene_fan.p, fan.data(), fan.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
 
   Stream S{};
   S.scenes = (const Params*)w.scenes.p;
@@ -230,14 +259,14 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   S.live = (int32_t*)w.live.p;
   S.free_list = (uint32_t*)w.free_list.p;
   S.new_tiles = (NewTile*)w.new_tiles.p;
-  S.scene_fan = (const int32_t*)w.scene_fan.p;
-  S.seg_off = (uint32_t*)w.seg.p;
-  S.seg_cap = S.seg_off + 2 * (size_t)n;
-  S.seg_cnt = S.seg_off + 4 * (size_t)n;
-  S.new_samp = (uint32_t*)w.new_samp.p;
-  S.work = (WorkEntry*)w.work.p;
-  S.work_pref = (uint32_t*)w.work_pref.p;
-  S.nee_pref = (uint32_t*)w.nee_pref.p;
+  S.ray_runs = (RunRec*)w.runs.p;
+  S.sh_runs = S.ray_runs + (size_t)PRT_SLAB_WAVES * geo.max_runs;
+  S.ray_nruns = (uint32_t*)w.nruns.p;
+  S.sh_nruns = S.ray_nruns + PRT_SLAB_WAVES;
+  S.desc[0] = (ChunkDesc*)w.desc[0].p;
+  S.desc[1] = (ChunkDesc*)w.desc[1].p;
+  S.sh_desc = (ChunkDesc*)w.sh_desc.p;
+  S.fetch = (uint32_t*)w.fetch.p;
   S.ctl = (StreamCtl*)w.ctl.p;
   S.accum = (uint64_t*)w.accum.p;
   S.ctr = (uint64_t*)w.ctr.p;
@@ -245,8 +274,13 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   S.n_tiles = (int32_t)tiles.size();
   S.n_slots = c.n_slots;
   S.tile_shift = c.tile_shift;
-  S.q_cap = (uint32_t)w.q_cap;
-  S.sq_cap = (uint32_t)w.sq_cap;
+  S.slab_ray = (uint32_t)geo.slab_ray;
+  S.slab_sh = (uint32_t)geo.slab_sh;
+  S.max_runs = (uint32_t)geo.max_runs;
+  S.prim_base = (uint32_t)geo.prim_base;
+  S.prim_cap = (uint32_t)geo.prim_cap;
+  S.max_desc = (uint32_t)geo.max_desc;
+  S.max_sh_desc = (uint32_t)geo.max_sh_desc;
   RayQ q[2] = {ray_queue(w.q[0], w.q_cap), ray_queue(w.q[1], w.q_cap)};
   ShadowQ sq = shadow_queue(w.sq, w.sq_cap);
   double* hit_t = (double*)w.hit_t.p;
@@ -279,13 +313,15 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
     if ((e = launch_stream_refill(S, cur, s)) != hipSuccess) return e;
     if ((e = launch_stream_gen(S, q[cur], cur, work_blocks, s)) != hipSuccess) return e;
     if ((e = mark(T_OTHER)) != hipSuccess) return e;
-    if ((e = launch_stream_extend(S, q[cur], hit_t, hit_id, c.stack_lds, trav_blocks, s)) != hipSuccess) return e;
+    if ((e = launch_stream_extend(S, q[cur], cur, hit_t, hit_id, c.stack_lds, trav_blocks, s)) != hipSuccess)
+      return e;
     if ((e = mark(T_EXTEND)) != hipSuccess) return e;
-    if ((e = launch_stream_shade(S, q[cur], hit_t, hit_id, q[nxt], nxt, work_blocks, s)) != hipSuccess) return e;
+    if ((e = launch_stream_shade(S, q[cur], hit_t, hit_id, q[nxt], nxt, s)) != hipSuccess) return e;
+    if ((e = launch_stream_index(S, nxt, s)) != hipSuccess) return e;
     if ((e = mark(T_SHADE)) != hipSuccess) return e;
     if (nee) {
-      if ((e = launch_stream_scan(S, nxt, s)) != hipSuccess) return e;
-      if ((e = launch_stream_nee(S, q[nxt], nxt, sq, work_blocks, s)) != hipSuccess) return e;
+      if ((e = launch_stream_nee(S, q[nxt], nxt, sq, s)) != hipSuccess) return e;
+      if ((e = launch_stream_index(S, 2, s)) != hipSuccess) return e;
       if ((e = mark(T_NEE)) != hipSuccess) return e;
       if ((e = launch_stream_shadow(S, sq, c.stack_lds, trav_blocks, s)) != hipSuccess) return e;
       if ((e = mark(T_SHADOW)) != hipSuccess) return e;

@@ -58,18 +58,17 @@ RayQ ray_queue(DevBuf* b, int64_t cap);
 ShadowQ shadow_queue(DevBuf* b, int64_t cap);
 
 // Per-scene render workspace sized for a batch of samples.  The ray and
-// shadow queues are slab queues (prt_kernels.h): PRT_SLAB_WAVES slabs of
-// slab_ray / slab_sh entries, plus chunk descriptors for each.
+// shadow queues are slab queues (prt_kernels.h), each with its chunk
+// descriptors.
 struct Workspace {
   int64_t batch_samples = 0;  // camera samples per batch
   int64_t queue_cap = 0;      // entries allocated per ray queue
   int64_t shadow_cap = 0;     // entries allocated in the shadow queue
-  uint32_t slab_ray = 0, slab_sh = 0;
   DevBuf q[2][RAYQ_COLS];     // two ray queues (SoA)
   DevBuf hit_t, hit_id;
   DevBuf sq[SHADOWQ_COLS];
   DevBuf desc[3];             // chunk descriptors (uint2) of q[0], q[1], the shadow queue
-  DevBuf slab_cnt;            // uint32 [3][PRT_SLAB_WAVES] slab counts of the same three
+  DevBuf slabs;               // uint2 [3][PRT_SLAB_WAVES] slab (start, count) of the same three
   DevBuf accum, accum2;       // fixed-point sample radiance (accum2: adjoint primal)
   DevBuf counts;              // uint32 [16]: chunks of q0, q1, sq at 0..2, entries at 4..6
   DevBuf fetch;               // uint32 claim counters, PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE per launch

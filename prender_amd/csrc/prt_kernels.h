@@ -83,10 +83,11 @@ struct Batch {
 };
 
 // Slab queues of the per-scene renderer.  k_shade and k_nee run
-// PRT_SLAB_WAVES waves; wave w writes its outputs to entries
-// [w * slab_cap, w * slab_cap + cnt[w]) of its output queue, counting them in
-// LDS, and stores cnt[w] once at its end, so no counter is shared between
-// waves (same-address atomics serialise across the device).  k_slab_index
+// PRT_SLAB_WAVES waves, each over a static range [c0, c1) of input chunks;
+// wave w writes its outputs to its slab, entries from fan * 64 * c0 on (fan:
+// most outputs per input entry), counting them in LDS, and stores the slab
+// (start, count) once at its end, so no counter is shared between waves
+// (same-address atomics serialise across the device).  k_slab_index
 // turns the counts into descriptors of 64-entry chunks, none spanning two
 // slabs, which the consuming kernels walk: k_shade and k_nee in static
 // per-wave ranges, k_extend and k_shadow through PRT_FETCH_COUNTERS claim
@@ -103,16 +104,15 @@ struct ChunkList {
 hipError_t launch_trace(const Params& P, const prt_ray_dev* rays, int64_t n, prt_hit_dev* hits, uint64_t* ctr,
                         int max_blocks, hipStream_t s);
 hipError_t launch_gen_primary(const Params& P, const Batch& B, const RayQ& q, hipStream_t s);
-hipError_t launch_slab_index(const uint32_t* slab_cnt, uint32_t n_slabs, uint32_t slab_cap, uint2* desc,
-                             uint32_t max_chunks, uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr,
-                             hipStream_t s);
+hipError_t launch_slab_index(const uint2* slabs, uint32_t n_slabs, uint2* desc, uint32_t max_chunks, uint32_t* n_chunks,
+                             uint32_t* n_entries, uint64_t* ctr, hipStream_t s);
 hipError_t launch_extend(const Params& P, const RayQ& q, const ChunkList& cl, int64_t n_host, uint32_t* fetch,
                          double* hit_t, int32_t* hit_id, uint64_t* ctr, int max_blocks, hipStream_t s);
 hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const ChunkList& cl,
-                        const double* hit_t, const int32_t* hit_id, const RayQ& qout, uint32_t slab_cap,
-                        uint32_t* slab_cnt, uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s);
+                        const double* hit_t, const int32_t* hit_id, const RayQ& qout, uint32_t fan, uint2* slabs,
+                        uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s);
 hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const ChunkList& cl, const ShadowQ& sq,
-                      uint32_t slab_cap, uint32_t* slab_cnt, uint64_t* ctr, hipStream_t s);
+                      uint2* slabs, uint64_t* ctr, hipStream_t s);
 hipError_t launch_shadow(const Params& P, const Batch& B, const ShadowQ& q, const ChunkList& cl, uint32_t* fetch,
                          uint64_t* accum, const Adjoint& A, uint64_t* ctr, int max_blocks, hipStream_t s);
 hipError_t launch_resolve(const Params& P, const Batch& B, const uint64_t* accum, void* rgb, void* var, int f64,

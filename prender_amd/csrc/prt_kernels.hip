@@ -28,11 +28,6 @@
 #ifndef PRT_WAVES_NEE
 #define PRT_WAVES_NEE 0
 #endif
-// 1: k_extend / k_shadow refill lanes from a per-wave pool of claimed rays
-// (traverse_pool); 0: one ray per lane until the wave's 64 rays are done
-#ifndef PRT_TRAVERSE_POOL
-#define PRT_TRAVERSE_POOL 1
-#endif
 #define PRT_WAVES_ATTR(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1)))
 
 // ------------------------------------------------------------ trace API kernel
@@ -156,8 +151,8 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_shad
 template <bool ADJ>
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
     Params P, Batch B, int primary, RayQ qin, ChunkList cl, const double* __restrict__ hit_t,
-    const int32_t* __restrict__ hit_id, RayQ qout, uint32_t slab_cap, uint32_t* slab_cnt,
-    uint64_t* __restrict__ accum, Adjoint A, uint64_t* ctr) {
+    const int32_t* __restrict__ hit_id, RayQ qout, uint32_t fan, uint2* slabs, uint64_t* __restrict__ accum,
+    Adjoint A, uint64_t* ctr) {
   __shared__ uint32_t s_cnt[4];  // children appended by each wave of the block
   const uint32_t wib = threadIdx.x >> 6;
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
@@ -165,15 +160,16 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
   const uint32_t n_samples = (uint32_t)(B.n_pixels * P.spp);
   const uint32_t n_chunks = primary ? (n_samples + 63u) >> 6 : *cl.n;
   const int ss2 = P.SS * P.SS;
+  uint32_t c, c_end;
+  wave_range(n_chunks, c, c_end);
+  // this wave's slab: fan children per entry of its chunks
   ShadeCtx C;
   C.qout = qout;
   C.count_out = &s_cnt[wib];
-  C.off = wave * slab_cap;
-  C.cap = slab_cap;
+  C.off = fan * 64u * c;
+  C.cap = fan * 64u * (c_end - c);
   C.truncated = C.n_ext = C.pickups = C.dropped = 0;
   double ge[3] = {0.0, 0.0, 0.0};
-  uint32_t c, c_end;
-  wave_range(n_chunks, c, c_end);
   for (; c < c_end; ++c) {
     uint32_t i;
     if (primary) {
@@ -253,7 +249,7 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
   const uint64_t s_pick = wave_sum_u32(C.pickups), s_tr = wave_sum_u32(C.truncated);
   const uint64_t s_ext = wave_sum_u32(C.n_ext), s_drop = wave_sum_u32(C.dropped);
   if (lane_id() == 0) {
-    slab_cnt[wave] = min(s_cnt[wib], slab_cap);
+    slabs[wave] = make_uint2(C.off, min(s_cnt[wib], C.cap));
     if (s_pick) atomicAdd((unsigned long long*)&ctr[CTR_PICKUP], (unsigned long long)s_pick);
     if (s_tr) atomicAdd((unsigned long long*)&ctr[CTR_TRUNC], (unsigned long long)s_tr);
     if (s_ext) atomicAdd((unsigned long long*)&ctr[CTR_EXT], (unsigned long long)s_ext);
@@ -264,18 +260,19 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
 // ------------------------------------------------------------ NEE
 // NEE for the diffuse vertices that produced the rays of queue q (their
 // origin, throughput T*colour/rr and orienting normal), with the parent
-// vertex's random numbers.  Each wave appends its shadow rays with one prefix
-// sum and one atomic.
+// vertex's random numbers.  Shadow rays go to the wave's slab of the shadow
+// queue.
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
-    Params P, Batch B, RayQ q, ChunkList cl, ShadowQ sq, uint32_t slab_cap, uint32_t* slab_cnt, uint64_t* ctr) {
+    Params P, Batch B, RayQ q, ChunkList cl, ShadowQ sq, uint2* slabs, uint64_t* ctr) {
   __shared__ uint32_t s_cnt[4];  // shadow rays appended by each wave of the block
   const uint32_t wib = threadIdx.x >> 6;
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (lane_id() == 0) s_cnt[wib] = 0;
-  const uint32_t off = wave * slab_cap;
   uint32_t emitted = 0, dropped = 0;
   uint32_t c, c_end;
   wave_range(*cl.n, c, c_end);
+  // this wave's slab: three shadow rays per entry of its chunks
+  const uint32_t off = 3u * 64u * c, slab_cap = 3u * 64u * (c_end - c);
   for (; c < c_end; ++c) {
     const uint2 dc = cl.desc[c];
     bool have = false;
@@ -294,26 +291,27 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
         ct = mk(q.tr[i], q.tg[i], q.tb[i]);
       }
     }
-    ShadowRay s0, s1, s2;
-    int32_t l0 = -1, l1 = -1, l2 = -1;
-    const bool v0 = have && nee_one(P, w, 0, x, nrm, ct, s0, l0);
-    const bool v1 = have && nee_one(P, w, 1, x, nrm, ct, s1, l1);
-    const bool v2 = have && nee_one(P, w, 2, x, nrm, ct, s2, l2);
-    const uint32_t cnt = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
-    uint32_t j = wave_append_n(&s_cnt[wib], cnt);
-    if (j + cnt > slab_cap) {
-      dropped += cnt;
-      continue;
+    // one light sample at a time, appended through the LDS count: only one
+    // shadow ray's registers are live
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      ShadowRay sr;
+      int32_t l = -1;
+      const bool v = have && nee_one(P, w, m, x, nrm, ct, sr, l);
+      const uint32_t j = wave_append_bit(&s_cnt[wib], v);
+      if (v) {
+        if (j < slab_cap) {
+          put_shadow(sq, off + j, sr, slot, l);
+          ++emitted;
+        } else {
+          ++dropped;
+        }
+      }
     }
-    j += off;
-    if (v0) put_shadow(sq, j++, s0, slot, l0);
-    if (v1) put_shadow(sq, j++, s1, slot, l1);
-    if (v2) put_shadow(sq, j++, s2, slot, l2);
-    emitted += cnt;
   }
   const uint64_t s_sh = wave_sum_u32(emitted), s_drop = wave_sum_u32(dropped);
   if (lane_id() == 0) {
-    slab_cnt[wave] = min(s_cnt[wib], slab_cap);
+    slabs[wave] = make_uint2(off, min(s_cnt[wib], slab_cap));
     if (s_sh) atomicAdd((unsigned long long*)&ctr[CTR_SHADOW], (unsigned long long)s_sh);
     if (s_drop) atomicAdd((unsigned long long*)&ctr[CTR_DROP], (unsigned long long)s_drop);
   }
@@ -321,12 +319,12 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
 
 // ------------------------------------------------------------ slab index
 // Chunk descriptors of a slab queue (one block of 1024 threads): slab s holds
-// min(cnt[s], slab_cap) entries from s * slab_cap on and ceil(that / 64)
-// chunks; chunks are numbered slab by slab.  Also stores the number of chunks
-// and of entries.  More chunks than max_chunks (never, with the host's
-// sizing) are dropped and counted.
-__global__ __launch_bounds__(1024) void k_slab_index(const uint32_t* __restrict__ cnt, uint32_t n_slabs,
-                                                     uint32_t slab_cap, uint2* __restrict__ desc, uint32_t max_chunks,
+// slabs[s].y entries from queue index slabs[s].x on, ceil(that / 64) chunks;
+// chunks are numbered slab by slab.  Also stores the number of chunks and of
+// entries.  More chunks than max_chunks (never, with the host's sizing) are
+// dropped and counted.
+__global__ __launch_bounds__(1024) void k_slab_index(const uint2* __restrict__ slabs, uint32_t n_slabs,
+                                                     uint2* __restrict__ desc, uint32_t max_chunks,
                                                      uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr) {
   __shared__ uint32_t s_pref[PRT_SLAB_INDEX_MAX + 1];
   __shared__ uint32_t s_wch[16], s_went[16];
@@ -335,7 +333,7 @@ __global__ __launch_bounds__(1024) void k_slab_index(const uint32_t* __restrict_
   const uint32_t b = min(tid * per, n_slabs), e = min(b + per, n_slabs);
   uint32_t ch = 0, ent = 0;
   for (uint32_t s = b; s < e; ++s) {
-    const uint32_t c = min(cnt[s], slab_cap);
+    const uint32_t c = slabs[s].y;
     ch += (c + 63u) >> 6;
     ent += c;
   }
@@ -366,7 +364,7 @@ __global__ __launch_bounds__(1024) void k_slab_index(const uint32_t* __restrict_
   uint32_t run = wb_ch + ich - ch;
   for (uint32_t s = b; s < e; ++s) {
     s_pref[s] = run;
-    run += (min(cnt[s], slab_cap) + 63u) >> 6;
+    run += (slabs[s].y + 63u) >> 6;
   }
   if (tid == 0) s_pref[n_slabs] = tot_ch;
   __syncthreads();
@@ -379,8 +377,8 @@ __global__ __launch_bounds__(1024) void k_slab_index(const uint32_t* __restrict_
       else hi = mid;
     }
     const uint32_t k = c - s_pref[lo];
-    const uint32_t cs = min(cnt[lo], slab_cap);
-    desc[c] = make_uint2(lo * slab_cap + (k << 6), min(64u, cs - (k << 6)));
+    const uint2 sl = slabs[lo];
+    desc[c] = make_uint2(sl.x + (k << 6), min(64u, sl.y - (k << 6)));
   }
   if (tid == 0) {
     *n_chunks = total;
@@ -474,28 +472,27 @@ hipError_t launch_shadow(const Params& P, const Batch& B, const ShadowQ& q, cons
   return hipGetLastError();
 }
 hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const ChunkList& cl,
-                        const double* hit_t, const int32_t* hit_id, const RayQ& qout, uint32_t slab_cap,
-                        uint32_t* slab_cnt, uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s) {
+                        const double* hit_t, const int32_t* hit_id, const RayQ& qout, uint32_t fan, uint2* slabs,
+                        uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s) {
   const dim3 grid(PRT_SLAB_WAVES / 4);
   if (A.on)
-    hipLaunchKernelGGL(k_shade<true>, grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, slab_cap,
-                       slab_cnt, accum, A, ctr);
+    hipLaunchKernelGGL(k_shade<true>, grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, fan, slabs,
+                       accum, A, ctr);
   else
-    hipLaunchKernelGGL(k_shade<false>, grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, slab_cap,
-                       slab_cnt, accum, A, ctr);
+    hipLaunchKernelGGL(k_shade<false>, grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, fan, slabs,
+                       accum, A, ctr);
   return hipGetLastError();
 }
 hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const ChunkList& cl, const ShadowQ& sq,
-                      uint32_t slab_cap, uint32_t* slab_cnt, uint64_t* ctr, hipStream_t s) {
-  hipLaunchKernelGGL(k_nee, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slab_cap, slab_cnt, ctr);
+                      uint2* slabs, uint64_t* ctr, hipStream_t s) {
+  hipLaunchKernelGGL(k_nee, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slabs, ctr);
   return hipGetLastError();
 }
-hipError_t launch_slab_index(const uint32_t* slab_cnt, uint32_t n_slabs, uint32_t slab_cap, uint2* desc,
-                             uint32_t max_chunks, uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr,
-                             hipStream_t s) {
+hipError_t launch_slab_index(const uint2* slabs, uint32_t n_slabs, uint2* desc, uint32_t max_chunks, uint32_t* n_chunks,
+                             uint32_t* n_entries, uint64_t* ctr, hipStream_t s) {
   if (n_slabs == 0 || n_slabs > PRT_SLAB_INDEX_MAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_slab_index, dim3(1), dim3(1024), 0, s, slab_cnt, n_slabs, slab_cap, desc, max_chunks, n_chunks,
-                     n_entries, ctr);
+  hipLaunchKernelGGL(k_slab_index, dim3(1), dim3(1024), 0, s, slabs, n_slabs, desc, max_chunks, n_chunks, n_entries,
+                     ctr);
   return hipGetLastError();
 }
 hipError_t launch_resolve(const Params& P, const Batch& B, const uint64_t* accum, void* rgb, void* var, int f64,

@@ -6,24 +6,27 @@
 // the tiles in flight: a slot owns T fixed-point sample accumulators and a
 // count of the tile's live rays (queued path segments and shadow rays).
 //
-// Ray queues are binned by scene: each iteration, refill lays the next queue
-// out as one region per scene, sized by an exact bound on the children that
-// scene's vertices can spawn (two per vertex with glass, else one), and the
-// kernels walk a padded prefix table so that every 64-wide chunk of work —
-// one wave — holds rays of one scene only and reads that scene's parameters
-// with scalar loads.
+// Queues are slab queues, as in the per-scene renderer: the shade and NEE
+// kernels run PRT_SLAB_WAVES waves, and wave w writes its outputs into its
+// own slab of the output queue, counting them in LDS (no counter is shared
+// between waves).  A wave walks its input in runs of one scene and records
+// each run (scene, slab range), so the index kernel can list the queue as
+// 64-entry chunks of one scene each: every wave of a consuming kernel holds
+// rays of one scene and reads that scene's parameters with scalar loads.
 //
 // Every iteration runs, on one stream:
 //   refill  (1 block)  assign free slots to the next tiles of the work queue;
-//                      lay out the next queue's scene regions; build the
-//                      current queue's work table
-//   gen                unique sub-pixel camera rays of the new tiles, after the
-//                      scene regions of the current queue
-//   extend             closest hit of the current queue
+//                      append the chunks of their camera rays to the current
+//                      queue's chunk list; clear the claim counters
+//   gen                unique sub-pixel camera rays of the new tiles, in the
+//                      primary region after the slabs
+//   extend             closest hit of every chunk of the current queue
 //   shade              one vertex per queued ray and per new camera sample;
-//                      children go to their scene's region of the next queue
-//   scan    (1 block)  prefix table of the next queue's regions (NEE input)
-//   nee / shadow       light samples of the diffuse vertices, any-hit tests
+//                      children go to the wave's slab of the next queue
+//   index   (1 block)  chunk list of the next queue (NEE input, and the next
+//                      iteration's extend / shade input)
+//   nee / index / shadow  light samples of the diffuse vertices into shadow
+//                      slabs, their chunk list, any-hit tests
 //   resolve            tiles with no live ray left: fold samples into their
 //                      scene's image, clear the slot, free it
 // so the GPU always holds rays from as many tiles as there are slots, of
@@ -57,8 +60,7 @@ struct SlotInfo {
 };
 
 // A tile assigned by this iteration's refill: its slot, and where its unique
-// camera rays sit in the primary region of the current queue (ray_base is a
-// multiple of 64).
+// camera rays sit in the primary region (ray_base is a multiple of 64).
 struct NewTile {
   uint32_t slot;
   uint32_t ray_base;
@@ -70,10 +72,18 @@ struct NewTile {
   uint32_t pad_;
 };
 
-// One entry of a work table: `cnt` rays of one scene at queue positions
-// [off, off + cnt), occupying roundup64(cnt) entries of the padded index space.
-struct WorkEntry {
+// A run of one scene's entries that a slab writer appended: queue entries
+// [off, off + cnt).
+struct RunRec {
   uint32_t off;
+  uint32_t cnt;
+  int32_t scene;
+  uint32_t pad_;
+};
+
+// A chunk of a queue: entries [base, base + cnt), cnt <= 64, of one scene.
+struct ChunkDesc {
+  uint32_t base;
   uint32_t cnt;
   int32_t scene;
   uint32_t pad_;
@@ -81,22 +91,20 @@ struct WorkEntry {
 
 // Control block (device memory; the host reads it after each graph replay).
 struct StreamCtl {
-  uint32_t next_tile;      // first tile of the work queue not yet assigned
-  uint32_t free_count;     // entries of free_list
-  uint32_t n_new;          // tiles assigned by this iteration's refill
-  uint32_t n_cont;         // rays carried over in the current queue's scene regions
-  uint32_t n_prim;         // unused (gen adds the new tiles' camera rays to total_prim)
-  uint32_t status;         // 0 running, 1 finished, 2 inconsistent state, 3 queue capacity
-  uint32_t iter;           // iterations run
-  uint32_t cnt_sq;         // shadow queue count
-  uint32_t fetch[4];       // work-fetch counters of extend, shadow, shade and NEE
-  uint32_t region_end[2];  // end of the scene regions of queue p (its primary region starts there)
-  uint32_t work_width;     // padded index space of the current queue's work table
-  uint32_t cont_width;     // ... of its scene regions alone (the first n_scenes entries)
-  uint32_t nee_width;      // padded index space of the next queue's scene regions
-  uint32_t n_work;         // entries of the work table (n_scenes + n_new)
-  uint64_t total_prim;     // camera rays traversed
-  uint64_t total_cont;     // bounce rays traversed
+  uint32_t next_tile;        // first tile of the work queue not yet assigned
+  uint32_t free_count;       // entries of free_list
+  uint32_t n_new;            // tiles assigned by this iteration's refill
+  uint32_t n_cont;           // rays carried over in the current queue
+  uint32_t status;           // 0 running, 1 finished, 2 inconsistent state, 3 queue capacity
+  uint32_t iter;             // iterations run
+  uint32_t cont_chunks[2];   // chunks of the slabs of queue p (index kernel)
+  uint32_t cont_entries[2];  // rays in them
+  uint32_t ext_chunks;       // chunks of the current queue: the slabs', then the new tiles' camera rays
+  uint32_t sh_chunks;        // chunks of the shadow queue
+  uint32_t sh_entries;
+  uint32_t pad_;
+  uint64_t total_prim;       // camera rays traversed
+  uint64_t total_cont;       // bounce rays traversed
 };
 
 struct Stream {
@@ -104,33 +112,38 @@ struct Stream {
   const TileDesc* tiles;    // [n_tiles] the work queue, in order
   float* const* out_rgb;    // [n_scenes] W*H*3 device images
   float* const* out_var;    // [n_scenes] (entries may be null)
-  const int32_t* scene_fan; // [n_scenes] most children one vertex can spawn (2 with glass, else 1)
   SlotInfo* slot_info;      // [n_slots]
   int32_t* live;            // [n_slots] live rays of the slot's tile
   uint32_t* free_list;      // [n_slots]
   NewTile* new_tiles;       // [n_slots]
-  uint32_t* seg_off;        // [2][n_scenes] region start of scene s in queue p
-  uint32_t* seg_cap;        // [2][n_scenes] region capacity
-  uint32_t* seg_cnt;        // [2][n_scenes] rays appended to the region
-  uint32_t* new_samp;       // [n_scenes] camera samples of this iteration's new tiles
-  WorkEntry* work;          // [n_scenes + n_slots] work table of the current queue
-  uint32_t* work_pref;      // [n_scenes + n_slots + 1] its padded prefix
-  uint32_t* nee_pref;       // [n_scenes + 1] padded prefix of the next queue's scene regions
+  RunRec* ray_runs;         // [PRT_SLAB_WAVES * max_runs] runs of the shade waves
+  uint32_t* ray_nruns;      // [PRT_SLAB_WAVES]
+  RunRec* sh_runs;          // [PRT_SLAB_WAVES * max_runs] runs of the NEE waves
+  uint32_t* sh_nruns;       // [PRT_SLAB_WAVES]
+  ChunkDesc* desc[2];       // [max_desc] chunk list of queue p
+  ChunkDesc* sh_desc;       // [max_sh_desc] chunk list of the shadow queue
+  uint32_t* fetch;          // [2][PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE] claim counters of extend, shadow
   StreamCtl* ctl;
   uint64_t* accum;          // [n_slots << tile_shift][3] fixed point
   uint64_t* ctr;            // [CTR_COUNT]
   int32_t n_scenes, n_tiles, n_slots, tile_shift;
-  uint32_t q_cap, sq_cap;
+  uint32_t slab_ray;        // entries per shade slab
+  uint32_t slab_sh;         // entries per NEE slab
+  uint32_t max_runs;        // run records per slab writer
+  uint32_t prim_base;       // start of the primary region (= PRT_SLAB_WAVES * slab_ray)
+  uint32_t prim_cap;        // its entries
+  uint32_t max_desc, max_sh_desc;
+  uint32_t pad_;
 };
 
 hipError_t launch_stream_init(const Stream& st, hipStream_t s);
 hipError_t launch_stream_refill(const Stream& st, int cur, hipStream_t s);
 hipError_t launch_stream_gen(const Stream& st, const RayQ& q, int cur, int blocks, hipStream_t s);
-hipError_t launch_stream_extend(const Stream& st, const RayQ& q, double* hit_t, int32_t* hit_id, int stack_lds,
-                                int blocks, hipStream_t s);
+hipError_t launch_stream_extend(const Stream& st, const RayQ& q, int cur, double* hit_t, int32_t* hit_id,
+                                int stack_lds, int blocks, hipStream_t s);
 hipError_t launch_stream_shade(const Stream& st, const RayQ& qin, const double* hit_t, const int32_t* hit_id,
-                               const RayQ& qout, int nxt, int blocks, hipStream_t s);
-hipError_t launch_stream_scan(const Stream& st, int nxt, hipStream_t s);
-hipError_t launch_stream_nee(const Stream& st, const RayQ& q, int nxt, const ShadowQ& sq, int blocks, hipStream_t s);
+                               const RayQ& qout, int nxt, hipStream_t s);
+hipError_t launch_stream_index(const Stream& st, int which, hipStream_t s);  // which: 0 / 1 ray queue, 2 shadow
+hipError_t launch_stream_nee(const Stream& st, const RayQ& q, int nxt, const ShadowQ& sq, hipStream_t s);
 hipError_t launch_stream_shadow(const Stream& st, const ShadowQ& sq, int stack_lds, int blocks, hipStream_t s);
 hipError_t launch_stream_resolve(const Stream& st, int blocks, hipStream_t s);

@@ -1,11 +1,11 @@
 // prt_stream.hip — gfx950 kernels of the multi-scene streaming renderer
 // (the iteration structure is described in prt_stream.h).
 //
-// Extend, shade and NEE walk padded work tables whose entries are 64-aligned
-// runs of one scene's rays, so a wave's scene is uniform and its parameters
-// are read with scalar loads.  The shadow queue is shared by all scenes (each
-// wave appends its shadow rays contiguously); its kernel runs each distinct
-// scene of a wave in turn.  The per-vertex arithmetic is the per-scene
+// Every kernel that reads a queue walks its chunk list, whose chunks each
+// hold rays of one scene, so a wave's scene is uniform and its parameters are
+// read with scalar loads.  Shade and NEE take static chunk ranges and write
+// into their own slabs (no shared counter); extend and shadow claim chunks
+// through sub-range counters.  The per-vertex arithmetic is the per-scene
 // renderer's (prt_shade.h), and a camera sample's random numbers depend only
 // on (pixel, sample, depth, branch), so a tile renders bit-identically to the
 // same pixels of a per-scene render.
@@ -21,6 +21,9 @@
 
 __device__ __forceinline__ uint32_t round64(uint32_t n) { return (n + 63u) & ~63u; }
 
+// chunks per claim of the traversal kernels
+#define FETCH_CHUNKS 2u
+
 // Parameters of scene s (s wave-uniform) through a constant-address-space
 // pointer, so that their loads are scalar loads as kernel arguments' are.
 __device__ __forceinline__ const Params& scene_params(const Params* scenes, int s) {
@@ -31,47 +34,12 @@ __device__ __forceinline__ const Params& scene_params(const Params* scenes, int 
   return *(const Params*)Pc;
 }
 
-// Entry e of a padded prefix table with pref[e] <= x < pref[e + 1]
-// (pref[0] = 0, pref[n] > x); x wave-uniform.
-__device__ __forceinline__ uint32_t find_entry(const uint32_t* __restrict__ pref, uint32_t n, uint32_t x) {
-  uint32_t lo = 0, hi = n;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (pref[mid] <= x) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// Calls f(scenes[s]) for the lanes whose scene is s, one distinct s at a time
-// (s wave-uniform); lanes with sid < 0 take no part.  Returns the number of
-// passes (distinct scenes).  Every lane of the wave must reach this call.
-template <class F>
-__device__ __forceinline__ uint32_t per_scene(const Params* __restrict__ scenes, int sid, F&& f) {
-  bool pend = sid >= 0;
-  uint64_t m = __ballot(pend);
-  uint32_t passes = 0;
-  while (m) {
-    ++passes;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    const int s = __builtin_amdgcn_readfirstlane(__shfl(sid, leader, 64));
-    // the parameters' address is formed from the uniform s before the lane
-    // test, so the compiler cannot substitute the per-lane sid (equal inside
-    // the branch) and the loads stay scalar
-    const Params& P = scene_params(scenes, s);
-    if (pend && sid == s) {
-      f(P);
-      pend = false;
-    }
-    m = __ballot(pend);
-  }
-  return passes;
-}
-
-// Extra waterfall passes of a wave (waves holding rays of several scenes),
-// summed per wave and added to the counter at the end of a kernel.
-__device__ __forceinline__ void flush_mixed(uint64_t* ctr, uint32_t extra) {
-  if (lane_id() == 0 && extra) atomicAdd((unsigned long long*)&ctr[CTR_MIXED], (unsigned long long)extra);
+// Chunk c of a chunk list (c wave-uniform), read with scalar loads.
+__device__ __forceinline__ ChunkDesc chunk_at(const ChunkDesc* list, uint32_t c) {
+  const uint64_t a = (uint64_t)(list + c);
+  const uint64_t au = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
+                      (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+  return *(const ChunkDesc*)(const __attribute__((address_space(4))) ChunkDesc*)au;
 }
 
 // Adds delta to the live-ray count of tile slot ts (one atomic per distinct
@@ -126,6 +94,37 @@ __device__ __forceinline__ ChunkScan block_scan(uint32_t n, F val, uint32_t* s_p
   return r;
 }
 
+// Run bookkeeping of a slab writer: the wave's entries in its slab are
+// counted in LDS; a run of one scene starts where that count stood when the
+// scene changed.  Runs beyond max_runs (never, with the host's sizing) are
+// dropped and counted.  All state is wave-uniform.
+struct RunLog {
+  RunRec* recs;
+  uint32_t max_runs, n, dropped;
+  uint32_t slab_off;  // queue index of the slab's first entry
+  uint32_t start;     // slab count at the start of the open run
+  int scene;          // scene of the open run (-1: none)
+  __device__ __forceinline__ void close(uint32_t count) {
+    if (scene < 0) return;
+    const uint32_t c = count - start;
+    if (c > 0) {
+      if (n < max_runs) {
+        if (lane_id() == 0) recs[n] = RunRec{slab_off + start, c, scene, 0u};
+        ++n;
+      } else {
+        dropped += c;
+      }
+    }
+  }
+  // called with the scene of the next chunk, before that chunk's appends
+  __device__ __forceinline__ void enter(int s, uint32_t count) {
+    if (s == scene) return;
+    close(count);
+    scene = s;
+    start = count;
+  }
+};
+
 __global__ void k_stream_init(Stream st) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int stride = gridDim.x * blockDim.x;
@@ -134,47 +133,39 @@ __global__ void k_stream_init(Stream st) {
     st.live[i] = 0;
     st.slot_info[i].tile = -1;  // free
   }
-  for (int i = gid; i < 2 * st.n_scenes; i += stride) {
-    st.seg_off[i] = 0;
-    st.seg_cap[i] = 0;
-    st.seg_cnt[i] = 0;
-  }
   if (gid == 0) {
     StreamCtl* c = st.ctl;
     c->next_tile = 0;
     c->free_count = (uint32_t)st.n_slots;
-    c->n_new = c->n_cont = c->n_prim = 0;
+    c->n_new = c->n_cont = 0;
     c->status = 0;
     c->iter = 0;
-    c->cnt_sq = 0;
-    c->fetch[0] = c->fetch[1] = c->fetch[2] = c->fetch[3] = 0;
-    c->region_end[0] = c->region_end[1] = 0;
-    c->work_width = c->cont_width = c->nee_width = c->n_work = 0;
+    c->cont_chunks[0] = c->cont_chunks[1] = 0;
+    c->cont_entries[0] = c->cont_entries[1] = 0;
+    c->ext_chunks = c->sh_chunks = c->sh_entries = 0;
     c->total_prim = c->total_cont = 0;
   }
 }
 
-// One block: hand the free slots to the next tiles of the work queue, lay out
-// the next queue's scene regions and build the current queue's work table;
-// reset the per-iteration counters.
+// One block: hand the free slots to the next tiles of the work queue, append
+// the chunks of their camera rays to queue cur's chunk list, clear the claim
+// counters and update the status.
 __global__ __launch_bounds__(256) void k_stream_refill(Stream st, int cur) {
   __shared__ uint32_t s_part[257];
   __shared__ uint32_t s_hdr[4];
   StreamCtl* c = st.ctl;
-  const int nxt = cur ^ 1;
   const uint32_t tid = threadIdx.x;
-  const uint32_t NS = (uint32_t)st.n_scenes;
   if (tid == 0) {
     const uint32_t free_n = min(c->free_count, (uint32_t)st.n_slots), next = c->next_tile;
     const uint32_t left = (uint32_t)st.n_tiles - next;
     s_hdr[0] = free_n < left ? free_n : left;
     s_hdr[1] = free_n;
     s_hdr[2] = next;
-    s_hdr[3] = c->region_end[cur];  // the primary region of the current queue starts here
+    s_hdr[3] = c->cont_chunks[cur];
   }
-  for (uint32_t s = tid; s < NS; s += 256) st.new_samp[s] = 0;
+  for (uint32_t k = tid; k < 2u * PRT_FETCH_COUNTERS; k += 256) st.fetch[(size_t)k * PRT_FETCH_STRIDE] = 0u;
   __syncthreads();
-  const uint32_t n_want = s_hdr[0], free_n = s_hdr[1], next = s_hdr[2], prim_base = s_hdr[3];
+  const uint32_t n_want = s_hdr[0], free_n = s_hdr[1], next = s_hdr[2], cont_chunks = s_hdr[3];
 
   // 1. camera rays of the candidate tiles (64-aligned per tile)
   auto tile_rays = [&](uint32_t k) {
@@ -183,10 +174,11 @@ __global__ __launch_bounds__(256) void k_stream_refill(Stream st, int cur) {
     return round64((uint32_t)td.n_pixels * (uint32_t)(P.SS * P.SS));
   };
   const ChunkScan tr = block_scan(n_want, tile_rays, s_part);
-  const bool fits_prim = (uint64_t)prim_base + tr.total <= st.q_cap;
-  const uint32_t n_new = fits_prim ? n_want : 0u;
-  if (fits_prim) {
+  const bool fits = tr.total <= st.prim_cap && cont_chunks + (tr.total >> 6) <= st.max_desc;
+  const uint32_t n_new = fits ? n_want : 0u;
+  if (fits) {
     uint32_t base = tr.base;
+    ChunkDesc* desc = st.desc[cur];
     for (uint32_t k = tr.b; k < tr.e; ++k) {
       const uint32_t tile = next + k;
       const TileDesc td = st.tiles[tile];
@@ -210,85 +202,29 @@ __global__ __launch_bounds__(256) void k_stream_refill(Stream st, int cur) {
       si.pad_ = 0;
       st.slot_info[slot] = si;
       st.live[slot] = (int32_t)nt.n_samples;
-      atomicAdd(&st.new_samp[td.scene], nt.n_samples);
+      // 2. the tile's camera-ray chunks, after the slabs' chunks
+      for (uint32_t j = 0; j < nt.n_rays; j += 64) {
+        ChunkDesc d;
+        d.base = st.prim_base + base + j;
+        d.cnt = min(64u, nt.n_rays - j);
+        d.scene = td.scene;
+        d.pad_ = 0;
+        desc[cont_chunks + ((base + j) >> 6)] = d;
+      }
       base += round64(nt.n_rays);
     }
   }
-  __syncthreads();
-
-  // 2. scene regions of the next queue: bound = fan * (carried-over rays +
-  //    new camera samples), the vertices this iteration shades
-  const uint32_t* cnt_cur = st.seg_cnt + cur * NS;
-  const uint32_t* cap_cur = st.seg_cap + cur * NS;
-  auto bound = [&](uint32_t s) {
-    const uint32_t cnt = min(cnt_cur[s], cap_cur[s]);
-    return round64((uint32_t)st.scene_fan[s] * (cnt + st.new_samp[s]));
-  };
-  const ChunkScan rb = block_scan(NS, bound, s_part);
-  // regions past the queue's end are never handed out: with no room, every
-  // region gets capacity 0 (shade then drops and counts the children, and the
-  // status below reports the overflow)
-  const bool fits_next = rb.total <= st.q_cap;
-  {
-    uint32_t base = rb.base;
-    for (uint32_t s = rb.b; s < rb.e; ++s) {
-      const uint32_t w = bound(s);
-      st.seg_off[nxt * NS + s] = fits_next ? base : 0u;
-      st.seg_cap[nxt * NS + s] = fits_next ? w : 0u;
-      st.seg_cnt[nxt * NS + s] = 0;
-      base += w;
-    }
-  }
-
-  // 3. work table of the current queue: the scene regions, then the new tiles
-  const uint32_t n_work = NS + n_new;
-  auto width = [&](uint32_t e) {
-    if (e < NS) return round64(min(cnt_cur[e], cap_cur[e]));
-    return round64(st.new_tiles[e - NS].n_rays);
-  };
-  const ChunkScan wt = block_scan(n_work, width, s_part);
-  {
-    uint32_t base = wt.base;
-    for (uint32_t e = wt.b; e < wt.e; ++e) {
-      WorkEntry we;
-      if (e < NS) {
-        we.off = st.seg_off[cur * NS + e];
-        we.cnt = min(cnt_cur[e], cap_cur[e]);
-        we.scene = (int32_t)e;
-      } else {
-        const NewTile nt = st.new_tiles[e - NS];
-        we.off = prim_base + nt.ray_base;
-        we.cnt = nt.n_rays;
-        we.scene = nt.scene;
-      }
-      we.pad_ = 0;
-      st.work[e] = we;
-      st.work_pref[e] = base;
-      base += round64(we.cnt);
-    }
-    if (tid == 255) st.work_pref[n_work] = wt.total;
-  }
-  // carried-over rays (sum of the scene regions' counts)
-  auto cont = [&](uint32_t s) { return min(cnt_cur[s], cap_cur[s]); };
-  const ChunkScan cs = block_scan(NS, cont, s_part);
   if (tid == 0) {
-    const uint32_t n_cont = cs.total;
-    const uint32_t cont_width = wt.total - (fits_prim ? tr.total : 0u);  // = work_pref[n_scenes]
+    const uint32_t n_cont = c->cont_entries[cur];
     c->n_new = n_new;
     c->n_cont = n_cont;
-    c->n_prim = 0;  // counted by gen into total_prim
+    c->ext_chunks = cont_chunks + (fits ? tr.total >> 6 : 0u);
     c->free_count = free_n - n_new;
     c->next_tile = next + n_new;
-    c->region_end[nxt] = fits_next ? rb.total : 0u;
-    c->work_width = wt.total;
-    c->cont_width = cont_width;
-    c->n_work = n_work;
-    c->cnt_sq = 0;
-    c->fetch[0] = c->fetch[1] = c->fetch[2] = c->fetch[3] = 0;
     c->total_cont += n_cont;
     c->iter += 1;
     if (c->status < 2) {
-      if (!fits_prim || !fits_next) c->status = 3;
+      if (!fits) c->status = 3;
       else if (n_cont == 0 && n_new == 0)
         c->status = (next == (uint32_t)st.n_tiles && free_n == (uint32_t)st.n_slots) ? 1u : 2u;
       else c->status = 0;
@@ -297,9 +233,9 @@ __global__ __launch_bounds__(256) void k_stream_refill(Stream st, int cur) {
 }
 
 // Unique sub-pixel camera rays of the new tiles (one wave per tile), in the
-// primary region of queue `cur`.
+// primary region.
 __global__ __launch_bounds__(256) void k_stream_gen(Stream st, RayQ q, int cur) {
-  const uint32_t n_new = st.ctl->n_new, prim_base = st.ctl->region_end[cur];
+  const uint32_t n_new = st.ctl->n_new;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
   uint32_t rays = 0;
@@ -313,7 +249,7 @@ __global__ __launch_bounds__(256) void k_stream_gen(Stream st, RayQ q, int cur) 
       const int sub = (int)(r % ss2);
       d3 o, d;
       camera_ray(P, (int)(pix % P.W), (int)(pix / P.W), sub % P.SS, sub / P.SS, o, d);
-      const uint32_t i = prim_base + nt.ray_base + r;
+      const uint32_t i = st.prim_base + nt.ray_base + r;
       q.ox[i] = o.x; q.oy[i] = o.y; q.oz[i] = o.z;
       q.dx[i] = d.x; q.dy[i] = d.y; q.dz[i] = d.z;
       q.slot[i] = nt.slot << st.tile_shift;
@@ -324,271 +260,302 @@ __global__ __launch_bounds__(256) void k_stream_gen(Stream st, RayQ q, int cur) 
   if (lane_id() == 0 && rays) atomicAdd((unsigned long long*)&st.ctl->total_prim, (unsigned long long)rays);
 }
 
-// Waves take FETCH_CHUNKS 64-wide chunks of a padded index space per atomic
-// and walk them in runs of one table entry (one scene), so the scene's
-// parameters are loop-invariant within a run and their loads are hoisted.
-#define FETCH_CHUNKS 4u
-
-__device__ __forceinline__ uint32_t fetch_chunks(uint32_t* counter) {
-  uint32_t c = 0;
-  if (lane_id() == 0) c = atomicAdd(counter, FETCH_CHUNKS);
-  return (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)c, 0, 64));
-}
-
-// Closest hit for the current queue through its work table.
+// Closest hit for every chunk of queue cur (slab chunks, then camera rays).
 template <int NL>
-__global__ __launch_bounds__(256) void k_stream_extend(Stream st, RayQ q, double* __restrict__ hit_t,
+__global__ __launch_bounds__(256) void k_stream_extend(Stream st, RayQ q, int cur, double* __restrict__ hit_t,
                                                        int32_t* __restrict__ hit_id) {
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL);
-  const uint32_t n_chunks = st.ctl->work_width >> 6, n_work = st.ctl->n_work;
-  uint32_t* fetch = &st.ctl->fetch[0];
+  const ChunkDesc* desc = st.desc[cur];
+  const ClaimRange cr = claim_range(st.fetch, st.ctl->ext_chunks);
   TravStats ts{0, 0, 0};
-  while (true) {
-    uint32_t c = fetch_chunks(fetch);
-    if (c >= n_chunks) break;
-    const uint32_t c_end = min(c + FETCH_CHUNKS, n_chunks);
-    while (c < c_end) {
-      const uint32_t e = find_entry(st.work_pref, n_work, c << 6);
-      const WorkEntry we = st.work[e];
-      const uint32_t e_base = st.work_pref[e];
-      const uint32_t run_end = min(st.work_pref[e + 1] >> 6, c_end);
-      const Params& P = scene_params(st.scenes, we.scene);
-      for (; c < run_end; ++c) {
-        const uint32_t local = (c << 6) - e_base + lane_id();
-        if (local < we.cnt) {
-          const uint32_t i = we.off + local;
-          double t, u, v;
-          int32_t id;
-          closest_hit(P, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), t, id, u, v, S, ts);
-          hit_t[i] = t;
-          hit_id[i] = id;
-        }
+  uint32_t c, c1;
+  while (claim(cr, FETCH_CHUNKS, c, c1)) {
+    for (; c < c1; ++c) {
+      const ChunkDesc dc = chunk_at(desc, c);
+      const Params& P = scene_params(st.scenes, dc.scene);
+      if (lane_id() < dc.cnt) {
+        const uint32_t i = dc.base + lane_id();
+        double t, u, v;
+        int32_t id;
+        closest_hit(P, mk(q.ox[i], q.oy[i], q.oz[i]), mk(q.dx[i], q.dy[i], q.dz[i]), t, id, u, v, S, ts);
+        hit_t[i] = t;
+        hit_id[i] = id;
       }
     }
   }
   flush_stats(st.ctr, CTR_NODES, ts);
 }
 
-// One vertex per carried-over ray (scene regions of queue cur, through the
-// work table) and per camera sample of the new tiles (T padded entries per
-// tile); children go to their scene's region of queue nxt.
+// One vertex per ray of queue cur's slab chunks and per camera sample of the
+// new tiles (T / 64 chunks per tile); children go to this wave's slab of
+// queue nxt, logged as runs of one scene.
 __global__ __launch_bounds__(256) void k_stream_shade(Stream st, RayQ qin, const double* __restrict__ hit_t,
                                                       const int32_t* __restrict__ hit_id, RayQ qout, int nxt) {
-  const uint32_t cont_chunks = st.ctl->cont_width >> 6, n_new = st.ctl->n_new;
-  const uint32_t prim_base = st.ctl->region_end[nxt ^ 1];
-  const uint32_t NS = (uint32_t)st.n_scenes;
+  __shared__ uint32_t s_cnt[4];  // entries in each wave's slab
+  const int cur = nxt ^ 1;
+  const uint32_t wib = threadIdx.x >> 6;
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (lane_id() == 0) s_cnt[wib] = 0;
+  const uint32_t cont_chunks = st.ctl->cont_chunks[cur], n_new = st.ctl->n_new;
   const uint32_t shift = (uint32_t)st.tile_shift, tmask = (1u << shift) - 1u;
   const uint32_t n_chunks = cont_chunks + (n_new << (shift - 6));  // T >= 64: T/64 chunks per new tile
+  const ChunkDesc* desc = st.desc[cur];
   ShadeCtx C;
   C.qout = qout;
+  C.count_out = &s_cnt[wib];
+  C.off = wave * st.slab_ray;
+  C.cap = st.slab_ray;
   C.truncated = C.n_ext = C.pickups = C.dropped = 0;
-  {
-    uint32_t c, c_end;
-    wave_range(n_chunks, c, c_end);
-    while (c < c_end) {
-      // a run: one scene region of the carried-over rays, or one new tile
-      const bool prim = c >= cont_chunks;
-      int scene;
-      uint32_t run_end, e_base = 0;
-      WorkEntry we{0, 0, 0, 0};
-      NewTile nt{};
-      if (!prim) {
-        const uint32_t e = find_entry(st.work_pref, NS, c << 6);
-        we = st.work[e];
-        e_base = st.work_pref[e];
-        run_end = min(st.work_pref[e + 1] >> 6, c_end);
-        scene = we.scene;
-      } else {
-        const uint32_t k = (c - cont_chunks) >> (shift - 6);
-        nt = st.new_tiles[k];
-        run_end = min(cont_chunks + ((k + 1) << (shift - 6)), c_end);
-        scene = nt.scene;
+  RunLog log{st.ray_runs + (size_t)wave * st.max_runs, st.max_runs, 0u, 0u, C.off, 0u, -1};
+  uint32_t c, c_end;
+  wave_range(n_chunks, c, c_end);
+  for (; c < c_end; ++c) {
+    const bool prim = c >= cont_chunks;
+    ChunkDesc dc{0, 0, 0, 0};
+    NewTile nt{};
+    int scene;
+    if (!prim) {
+      dc = chunk_at(desc, c);
+      scene = dc.scene;
+    } else {
+      nt = st.new_tiles[(c - cont_chunks) >> (shift - 6)];
+      scene = nt.scene;
+    }
+    log.enter(scene, min(s_cnt[wib], st.slab_ray));
+    const Params& P = scene_params(st.scenes, scene);
+    bool valid;
+    uint32_t slot = 0, ray = 0, k = 0, branch = 1;
+    int depth = 0;
+    int64_t pixel = 0;
+    d3 T = mk(1.0, 1.0, 1.0), Pw = mk(0.0, 0.0, 0.0);
+    if (!prim) {
+      valid = lane_id() < dc.cnt;
+      if (valid) {
+        ray = dc.base + lane_id();
+        slot = qin.slot[ray];
+        T = mk(qin.tr[ray], qin.tg[ray], qin.tb[ray]);
+        Pw = mk(qin.pr[ray], qin.pg[ray], qin.pb[ray]);
+        const uint32_t meta = qin.meta[ray];
+        depth = (int)(meta & 0xFFu);
+        branch = META_BRANCH(meta);
+        const SlotInfo si = st.slot_info[slot >> shift];
+        const uint32_t r = slot & tmask;
+        k = r % (uint32_t)si.spp;
+        pixel = si.pixel_base + r / (uint32_t)si.spp;
       }
-      const Params& P = scene_params(st.scenes, scene);
-      C.count_out = &st.seg_cnt[nxt * NS + scene];
-      C.off = st.seg_off[nxt * NS + scene];
-      C.cap = st.seg_cap[nxt * NS + scene];
-      for (; c < run_end; ++c) {
-        bool valid;
-        uint32_t slot = 0, ray = 0, k = 0, branch = 1;
-        int depth = 0;
-        int64_t pixel = 0;
-        d3 T = mk(1.0, 1.0, 1.0), Pw = mk(0.0, 0.0, 0.0);
-        if (!prim) {
-          const uint32_t local = (c << 6) - e_base + lane_id();
-          valid = local < we.cnt;
-          if (valid) {
-            ray = we.off + local;
-            slot = qin.slot[ray];
-            T = mk(qin.tr[ray], qin.tg[ray], qin.tb[ray]);
-            Pw = mk(qin.pr[ray], qin.pg[ray], qin.pb[ray]);
-            const uint32_t meta = qin.meta[ray];
-            depth = (int)(meta & 0xFFu);
-            branch = META_BRANCH(meta);
-            const SlotInfo si = st.slot_info[slot >> shift];
-            const uint32_t r = slot & tmask;
-            k = r % (uint32_t)si.spp;
-            pixel = si.pixel_base + r / (uint32_t)si.spp;
-          }
-        } else {
-          const uint32_t r = (((c - cont_chunks) << 6) & tmask) + lane_id();
-          valid = r < nt.n_samples;
-          if (valid) {
-            slot = (nt.slot << shift) | r;
-            const uint32_t pl = r / nt.spp;
-            k = r % nt.spp;
-            ray = prim_base + nt.ray_base + pl * nt.ss2 + k % nt.ss2;
-            pixel = st.slot_info[nt.slot].pixel_base + pl;
-          }
-        }
-        int delta = 0;
-        if (valid) {
-          C.slot = slot;
-          const uint32_t e0 = C.n_ext;
-          VertexAdj va;
-          const d3 add = shade_vertex(P, C, mk(qin.ox[ray], qin.oy[ray], qin.oz[ray]),
-                                      mk(qin.dx[ray], qin.dy[ray], qin.dz[ray]), hit_t[ray], hit_id[ray], T, Pw,
-                                      (uint32_t)pixel, k, depth, branch, va);
-          delta = (int)(C.n_ext - e0) - 1;
-          accum_add(&st.accum[3 * (size_t)slot + 0], add.x);
-          accum_add(&st.accum[3 * (size_t)slot + 1], add.y);
-          accum_add(&st.accum[3 * (size_t)slot + 2], add.z);
-        }
-        live_update(st, valid ? (int)(slot >> shift) : -1, delta);
+    } else {
+      const uint32_t r = (((c - cont_chunks) << 6) & tmask) + lane_id();
+      valid = r < nt.n_samples;
+      if (valid) {
+        slot = (nt.slot << shift) | r;
+        const uint32_t pl = r / nt.spp;
+        k = r % nt.spp;
+        ray = st.prim_base + nt.ray_base + pl * nt.ss2 + k % nt.ss2;
+        pixel = st.slot_info[nt.slot].pixel_base + pl;
       }
     }
+    int delta = 0;
+    if (valid) {
+      C.slot = slot;
+      const uint32_t e0 = C.n_ext;
+      VertexAdj va;
+      const d3 add = shade_vertex(P, C, mk(qin.ox[ray], qin.oy[ray], qin.oz[ray]),
+                                  mk(qin.dx[ray], qin.dy[ray], qin.dz[ray]), hit_t[ray], hit_id[ray], T, Pw,
+                                  (uint32_t)pixel, k, depth, branch, va);
+      delta = (int)(C.n_ext - e0) - 1;
+      accum_add(&st.accum[3 * (size_t)slot + 0], add.x);
+      accum_add(&st.accum[3 * (size_t)slot + 1], add.y);
+      accum_add(&st.accum[3 * (size_t)slot + 2], add.z);
+    }
+    live_update(st, valid ? (int)(slot >> shift) : -1, delta);
   }
+  log.close(min(s_cnt[wib], st.slab_ray));
   const uint64_t s_pick = wave_sum_u32(C.pickups), s_tr = wave_sum_u32(C.truncated);
   const uint64_t s_ext = wave_sum_u32(C.n_ext), s_drop = wave_sum_u32(C.dropped);
   if (lane_id() == 0) {
+    st.ray_nruns[wave] = log.n < st.max_runs ? log.n : st.max_runs;
     if (s_pick) atomicAdd((unsigned long long*)&st.ctr[CTR_PICKUP], (unsigned long long)s_pick);
     if (s_tr) atomicAdd((unsigned long long*)&st.ctr[CTR_TRUNC], (unsigned long long)s_tr);
     if (s_ext) atomicAdd((unsigned long long*)&st.ctr[CTR_EXT], (unsigned long long)s_ext);
-    if (s_drop) atomicAdd((unsigned long long*)&st.ctr[CTR_DROP], (unsigned long long)s_drop);
+    if (s_drop + log.dropped)
+      atomicAdd((unsigned long long*)&st.ctr[CTR_DROP], (unsigned long long)(s_drop + log.dropped));
   }
 }
 
-// One block: padded prefix of the scene regions of queue nxt (NEE's input).
-__global__ __launch_bounds__(256) void k_stream_scan(Stream st, int nxt) {
-  __shared__ uint32_t s_part[257];
-  const uint32_t NS = (uint32_t)st.n_scenes;
-  const uint32_t* cnt = st.seg_cnt + nxt * NS;
-  const uint32_t* cap = st.seg_cap + nxt * NS;
-  auto width = [&](uint32_t s) { return round64(min(cnt[s], cap[s])); };
-  const ChunkScan sc = block_scan(NS, width, s_part);
-  uint32_t base = sc.base;
-  for (uint32_t s = sc.b; s < sc.e; ++s) {
-    st.nee_pref[s] = base;
-    base += width(s);
+// One block of 1024 threads: the chunk list of a queue from its slab
+// writers' runs (which 0 / 1: ray queue p; 2: the shadow queue), in writer
+// order, and its chunk and entry totals.  Chunks beyond the list's capacity
+// (never, with the host's sizing) are dropped and counted.
+__global__ __launch_bounds__(1024) void k_stream_index(Stream st, int which) {
+  __shared__ uint32_t s_wch[16], s_went[16];
+  const bool sh = which == 2;
+  const RunRec* runs = sh ? st.sh_runs : st.ray_runs;
+  const uint32_t* nruns = sh ? st.sh_nruns : st.ray_nruns;
+  ChunkDesc* desc = sh ? st.sh_desc : st.desc[which];
+  const uint32_t max_desc = sh ? st.max_sh_desc : st.max_desc;
+  const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
+  const uint32_t W = PRT_SLAB_WAVES, per = (W + 1023u) / 1024u;
+  const uint32_t b = min(tid * per, W), e = min(b + per, W);
+  uint32_t ch = 0, ent = 0;
+  for (uint32_t w = b; w < e; ++w) {
+    const uint32_t n = min(nruns[w], st.max_runs);
+    for (uint32_t r = 0; r < n; ++r) {
+      const uint32_t cnt = runs[(size_t)w * st.max_runs + r].cnt;
+      ch += (cnt + 63u) >> 6;
+      ent += cnt;
+    }
   }
-  if (threadIdx.x == 255) st.nee_pref[NS] = sc.total;
-  if (threadIdx.x == 0) st.ctl->nee_width = sc.total;
-}
-
-// Light samples of the diffuse vertices behind the rays of queue nxt.
-__global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, ShadowQ sq) {
-  const uint32_t n_chunks = st.ctl->nee_width >> 6;
-  const uint32_t NS = (uint32_t)st.n_scenes;
-  const uint32_t shift = (uint32_t)st.tile_shift, tmask = (1u << shift) - 1u;
-  uint32_t emitted = 0, dropped = 0;
-  {
-    uint32_t c, c_end;
-    wave_range(n_chunks, c, c_end);
-    while (c < c_end) {
-      const uint32_t s = find_entry(st.nee_pref, NS, c << 6);
-      const uint32_t s_base = st.nee_pref[s];
-      const uint32_t run_end = min(st.nee_pref[s + 1] >> 6, c_end);
-      const uint32_t cnt = min(st.seg_cnt[nxt * NS + s], st.seg_cap[nxt * NS + s]);
-      const uint32_t off = st.seg_off[nxt * NS + s];
-      const Params& P = scene_params(st.scenes, (int)s);
-      for (; c < run_end; ++c) {
-        const uint32_t local = (c << 6) - s_base + lane_id();
-        bool have = false;
-        uint32_t slot = 0, w[9];
-        d3 x, nrm, ct;
-        if (local < cnt) {
-          const uint32_t i = off + local;
-          const uint32_t meta = q.meta[i];
-          if (meta & META_NEE) {
-            have = true;
-            slot = q.slot[i];
-            const SlotInfo si = st.slot_info[slot >> shift];
-            const uint32_t r = slot & tmask;
-            const uint32_t k = r % (uint32_t)si.spp;
-            const int64_t pixel = si.pixel_base + r / (uint32_t)si.spp;
-            nee_words(P, (uint32_t)pixel, k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta), w);
-            x = mk(q.ox[i], q.oy[i], q.oz[i]);
-            nrm = mk(q.nx[i], q.ny[i], q.nz[i]);
-            ct = mk(q.tr[i], q.tg[i], q.tb[i]);
-          }
-        }
-        ShadowRay s0, s1, s2;
-        int32_t l0 = -1, l1 = -1, l2 = -1;
-        const bool v0 = have && nee_one(P, w, 0, x, nrm, ct, s0, l0);
-        const bool v1 = have && nee_one(P, w, 1, x, nrm, ct, s1, l1);
-        const bool v2 = have && nee_one(P, w, 2, x, nrm, ct, s2, l2);
-        uint32_t n_sh = (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2;
-        uint32_t j = wave_append_n(&st.ctl->cnt_sq, n_sh);
-        if (j + n_sh > sq.cap) {
-          dropped += n_sh;
-          n_sh = 0;
+  uint32_t ich = ch, ient = ent;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t a1 = (uint32_t)__shfl_up((int)ich, o, 64), a2 = (uint32_t)__shfl_up((int)ient, o, 64);
+    if (lane >= (uint32_t)o) {
+      ich += a1;
+      ient += a2;
+    }
+  }
+  if (lane == 63) {
+    s_wch[wv] = ich;
+    s_went[wv] = ient;
+  }
+  __syncthreads();
+  uint32_t base = 0, tot_ch = 0, tot_ent = 0;
+  for (uint32_t k = 0; k < 16; ++k) {
+    if (k < wv) base += s_wch[k];
+    tot_ch += s_wch[k];
+    tot_ent += s_went[k];
+  }
+  base += ich - ch;
+  uint32_t lost = 0;
+  for (uint32_t w = b; w < e; ++w) {
+    const uint32_t n = min(nruns[w], st.max_runs);
+    for (uint32_t r = 0; r < n; ++r) {
+      const RunRec rr = runs[(size_t)w * st.max_runs + r];
+      for (uint32_t j = 0; j < rr.cnt; j += 64) {
+        if (base < max_desc) {
+          ChunkDesc d;
+          d.base = rr.off + j;
+          d.cnt = min(64u, rr.cnt - j);
+          d.scene = rr.scene;
+          d.pad_ = 0;
+          desc[base] = d;
         } else {
-          if (v0) put_shadow(sq, j++, s0, slot, l0);
-          if (v1) put_shadow(sq, j++, s1, slot, l1);
-          if (v2) put_shadow(sq, j++, s2, slot, l2);
+          lost += min(64u, rr.cnt - j);
         }
-        emitted += n_sh;
-        live_update(st, n_sh > 0 ? (int)(slot >> shift) : -1, (int)n_sh);
+        ++base;
       }
     }
   }
-  const uint64_t s_sh = wave_sum_u32(emitted), s_drop = wave_sum_u32(dropped);
-  if (lane_id() == 0) {
-    if (s_sh) atomicAdd((unsigned long long*)&st.ctr[CTR_SHADOW], (unsigned long long)s_sh);
-    if (s_drop) atomicAdd((unsigned long long*)&st.ctr[CTR_DROP], (unsigned long long)s_drop);
+  if (lost) atomicAdd((unsigned long long*)&st.ctr[CTR_DROP], (unsigned long long)lost);
+  if (tid == 0) {
+    StreamCtl* c = st.ctl;
+    const uint32_t n_ch = min(tot_ch, max_desc);
+    if (sh) {
+      c->sh_chunks = n_ch;
+      c->sh_entries = tot_ent;
+    } else {
+      c->cont_chunks[which] = n_ch;
+      c->cont_entries[which] = tot_ent;
+    }
   }
 }
 
-// Any-hit tests of the shared shadow queue; unoccluded rays add their
+// Light samples of the diffuse vertices behind the rays of queue nxt's slab
+// chunks; shadow rays go to this wave's shadow slab, logged as runs.
+__global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, ShadowQ sq) {
+  __shared__ uint32_t s_cnt[4];  // entries in each wave's shadow slab
+  const uint32_t wib = threadIdx.x >> 6;
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (lane_id() == 0) s_cnt[wib] = 0;
+  const uint32_t shift = (uint32_t)st.tile_shift, tmask = (1u << shift) - 1u;
+  const ChunkDesc* desc = st.desc[nxt];
+  const uint32_t off = wave * st.slab_sh;
+  RunLog log{st.sh_runs + (size_t)wave * st.max_runs, st.max_runs, 0u, 0u, off, 0u, -1};
+  uint32_t emitted = 0, dropped = 0;
+  uint32_t c, c_end;
+  wave_range(st.ctl->cont_chunks[nxt], c, c_end);
+  for (; c < c_end; ++c) {
+    const ChunkDesc dc = chunk_at(desc, c);
+    log.enter(dc.scene, min(s_cnt[wib], st.slab_sh));
+    const Params& P = scene_params(st.scenes, dc.scene);
+    bool have = false;
+    uint32_t slot = 0, w[9];
+    d3 x, nrm, ct;
+    if (lane_id() < dc.cnt) {
+      const uint32_t i = dc.base + lane_id();
+      const uint32_t meta = q.meta[i];
+      if (meta & META_NEE) {
+        have = true;
+        slot = q.slot[i];
+        const SlotInfo si = st.slot_info[slot >> shift];
+        const uint32_t r = slot & tmask;
+        const uint32_t k = r % (uint32_t)si.spp;
+        const int64_t pixel = si.pixel_base + r / (uint32_t)si.spp;
+        nee_words(P, (uint32_t)pixel, k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta), w);
+        x = mk(q.ox[i], q.oy[i], q.oz[i]);
+        nrm = mk(q.nx[i], q.ny[i], q.nz[i]);
+        ct = mk(q.tr[i], q.tg[i], q.tb[i]);
+      }
+    }
+    // one light sample at a time, appended through the LDS count: only one
+    // shadow ray's registers are live
+    uint32_t n_sh = 0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      ShadowRay sr;
+      int32_t l = -1;
+      const bool v = have && nee_one(P, w, m, x, nrm, ct, sr, l);
+      const uint32_t j = wave_append_bit(&s_cnt[wib], v);
+      if (v) {
+        if (j < st.slab_sh) {
+          put_shadow(sq, off + j, sr, slot, l);
+          ++n_sh;
+        } else {
+          ++dropped;
+        }
+      }
+    }
+    emitted += n_sh;
+    live_update(st, n_sh > 0 ? (int)(slot >> shift) : -1, (int)n_sh);
+  }
+  log.close(min(s_cnt[wib], st.slab_sh));
+  const uint64_t s_sh = wave_sum_u32(emitted), s_drop = wave_sum_u32(dropped);
+  if (lane_id() == 0) {
+    st.sh_nruns[wave] = log.n < st.max_runs ? log.n : st.max_runs;
+    if (s_sh) atomicAdd((unsigned long long*)&st.ctr[CTR_SHADOW], (unsigned long long)s_sh);
+    if (s_drop + log.dropped)
+      atomicAdd((unsigned long long*)&st.ctr[CTR_DROP], (unsigned long long)(s_drop + log.dropped));
+  }
+}
+
+// Any-hit tests of the shadow queue's chunks; unoccluded rays add their
 // contribution.
 template <int NL>
 __global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq) {
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL);
-  const uint32_t n = min(st.ctl->cnt_sq, sq.cap);
-  uint32_t* fetch = &st.ctl->fetch[1];
   const uint32_t shift = (uint32_t)st.tile_shift;
+  const ClaimRange cr = claim_range(st.fetch + (size_t)PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE, st.ctl->sh_chunks);
   TravStats ts{0, 0, 0};
-  uint32_t extra = 0;
-  while (true) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(fetch, 64u);
-    base = (uint32_t)__shfl((int)base, 0, 64);
-    if (base >= n) break;
-    const uint32_t i = base + lane_id();
-    uint32_t slot = 0;
-    int sid = -1;
-    d3 o = mk(0.0, 0.0, 0.0), d = o;
-    double tmax = 0.0;
-    if (i < n) {
-      slot = sq.slot[i];
-      o = mk(sq.ox[i], sq.oy[i], sq.oz[i]);
-      d = mk(sq.dx[i], sq.dy[i], sq.dz[i]);
-      tmax = sq.tmax[i];
-      sid = st.slot_info[slot >> shift].scene;
+  uint32_t c, c1;
+  while (claim(cr, FETCH_CHUNKS, c, c1)) {
+    for (; c < c1; ++c) {
+      const ChunkDesc dc = chunk_at(st.sh_desc, c);
+      const Params& P = scene_params(st.scenes, dc.scene);
+      const bool valid = lane_id() < dc.cnt;
+      uint32_t slot = 0;
+      if (valid) {
+        const uint32_t i = dc.base + lane_id();
+        slot = sq.slot[i];
+        if (!occluded(P, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), sq.tmax[i], S, ts)) {
+          accum_add(&st.accum[3 * (size_t)slot + 0], sq.cr[i]);
+          accum_add(&st.accum[3 * (size_t)slot + 1], sq.cg[i]);
+          accum_add(&st.accum[3 * (size_t)slot + 2], sq.cb[i]);
+        }
+      }
+      live_update(st, valid ? (int)(slot >> shift) : -1, -1);
     }
-    bool lit = false;
-    const uint32_t passes =
-        per_scene(st.scenes, sid, [&](const Params& P) { lit = !occluded(P, o, d, tmax, S, ts); });
-    extra += passes > 1 ? passes - 1 : 0;
-    if (lit) {
-      accum_add(&st.accum[3 * (size_t)slot + 0], sq.cr[i]);
-      accum_add(&st.accum[3 * (size_t)slot + 1], sq.cg[i]);
-      accum_add(&st.accum[3 * (size_t)slot + 2], sq.cb[i]);
-    }
-    live_update(st, sid >= 0 ? (int)(slot >> shift) : -1, -1);
   }
   flush_stats(st.ctr, CTR_SNODES, ts);
-  flush_mixed(st.ctr, extra);
 }
 
 // Finished tiles (busy slots whose live count is zero after this iteration's
@@ -637,27 +604,27 @@ hipError_t launch_stream_gen(const Stream& st, const RayQ& q, int cur, int block
   hipLaunchKernelGGL(k_stream_gen, dim3(blocks), dim3(256), 0, s, st, q, cur);
   return hipGetLastError();
 }
-hipError_t launch_stream_extend(const Stream& st, const RayQ& q, double* hit_t, int32_t* hit_id, int stack_lds,
-                                int blocks, hipStream_t s) {
+hipError_t launch_stream_extend(const Stream& st, const RayQ& q, int cur, double* hit_t, int32_t* hit_id,
+                                int stack_lds, int blocks, hipStream_t s) {
   if (stack_lds <= 8)
-    hipLaunchKernelGGL(k_stream_extend<8>, dim3(blocks), dim3(256), 0, s, st, q, hit_t, hit_id);
+    hipLaunchKernelGGL(k_stream_extend<8>, dim3(blocks), dim3(256), 0, s, st, q, cur, hit_t, hit_id);
   else if (stack_lds <= 16)
-    hipLaunchKernelGGL(k_stream_extend<16>, dim3(blocks), dim3(256), 0, s, st, q, hit_t, hit_id);
+    hipLaunchKernelGGL(k_stream_extend<16>, dim3(blocks), dim3(256), 0, s, st, q, cur, hit_t, hit_id);
   else
-    hipLaunchKernelGGL(k_stream_extend<32>, dim3(blocks), dim3(256), 0, s, st, q, hit_t, hit_id);
+    hipLaunchKernelGGL(k_stream_extend<32>, dim3(blocks), dim3(256), 0, s, st, q, cur, hit_t, hit_id);
   return hipGetLastError();
 }
 hipError_t launch_stream_shade(const Stream& st, const RayQ& qin, const double* hit_t, const int32_t* hit_id,
-                               const RayQ& qout, int nxt, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_stream_shade, dim3(blocks), dim3(256), 0, s, st, qin, hit_t, hit_id, qout, nxt);
+                               const RayQ& qout, int nxt, hipStream_t s) {
+  hipLaunchKernelGGL(k_stream_shade, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, st, qin, hit_t, hit_id, qout, nxt);
   return hipGetLastError();
 }
-hipError_t launch_stream_scan(const Stream& st, int nxt, hipStream_t s) {
-  hipLaunchKernelGGL(k_stream_scan, dim3(1), dim3(256), 0, s, st, nxt);
+hipError_t launch_stream_index(const Stream& st, int which, hipStream_t s) {
+  hipLaunchKernelGGL(k_stream_index, dim3(1), dim3(1024), 0, s, st, which);
   return hipGetLastError();
 }
-hipError_t launch_stream_nee(const Stream& st, const RayQ& q, int nxt, const ShadowQ& sq, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_stream_nee, dim3(blocks), dim3(256), 0, s, st, q, nxt, sq);
+hipError_t launch_stream_nee(const Stream& st, const RayQ& q, int nxt, const ShadowQ& sq, hipStream_t s) {
+  hipLaunchKernelGGL(k_stream_nee, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, st, q, nxt, sq);
   return hipGetLastError();
 }
 hipError_t launch_stream_shadow(const Stream& st, const ShadowQ& sq, int stack_lds, int blocks, hipStream_t s) {

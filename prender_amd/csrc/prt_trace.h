@@ -7,6 +7,7 @@
 // reference's only in being conservative (boxes carry a margin and NaN slabs
 // impose no bound), so it never prunes a box that holds the closest hit.
 #pragma once
+#include "prt_kernels.h"
 #include "prt_math.h"
 
 struct TravStats {
@@ -291,16 +292,46 @@ __device__ __forceinline__ bool occluded(const Params& P, d3 o, d3 d, double tma
   return traverse<true>(P, o, d, tmax, id, u, v, S, ts);
 }
 
+// ------------------------------------------------ chunk claims
+// The chunks [0, n_chunks) of a queue are split into PRT_FETCH_COUNTERS
+// sub-ranges (fewer when the grid has fewer waves: every sub-range needs a
+// wave that drains it), each with its own claim counter on its own 128-byte
+// line, so no counter is shared by the whole device.  A wave draws from the
+// sub-range wave % n_sub.
+struct ClaimRange {
+  uint32_t* counter;
+  uint32_t b, e;
+};
+__device__ __forceinline__ ClaimRange claim_range(uint32_t* fetch, uint32_t n_chunks) {
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t n_sub = n_waves < (uint32_t)PRT_FETCH_COUNTERS ? n_waves : (uint32_t)PRT_FETCH_COUNTERS;
+  const uint32_t sub = wave % n_sub;
+  ClaimRange r;
+  r.counter = fetch + (size_t)sub * PRT_FETCH_STRIDE;
+  r.b = (uint32_t)((uint64_t)sub * n_chunks / n_sub);
+  r.e = (uint32_t)((uint64_t)(sub + 1) * n_chunks / n_sub);
+  return r;
+}
+// Claims up to k chunks [c0, c1) of the wave's sub-range (one atomic);
+// false once it is drained.  Every lane of the wave must call it.
+__device__ __forceinline__ bool claim(const ClaimRange& r, uint32_t k, uint32_t& c0, uint32_t& c1) {
+  uint32_t x = 0;
+  if (lane_id() == 0) x = atomicAdd(r.counter, k);
+  x = (uint32_t)__builtin_amdgcn_readfirstlane(__shfl((int)x, 0, 64));
+  c0 = r.b + x;
+  if (c0 >= r.e) return false;
+  c1 = min(c0 + k, r.e);
+  return true;
+}
+
 // ------------------------------------------------ queue traversal with refill
 //
 // Closest or any hit for the rays of a chunked queue (chunk c holds desc[c].y
 // rays from queue index desc[c].x on), the while-while loop of traverse_ww
-// with lanes refilled as their rays finish.  The chunks are split into
-// PRT_FETCH_COUNTERS sub-ranges (fewer when the grid has fewer waves), each
-// with its own claim counter on its own
-// 128-byte line; a wave claims PRT_FETCH_RAYS / 64 chunks per atomic from
-// the counter of its sub-range (so no counter is shared by the whole
-// device), keeps the claimed entries not yet started as a pool, and hands
+// with lanes refilled as their rays finish.  A wave claims PRT_FETCH_RAYS /
+// 64 chunks per atomic from its claim sub-range (claim_range), keeps the
+// claimed entries not yet started as a pool, and hands
 // them to idle lanes at the top of each round once at least PRT_REFILL_MIN
 // lanes are idle.  Without refill a wave lives as long as its slowest ray
 // and the other lanes idle meanwhile.  Each ray's own traversal is
@@ -323,16 +354,7 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
   const uint32_t lane = lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
   constexpr uint32_t kClaim = (uint32_t)PRT_FETCH_RAYS / 64u;  // chunks per claim
-  // this wave's sub-range of chunks and its counter
-  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-  // as many sub-ranges as counters, but no more than waves: every sub-range
-  // needs a wave that drains it
-  const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-  const uint32_t n_sub = n_waves < (uint32_t)PRT_FETCH_COUNTERS ? n_waves : (uint32_t)PRT_FETCH_COUNTERS;
-  const uint32_t sub = wave % n_sub;
-  const uint32_t sub_b = (uint32_t)((uint64_t)sub * n_chunks / n_sub);
-  const uint32_t sub_e = (uint32_t)((uint64_t)(sub + 1) * n_chunks / n_sub);
-  uint32_t* counter = fetch + (size_t)sub * PRT_FETCH_STRIDE;
+  const ClaimRange cr = claim_range(fetch, n_chunks);
   d3 o = mk(0.0, 0.0, 0.0), d = o;
   float ix = 0.0f, iy = 0.0f, iz = 0.0f, oix = 0.0f, oiy = 0.0f, oiz = 0.0f, tb = 0.0f;
   double t_best = 0.0;
@@ -344,7 +366,7 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
   // g = chunk * 64 + lane slot) not yet started; more: the sub-range may hold
   // unclaimed chunks
   uint32_t pool_base = 0, pool_n = 0;
-  bool more = sub_b < sub_e;
+  bool more = cr.b < cr.e;
   auto pop_next = [&]() -> int32_t {
     while (sp > 0) {
       int32_t node;
@@ -366,20 +388,16 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
       pool_base += take;
       pool_n -= take;
       if (n_idle > take && more) {
-        uint32_t x = 0;
-        if (lane == 0) x = atomicAdd(counter, kClaim);
-        x = (uint32_t)__shfl((int)x, 0, 64);
-        const uint32_t c0 = sub_b + x;
-        if (c0 >= sub_e) {
+        uint32_t c0, c1;
+        if (!claim(cr, kClaim, c0, c1)) {
           more = false;
         } else {
-          const uint32_t nc = (sub_e - c0) < kClaim ? (sub_e - c0) : kClaim;
-          const uint32_t ent = nc * 64u;
+          const uint32_t ent = (c1 - c0) * 64u;
           const uint32_t need = n_idle - take, used = need < ent ? need : ent;
           if (!act && rank >= take && rank - take < ent) g = c0 * 64u + (rank - take);
           pool_base = c0 * 64u + used;
           pool_n = ent - used;
-          if (c0 + nc >= sub_e) more = false;
+          if (c1 >= cr.e) more = false;
         }
       }
       if (g >= 0) {
