@@ -37,8 +37,12 @@ struct BNode {
 };
 
 constexpr int kBins = 32;
+// SAH costs of a binary node step and of a triangle test
+#ifndef PRT_SAH_COST_TRI
+#define PRT_SAH_COST_TRI 1.0f
+#endif
 constexpr float kCostNode = 1.0f;
-constexpr float kCostTri = 1.0f;
+constexpr float kCostTri = PRT_SAH_COST_TRI;
 constexpr int64_t kParallelMin = 1 << 15;
 
 struct Builder {

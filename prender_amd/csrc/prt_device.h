@@ -14,7 +14,9 @@
 #pragma once
 #include <stdint.h>
 
+#ifndef PRT_MAX_LEAF
 #define PRT_MAX_LEAF 8
+#endif
 #ifndef PRT_STACK_LDS
 #define PRT_STACK_LDS 16
 #endif

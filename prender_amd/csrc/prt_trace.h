@@ -20,10 +20,10 @@ struct TravStats {
 // (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * 2 words.
 template <int NL>
 struct StackN {
-  uint32_t* lds_node;  // [NL][blockDim]
+  static constexpr int stride = 256;  // threads per block of every traversal kernel
+  uint32_t* lds_node;  // [NL][256]
   float* lds_t;
   uint32_t* ovf;       // (PRT_STACK_TOTAL - NL) * 2 words used
-  int stride;
   __device__ __forceinline__ void push(int sp, int32_t node, float t) {
     if (sp < NL) {
       lds_node[sp * stride] = (uint32_t)node;
@@ -52,8 +52,7 @@ typedef StackN<PRT_STACK_LDS> Stack;
   __shared__ float s_t_[(NL) * 256];                                                                         \
   StackN<NL> S{s_node_ + threadIdx.x, s_t_ + threadIdx.x,                                                    \
                (P).stack_ovf + (size_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % (P).ovf_threads) * \
-                                   2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN),                                \
-               (int)blockDim.x}
+                                   2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN)}
 #define PRT_DECLARE_STACK(S, P) PRT_DECLARE_STACK_N(S, P, PRT_STACK_LDS)
 
 // a float >= t (t > 0): the culling bound never drops below the double hit
@@ -62,13 +61,13 @@ __device__ __forceinline__ float fup(double t) {
   return (double)f < t ? __uint_as_float(__float_as_uint(f) + 1u) : f;
 }
 
-// 1: while-while traversal (traverse_ww); 0: one node or leaf per loop trip
-#ifndef PRT_TRAVERSE_WW
-#define PRT_TRAVERSE_WW 1
-#endif
-
 // Slab test of the 4 children of node `cur` against the ray's float slabs:
 // entry distances sorted ascending with their child references (+inf: miss).
+// The sort is branch-free: each distance (>= 0, sign cleared so -0 sorts as
+// 0) becomes a 32-bit key whose bit pattern orders like the float, with the
+// child slot in its two low bits, and a min/max network sorts the keys.  The
+// distance read back from a key is rounded down by at most 3 ulp, so culling
+// against it stays conservative; +inf stays +inf.
 __device__ __forceinline__ void node_children(const DevNode* nodes, int32_t cur, float ix, float iy, float iz,
                                               float oix, float oiy, float oiz, float tb, float tn[4],
                                               int32_t cc[4]) {
@@ -77,7 +76,7 @@ __device__ __forceinline__ void node_children(const DevNode* nodes, int32_t cur,
   const int4 ch = *reinterpret_cast<const int4*>(nd + 6);
   const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
   const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-  const int32_t cha[4] = {ch.x, ch.y, ch.z, ch.w};
+  uint32_t key[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const float x0 = __builtin_fmaf(lxa[k], ix, -oix), x1 = __builtin_fmaf(hxa[k], ix, -oix);
@@ -85,20 +84,32 @@ __device__ __forceinline__ void node_children(const DevNode* nodes, int32_t cur,
     const float z0 = __builtin_fmaf(lza[k], iz, -oiz), z1 = __builtin_fmaf(hza[k], iz, -oiz);
     const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
     const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
-    tn[k] = (tnear <= tfar) ? tnear : __builtin_inff();
-    cc[k] = cha[k];
+    const uint32_t tk = (tnear <= tfar) ? (__float_as_uint(tnear) & 0x7FFFFFFCu) : 0x7F800000u;
+    key[k] = tk | (uint32_t)k;
   }
-#define PRT_CSWAP(a, b)                  \
-  if (tn[b] < tn[a]) {                   \
-    const float tt = tn[a];              \
-    tn[a] = tn[b];                       \
-    tn[b] = tt;                          \
-    const int32_t c_ = cc[a];            \
-    cc[a] = cc[b];                       \
-    cc[b] = c_;                          \
+#define PRT_KSORT(a, b)                                \
+  {                                                    \
+    const uint32_t lo_ = min(key[a], key[b]);          \
+    key[b] = max(key[a], key[b]);                      \
+    key[a] = lo_;                                      \
   }
-  PRT_CSWAP(0, 1) PRT_CSWAP(2, 3) PRT_CSWAP(0, 2) PRT_CSWAP(1, 3) PRT_CSWAP(1, 2)
-#undef PRT_CSWAP
+  PRT_KSORT(0, 1) PRT_KSORT(2, 3) PRT_KSORT(0, 2) PRT_KSORT(1, 3) PRT_KSORT(1, 2)
+#undef PRT_KSORT
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t slot = key[j] & 3u;
+    tn[j] = key[j] >= 0x7F800000u ? __builtin_inff() : __uint_as_float(key[j] & ~3u);
+    cc[j] = (slot & 2u) ? ((slot & 1u) ? ch.w : ch.z) : ((slot & 1u) ? ch.y : ch.x);
+  }
+}
+
+// Pushes children 3, 2, 1 of a sorted node test (farthest first) that were
+// hit.
+template <class Stk>
+__device__ __forceinline__ void push_children(Stk& S, int& sp, const float tn[4], const int32_t cc[4]) {
+#pragma unroll
+  for (int j = 3; j >= 1; --j)
+    if (tn[j] != __builtin_inff()) S.push(sp++, cc[j], tn[j]);
 }
 
 // While-while traversal with speculative leaf postponement (Aila & Laine,
@@ -111,8 +122,9 @@ __device__ __forceinline__ void node_children(const DevNode* nodes, int32_t cur,
 //
 // Every triangle a ray tests is one whose box the conservative culling
 // leaves open, and a hit is kept only when strictly closer, so the closest
-// hit equals traverse()'s except which of several triangles at exactly the
-// same t is reported (the test order differs).
+// hit is the one over all triangles; of several triangles at exactly the
+// same t, the one tested first is reported (the test order is not the
+// reference QBVH's, whose tree differs anyway).
 #define PRT_TRAV_DONE 0x7FFFFFFF
 template <bool ANY, class Stk>
 __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double& t_best, int32_t& id_best, double& ub,
@@ -147,9 +159,7 @@ __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double&
       int32_t cc[4];
       node_children(P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
       ts.nodes++;
-      if (tn[3] != __builtin_inff()) S.push(sp++, cc[3], tn[3]);
-      if (tn[2] != __builtin_inff()) S.push(sp++, cc[2], tn[2]);
-      if (tn[1] != __builtin_inff()) S.push(sp++, cc[1], tn[1]);
+      push_children(S, sp, tn, cc);
       cur = (tn[0] != __builtin_inff()) ? cc[0] : pop_next();
       if (cur < 0 && lf >= 0) {  // park the first leaf and keep walking
         lf = cur;
@@ -192,71 +202,7 @@ __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double&
 template <bool ANY, class Stk>
 __device__ __forceinline__ bool traverse(const Params& P, d3 o, d3 d, double& t_best, int32_t& id_best, double& ub,
                                          double& vb, Stk& S, TravStats& ts) {
-#if PRT_TRAVERSE_WW
   return traverse_ww<ANY>(P, o, d, t_best, id_best, ub, vb, S, ts);
-#else
-  if (P.n_nodes <= 0) return false;
-  const float ox = (float)o.x, oy = (float)o.y, oz = (float)o.z;
-  float ix = (float)(1.0 / d.x), iy = (float)(1.0 / d.y), iz = (float)(1.0 / d.z);
-  if (__builtin_isinf(ix)) ix = __builtin_copysignf(1.0e30f, ix);
-  if (__builtin_isinf(iy)) iy = __builtin_copysignf(1.0e30f, iy);
-  if (__builtin_isinf(iz)) iz = __builtin_copysignf(1.0e30f, iz);
-  const float oix = ox * ix, oiy = oy * iy, oiz = oz * iz;
-  float tb = t_best < 1.0e30 ? fup(t_best) : __builtin_inff();
-  bool found = false;
-  int sp = 0;
-  int32_t cur = 0;
-  const float4* tris4 = reinterpret_cast<const float4*>(P.tris);
-  while (true) {
-    if (cur >= 0) {
-      float tn[4];
-      int32_t cc[4];
-      node_children(P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
-      ts.nodes++;
-      if (tn[3] != __builtin_inff()) S.push(sp++, cc[3], tn[3]);
-      if (tn[2] != __builtin_inff()) S.push(sp++, cc[2], tn[2]);
-      if (tn[1] != __builtin_inff()) S.push(sp++, cc[1], tn[1]);
-      if (tn[0] != __builtin_inff()) {
-        cur = cc[0];
-        continue;
-      }
-    } else {
-      const uint32_t code = ~(uint32_t)cur;
-      const uint32_t first = code & 0x07FFFFFFu;
-      const uint32_t count = (code >> 27) + 1u;
-      for (uint32_t k = 0; k < count; ++k) {
-        const float4* tr = tris4 + (size_t)(first + k) * 3;
-        const float4 a = tr[0], b = tr[1], c = tr[2];
-        ts.tris++;
-        double t, u, v;
-        if (mt_test(a, b, c, o, d, t, u, v) && t < t_best) {
-          if (ANY) {
-            id_best = (int32_t)(first + k);
-            return true;
-          }
-          t_best = t;
-          ub = u;
-          vb = v;
-          id_best = (int32_t)(first + k);
-          tb = fup(t);
-          found = true;
-        }
-      }
-    }
-    // pop the nearest pending entry that can still hold a closer hit
-    bool got = false;
-    while (sp > 0) {
-      float t;
-      S.pop(--sp, cur, t);
-      if (t <= tb) {
-        got = true;
-        break;
-      }
-    }
-    if (!got) break;
-  }
-  return found;
-#endif
 }
 
 // Closest hit over analytic entities then the BVH; o_in is the ray origin
@@ -454,9 +400,7 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
       int32_t cc[4];
       node_children(P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
       ts.nodes++;
-      if (tn[3] != __builtin_inff()) S.push(sp++, cc[3], tn[3]);
-      if (tn[2] != __builtin_inff()) S.push(sp++, cc[2], tn[2]);
-      if (tn[1] != __builtin_inff()) S.push(sp++, cc[1], tn[1]);
+      push_children(S, sp, tn, cc);
       cur = (tn[0] != __builtin_inff()) ? cc[0] : pop_next();
       if (cur < 0 && lf >= 0) {
         lf = cur;
