@@ -296,7 +296,7 @@ void prt_scene_file_free(prt_scene_file* file);
  * "%NAME%" environment references); *status = PRT_OK or an error. */
 double prt_eval_expression(const char* text, int* status);
 
-/* ---- image output (ppm.h:14-35, render.h:1458-1560, bitmap.cpp) ---------- */
+/* ---- image output (ppm.h:14-35, render.h:1458-1667, bitmap.cpp, hdr.h) --- */
 /* rgb: W*H*3 floats in the reference's storage order (row 0 = top).
  * 8-bit values int(pow(clamp(x,0,1), gamma) * 255 + 0.5); gamma <= 0 means
  * 1/2.2. */
@@ -306,6 +306,9 @@ int prt_write_ppm(const char* path, const float* rgb, int width, int height);
 /* 24-bit BMP as Render::OutputBmp writes it; |gamma_offset| > 0.01 replaces
  * the 1/2.2 exponent. */
 int prt_write_bmp(const char* path, const float* rgb, int width, int height, double gamma_offset);
+/* Radiance RGBE file as Render::OutputHdr writes it (render.h:1647-1667,
+ * hdr.h:212-246): linear values, no gamma. */
+int prt_write_hdr(const char* path, const float* rgb, int width, int height);
 
 #ifdef __cplusplus
 }

@@ -7,6 +7,14 @@
 //           first file row being image row H-1 (Render::OutputBmp
 //           render.h:1507-1560 over BitMap::Write_Bmp bitmap.cpp:190-285);
 //           gamma_offset > 0.01 in magnitude replaces 1/2.2 (render.h:1540).
+//   HDR     Radiance RGBE (Render::OutputHdr render.h:1647-1667 over
+//           HDRImage::save hdr.h:212-246): a five-line header, "-Y H +X W",
+//           then per scanline (image rows 0..H-1) the marker 2 2 W_hi W_lo
+//           and the R, G, B, E planes as literal runs of at most 127 bytes;
+//           a pixel is (r, g, b) * m * 256 / d truncated to bytes with
+//           d = max(r, g, b) = m * 2^e (frexp) and E = e + 128, or all zero
+//           when d <= 1e-32.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -103,6 +111,48 @@ int prt_write_bmp(const char* path, const float* rgb, int width, int height, dou
       line[3 * j + 2] = (unsigned char)to_int(src[3 * j + 0], g);
     }
     std::fwrite(line.data(), 1, line.size(), f);
+  }
+  const bool ok = std::ferror(f) == 0;
+  std::fclose(f);
+  return ok ? PRT_OK : fail(PRT_E_INVALID, std::string("write error on ") + path);
+}
+
+int prt_write_hdr(const char* path, const float* rgb, int width, int height) {
+  int rc;
+  if ((rc = check_image(rgb, width, height))) return rc;
+  if (width > 0xFFFF) return fail(PRT_E_INVALID, "HDR scanlines hold at most 65535 pixels");
+  FILE* f = path ? std::fopen(path, "wb") : nullptr;
+  if (!f) return fail(PRT_E_INVALID, std::string("cannot write ") + (path ? path : "(null)"));
+  std::fputs("#?RADIANCE\n# Made with 100% pure HDR Shop\nFORMAT=32-bit_rle_rgbe\nEXPOSURE=1.0000000000000\n\n", f);
+  std::fprintf(f, "-Y %d +X %d\n", height, width);
+  std::vector<unsigned char> planes((size_t)4 * width);
+  std::vector<unsigned char> out;
+  for (int y = 0; y < height; ++y) {
+    const float* src = rgb + (size_t)y * width * 3;
+    for (int x = 0; x < width; ++x) {
+      const double r = src[3 * x], g = src[3 * x + 1], b = src[3 * x + 2];
+      const double d = std::max(r, std::max(g, b));
+      unsigned char q[4] = {0, 0, 0, 0};
+      if (d > 1e-32) {
+        int e;
+        const double m = std::frexp(d, &e);
+        const double k = m * 256.0 / d;
+        q[0] = (unsigned char)(r * k);
+        q[1] = (unsigned char)(g * k);
+        q[2] = (unsigned char)(b * k);
+        q[3] = (unsigned char)(e + 128);
+      }
+      for (int c = 0; c < 4; ++c) planes[(size_t)c * width + x] = q[c];
+    }
+    out.assign({0x02, 0x02, (unsigned char)((width >> 8) & 0xFF), (unsigned char)(width & 0xFF)});
+    for (int c = 0; c < 4; ++c)
+      for (int x = 0; x < width;) {
+        const int run = std::min(127, width - x);
+        out.push_back((unsigned char)run);
+        out.insert(out.end(), planes.begin() + (size_t)c * width + x, planes.begin() + (size_t)c * width + x + run);
+        x += run;
+      }
+    std::fwrite(out.data(), 1, out.size(), f);
   }
   const bool ok = std::ferror(f) == 0;
   std::fclose(f);

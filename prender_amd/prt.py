@@ -77,6 +77,7 @@ def lib():
     L.prt_image_to_rgb8.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p]
     L.prt_write_ppm.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_int]
     L.prt_write_bmp.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_int, C.c_double]
+    L.prt_write_hdr.argtypes = [C.c_char_p, C.c_void_p, C.c_int, C.c_int]
     _lib = L
     return L
 
@@ -86,7 +87,7 @@ EXPORTED = ["prt_abi_version", "prt_last_error", "prt_device_info", "prt_scene_c
             "prt_render_batch_device", "prt_tune_batch", "prt_set_face_albedo_device", "prt_set_emission",
             "prt_hold_rr_albedo", "prt_adjoint_device", "prt_scene_file_load", "prt_scene_file_desc",
             "prt_scene_file_output", "prt_scene_file_free", "prt_eval_expression", "prt_image_to_rgb8",
-            "prt_write_ppm", "prt_write_bmp"]
+            "prt_write_ppm", "prt_write_bmp", "prt_write_hdr"]
 
 
 def check(rc: int, what: str):
@@ -291,3 +292,8 @@ def write_ppm(path: str, rgb: np.ndarray):
 def write_bmp(path: str, rgb: np.ndarray, gamma_offset: float = 0.0):
     a, w, h = _image(rgb)
     check(lib().prt_write_bmp(os.fsencode(path), a.ctypes.data, w, h, float(gamma_offset)), "prt_write_bmp")
+
+
+def write_hdr(path: str, rgb: np.ndarray):
+    a, w, h = _image(rgb)
+    check(lib().prt_write_hdr(os.fsencode(path), a.ctypes.data, w, h), "prt_write_hdr")

@@ -267,3 +267,43 @@ def test_bmp_file(tmp_path):
         line = b[54 + i * row: 54 + (i + 1) * row]
         expect = q[2 - i][:, ::-1].reshape(-1).tolist()
         assert list(line[:15]) == expect and list(line[15:]) == [0]
+
+
+def test_hdr_file(tmp_path):
+    # 200 pixels per scanline: each plane is two literal runs (127 + 73)
+    W, H = 200, 3
+    img = np.random.default_rng(2).uniform(0, 4, (H, W, 3)).astype(np.float32)
+    img[0, 0] = [1.0, 0.5, 0.25]   # frexp(1) = 0.5 * 2^1: scale 128, E 129
+    img[0, 1] = [3.0, 0.0, 0.0]    # frexp(3) = 0.75 * 2^2: scale 64, E 130
+    img[1, 0] = [0.0, 0.0, 0.0]    # d <= 1e-32: all zero
+    p = str(tmp_path / "a.hdr")
+    prt.write_hdr(p, img)
+    b = open(p, "rb").read()
+    head = (b"#?RADIANCE\n# Made with 100% pure HDR Shop\nFORMAT=32-bit_rle_rgbe\n"
+            b"EXPOSURE=1.0000000000000\n\n-Y 3 +X 200\n")
+    assert b.startswith(head)
+    pos = len(head)
+    for y in range(H):  # scanlines in image row order (row 0 = top)
+        assert b[pos:pos + 4] == bytes([2, 2, 0, 200])
+        pos += 4
+        planes = []
+        for c in range(4):
+            plane = b""
+            for run in (127, 73):
+                assert b[pos] == run
+                plane += b[pos + 1:pos + 1 + run]
+                pos += 1 + run
+            planes.append(plane)
+        for x in range(W):
+            r, g, bb = (float(v) for v in img[y, x])
+            d = max(r, g, bb)
+            if d <= 1e-32:
+                want = (0, 0, 0, 0)
+            else:
+                m, e = math.frexp(d)
+                k = m * 256.0 / d
+                want = (int(r * k), int(g * k), int(bb * k), e + 128)
+            assert tuple(pl[x] for pl in planes) == want, (y, x)
+    assert pos == len(b)
+    # hand-computed bytes of the first row: R of pixels 0 and 1
+    assert b[len(head) + 5] == 128 and b[len(head) + 6] == 192
