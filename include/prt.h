@@ -17,7 +17,8 @@
  *   - details of the last failure on the calling thread: prt_last_error();
  *   - the caller keeps ownership of every array it passes; arrays are copied
  *     to device memory inside the call;
- *   - a scene handle is not thread-safe and owns one HIP stream; different
+ *   - a scene handle is not thread-safe and owns two HIP streams (render work,
+ *     and shadow rays traced alongside it); different
  *     handles may be used from different threads (no library globals besides
  *     the thread-local error string).
  */
@@ -173,6 +174,10 @@ typedef struct prt_stats {
     double ms_nee;                /* next-event-estimation kernels (light sampling) */
     uint64_t mixed_scene_passes;  /* batched renders: extra passes of waves whose rays
                                      belong to more than one scene */
+    double ms_traversal;          /* device time during which a closest-hit or shadow kernel
+                                     ran (shadow kernels overlap the next depth's closest-hit
+                                     kernel on a second stream, so this can be less than
+                                     ms_extend + ms_shadow) */
 } prt_stats;
 
 typedef struct prt_render_opts {
@@ -180,7 +185,7 @@ typedef struct prt_render_opts {
     int32_t samples_override;  /* > 0 replaces prt_sampling.samples */
     int32_t batch_pixels;      /* pixels per wavefront batch (0 = auto) */
     int32_t max_depth;         /* safety cap on path vertices (0 = 128) */
-    int32_t use_graph;         /* 1 = replay the depth loop as a hipGraph */
+    int32_t use_graph;         /* reserved (per-scene renders launch the depth loop directly) */
     int32_t timing;            /* 1 = per-kernel device timing into prt_stats */
     int32_t image_f64;         /* 1 = out_rgb / out_var are double arrays (device entry points) */
 } prt_render_opts;

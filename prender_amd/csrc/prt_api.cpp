@@ -158,6 +158,11 @@ void prt_scene_destroy(prt_scene* s) {
   s->d_slot_of_tri.release();
   s->grad_buf.release();
   s->tri_alb_rr.release();
+  if (s->stream2) (void)hipStreamSynchronize(s->stream2);
+  if (s->ev_nee) (void)hipEventDestroy(s->ev_nee);
+  if (s->ev_shadow) (void)hipEventDestroy(s->ev_shadow);
+  if (s->ev_shade) (void)hipEventDestroy(s->ev_shade);
+  if (s->stream2) (void)hipStreamDestroy(s->stream2);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -183,6 +188,10 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   s->device = dev;
   s->cu_count = prop.multiProcessorCount;
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_nee, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_shadow, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_shade, hipEventDisableTiming));
 
   // materials (Material::isLight, material.h:195-210)
   s->h_mats.resize(d->n_materials);
@@ -565,19 +574,38 @@ static int setup_render(prt_scene* s, const prt_render_opts* o, RenderSetup& R) 
 
 // Counters of a call, summed over its batches.
 struct CallTotals {
-  uint64_t primary_rays = 0, ext_rays = 0;
+  uint64_t primary_rays = 0;
   int64_t launches = 0;
 };
+
+// Depths enqueued between host reads of the ray count (a depth with an
+// empty queue costs only near-empty launches).
+#ifndef PRT_DEPTHS_PER_SYNC
+#define PRT_DEPTHS_PER_SYNC 4
+#endif
+// 1: the shadow rays of a depth are traced on a second stream while the next
+// depth's extension rays are; 2: the NEE of a depth runs there as well
+// (forward renders only)
+#ifndef PRT_OVERLAP_SHADOW
+#define PRT_OVERLAP_SHADOW 2
+#endif
 
 // One batch through the wavefront: generate, then extend / shade / NEE /
 // shadow per path depth until no ray is left.  Sample radiance accumulates
 // into `accum` (fixed point, 3 per camera sample of the batch).  Each
 // slab-writing kernel is followed by k_slab_index, which lists the chunks
-// of the queue it wrote for the kernels that read it.
-static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* accum, const Adjoint& A, Timer& tm,
-                     CallTotals& tot) {
+// of the queue it wrote for the kernels that read it.  The host reads the
+// ray count once per PRT_DEPTHS_PER_SYNC depths.  Without the adjoint (which
+// reads the partial sums of a sample in path order) a depth's shadow rays are
+// traced on the scene's second stream, overlapping the next depth's
+// extension rays; the next NEE, which rewrites the shadow queue, waits for
+// them, and so does the batch's end.
+static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t* accum, const Adjoint& A, Timer& tm,
+                        CallTotals& tot) {
   Workspace& w = s->ws;
   hipStream_t st = s->stream;
+  const int overlap = A.on ? 0 : PRT_OVERLAP_SHADOW;
+  hipStream_t st_sh = overlap ? s->stream2 : st;
   uint32_t* counts = (uint32_t*)w.counts.p;
   uint32_t* fetch = (uint32_t*)w.fetch.p;
   uint2* slabs = (uint2*)w.slabs.p;
@@ -591,9 +619,10 @@ static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* ac
   // children per shaded vertex: glass splits in two up to depth 2
   auto fan = [&](int depth) { return (uint32_t)(s->has_glass && depth <= 2 ? 2 : 1); };
   auto chunks = [&](int k) { return ChunkList{(const uint2*)w.desc[k].p, counts + k}; };
-  auto index = [&](int k, uint32_t n_slabs, const uint2* sl) {
+  auto index = [&](int k, uint32_t n_slabs, const uint2* sl, hipStream_t on) {
     const uint32_t max_chunks = (uint32_t)(w.desc[k].bytes / sizeof(uint2));
-    return launch_slab_index(sl, n_slabs, (uint2*)w.desc[k].p, max_chunks, counts + k, counts + 4 + k, ctr, st);
+    return launch_slab_index(sl, n_slabs, (uint2*)w.desc[k].p, max_chunks, counts + k, counts + 4 + k, ctr,
+                             k < 2 ? CTR_QUEUED : -1, on);
   };
   int fi = 0;  // traversal launches so far; each takes fresh claim counters
   auto next_fetch = [&](uint32_t** f) -> int {
@@ -604,6 +633,44 @@ static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* ac
   };
   int rc;
   uint32_t* f;
+  // NEE of the rays in q[k] and the any-hit tests of its shadow rays
+  auto nee_and_shadow = [&](int k) -> int {
+    if (overlap == 2) {
+      // both on the second stream, after the shade that wrote q[k]; the
+      // claim counters are cleared on the main stream before the hand-over
+      if ((rc = next_fetch(&f))) return rc;
+      HIPCHK(hipEventRecord(s->ev_shade, st));
+      HIPCHK(hipStreamWaitEvent(st_sh, s->ev_shade, 0));
+      if ((rc = tm.begin_side(st_sh))) return rc;
+      HIPCHK(launch_nee(P, B, q[k], chunks(k), sq, slabs + 2 * W, ctr, st_sh));
+      HIPCHK(index(2, W, slabs + 2 * W, st_sh));
+      if ((rc = tm.end_side(st_sh, T_NEE))) return rc;
+      HIPCHK(hipEventRecord(s->ev_nee, st_sh));
+      if ((rc = tm.begin_side(st_sh))) return rc;
+      HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st_sh));
+      if ((rc = tm.end_side(st_sh, T_SHADOW))) return rc;
+      HIPCHK(hipEventRecord(s->ev_shadow, st_sh));
+      return PRT_OK;
+    }
+    if (overlap) HIPCHK(hipStreamWaitEvent(st, s->ev_shadow, 0));  // the last shadow launch has read sq
+    HIPCHK(launch_nee(P, B, q[k], chunks(k), sq, slabs + 2 * W, ctr, st));
+    HIPCHK(index(2, W, slabs + 2 * W, st));
+    if ((rc = tm.mark(T_NEE))) return rc;
+    if ((rc = next_fetch(&f))) return rc;
+    if (overlap) {
+      HIPCHK(hipEventRecord(s->ev_nee, st));
+      HIPCHK(hipStreamWaitEvent(st_sh, s->ev_nee, 0));
+      if ((rc = tm.begin_side(st_sh))) return rc;
+    }
+    HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st_sh));
+    if (overlap) {
+      if ((rc = tm.end_side(st_sh, T_SHADOW))) return rc;
+      HIPCHK(hipEventRecord(s->ev_shadow, st_sh));
+    } else if ((rc = tm.mark(T_SHADOW))) {
+      return rc;
+    }
+    return PRT_OK;
+  };
   const int64_t ns = B.n_pixels * P.spp;
   const int64_t np = B.n_pixels * P.SS * P.SS;
   HIPCHK(launch_fill_u64(accum, 0ull, 3 * ns, st));
@@ -612,7 +679,7 @@ static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* ac
   HIPCHK(launch_fill_u32((uint32_t*)slabs, 0u, 1, st));
   HIPCHK(launch_fill_u32((uint32_t*)slabs + 1, (uint32_t)np, 1, st));
   HIPCHK(launch_gen_primary(P, B, q[0], st));
-  HIPCHK(index(0, 1, slabs));
+  HIPCHK(index(0, 1, slabs, st));
   if ((rc = tm.mark(T_OTHER))) return rc;
   if ((rc = next_fetch(&f))) return rc;
   HIPCHK(launch_extend(P, q[0], chunks(0), np, f, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb, st));
@@ -622,44 +689,42 @@ static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* ac
   // depth 0: every camera sample against its sub-pixel's primary hit
   HIPCHK(launch_shade(P, B, 1, q[0], chunks(0), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1], fan(0),
                       slabs + W, accum, A, ctr, st));
-  HIPCHK(index(1, W, slabs + W));
+  HIPCHK(index(1, W, slabs + W, st));
   if ((rc = tm.mark(T_SHADE))) return rc;
-  if (nee) {
-    HIPCHK(launch_nee(P, B, q[1], chunks(1), sq, slabs + 2 * W, ctr, st));
-    HIPCHK(index(2, W, slabs + 2 * W));
-    if ((rc = tm.mark(T_NEE))) return rc;
-    if ((rc = next_fetch(&f))) return rc;
-    HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st));
-    if ((rc = tm.mark(T_SHADOW))) return rc;
-  }
+  if (nee && (rc = nee_and_shadow(1))) return rc;
   int cur = 1;
   for (int depth = 1;; ++depth) {  // depth of the rays in q[cur]
-    HIPCHK(hipMemcpyAsync(w.h_counts, counts, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    const uint32_t n = w.h_counts[4 + cur];
-    if (n == 0) break;
+    if ((depth - 1) % PRT_DEPTHS_PER_SYNC == 0) {
+      HIPCHK(hipMemcpyAsync(w.h_counts, counts, 16 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      if (w.h_counts[4 + cur] == 0) break;
+    }
     const int nxt = cur ^ 1;
     if ((rc = next_fetch(&f))) return rc;
     if ((rc = tm.mark(T_OTHER))) return rc;
-    HIPCHK(launch_extend(P, q[cur], chunks(cur), n, f, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr, tb, st));
+    HIPCHK(launch_extend(P, q[cur], chunks(cur), (int64_t)tb * 256, f, (double*)w.hit_t.p, (int32_t*)w.hit_id.p, ctr,
+                         tb, st));
     if ((rc = tm.mark(T_EXTEND))) return rc;
-    tot.ext_rays += n;
     tot.launches++;
+    // the NEE of the previous depth read q[nxt] and its chunk list
+    if (overlap == 2 && nee) HIPCHK(hipStreamWaitEvent(st, s->ev_nee, 0));
     HIPCHK(launch_shade(P, B, 0, q[cur], chunks(cur), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[nxt],
                         fan(depth), slabs + nxt * W, accum, A, ctr, st));
-    HIPCHK(index(nxt, W, slabs + nxt * W));
+    HIPCHK(index(nxt, W, slabs + nxt * W, st));
     if ((rc = tm.mark(T_SHADE))) return rc;
-    if (nee) {
-      HIPCHK(launch_nee(P, B, q[nxt], chunks(nxt), sq, slabs + 2 * W, ctr, st));
-      HIPCHK(index(2, W, slabs + 2 * W));
-      if ((rc = tm.mark(T_NEE))) return rc;
-      if ((rc = next_fetch(&f))) return rc;
-      HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st));
-      if ((rc = tm.mark(T_SHADOW))) return rc;
-    }
+    if (nee && (rc = nee_and_shadow(nxt))) return rc;
     cur = nxt;
   }
+  if (overlap) HIPCHK(hipStreamWaitEvent(st, s->ev_shadow, 0));
   return PRT_OK;
+}
+
+static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* accum, const Adjoint& A, Timer& tm,
+                     CallTotals& tot) {
+  const int rc = run_batch_on(s, P, B, accum, A, tm, tot);
+  // a failed batch may have left shadow work queued on the second stream
+  if (rc != PRT_OK) (void)hipStreamSynchronize(s->stream2);
+  return rc;
 }
 
 static int finish_stats(prt_scene* s, const RenderSetup& R, Timer& tm, const CallTotals& tot, prt_stats* stats,
@@ -669,19 +734,20 @@ static int finish_stats(prt_scene* s, const RenderSetup& R, Timer& tm, const Cal
   HIPCHK(hipStreamSynchronize(st));
   if (c[CTR_DROP] != 0) return fail(PRT_E_INVALID, "ray queue overflow (" + std::to_string(c[CTR_DROP]) + " rays)");
   // every queued ray must have been traced exactly once
-  if (c[CTR_TRACED] != tot.primary_rays + tot.ext_rays || c[CTR_STRACED] != c[CTR_SHADOW])
+  if (c[CTR_TRACED] != c[CTR_QUEUED] || c[CTR_STRACED] != c[CTR_SHADOW])
     return fail(PRT_E_INTERNAL, "traversal finished " + std::to_string(c[CTR_TRACED]) + " of " +
-                                    std::to_string(tot.primary_rays + tot.ext_rays) + " rays and " +
+                                    std::to_string(c[CTR_QUEUED]) + " rays and " +
                                     std::to_string(c[CTR_STRACED]) + " of " + std::to_string(c[CTR_SHADOW]) +
                                     " shadow rays");
   double cat[T_NCAT] = {0, 0, 0, 0, 0, 0};
-  tm.collect(cat);
+  double trav = 0.0;
+  const double wall = tm.collect(cat, &trav, 1u << T_EXTEND | 1u << T_EXTEND0 | 1u << T_SHADOW);
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     fill_stats(stats, c);
     stats->camera_samples = (uint64_t)R.n_pix * R.P.spp;
     stats->primary_rays = tot.primary_rays;
-    stats->extension_rays = tot.ext_rays;
+    stats->extension_rays = c[CTR_QUEUED] - tot.primary_rays;
     stats->shadow_rays = c[CTR_SHADOW];
     stats->ms_extend = cat[T_EXTEND] + cat[T_EXTEND0];
     stats->ms_shade = cat[T_SHADE];
@@ -689,11 +755,12 @@ static int finish_stats(prt_scene* s, const RenderSetup& R, Timer& tm, const Cal
     stats->ms_other = cat[T_OTHER];
     stats->ms_extend_primary = cat[T_EXTEND0];
     stats->ms_nee = cat[T_NEE];
-    stats->ms_total = cat[0] + cat[1] + cat[2] + cat[3] + cat[4] + cat[5];
+    stats->ms_total = wall;
+    stats->ms_traversal = trav;
     stats->launches_extend = tot.launches;
     // per ray query: 128 B per node record, 48 B per triangle record, 48 B
     // of ray in + hit out
-    stats->bytes_extend = 128 * c[CTR_NODES] + 48 * c[CTR_TRIS] + 48 * (tot.primary_rays + tot.ext_rays);
+    stats->bytes_extend = 128 * c[CTR_NODES] + 48 * c[CTR_TRIS] + 48 * c[CTR_QUEUED];
     stats->bytes_shadow = 128 * c[CTR_SNODES] + 48 * c[CTR_STRIS] + 48 * c[CTR_SHADOW];
     stats->bytes_algorithmic = stats->bytes_extend + stats->bytes_shadow;
   }

@@ -398,7 +398,8 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   float ms = 0;
   if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
   double cat[T_NCAT] = {0, 0, 0, 0, 0, 0};
-  tm.collect(cat);
+  double trav = 0.0;
+  tm.collect(cat, &trav, 1u << T_EXTEND | 1u << T_SHADOW);
   cleanup();
   if (he != hipSuccess) return fail(PRT_E_HIP, std::string("batch completion: ") + hipGetErrorString(he));
   if (status == 2) return fail(PRT_E_INVALID, "batched render stalled: tiles in flight without rays");
@@ -423,6 +424,7 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
     stats->ms_shade = cat[T_SHADE];
     stats->ms_nee = cat[T_NEE];
     stats->ms_shadow = cat[T_SHADOW];
+    stats->ms_traversal = trav;
     stats->ms_other = cat[T_OTHER];
     stats->launches_extend = (int64_t)final_ctl.iter;
     stats->bytes_extend = 128 * ctrh[CTR_NODES] + 48 * ctrh[CTR_TRIS] + 48 * (final_ctl.total_prim + final_ctl.total_cont);

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/prt.h"
@@ -101,18 +102,60 @@ struct Timer {
     marks.push_back({e, cat});
     return PRT_OK;
   }
-  // after the stream has completed
-  void collect(double* cat_ms) {
-    if (!on) return;
-    hipEvent_t prev = start;
-    for (auto& m : marks) {
+  // a span on another stream, overlapping the marks of the main stream
+  std::vector<std::pair<hipEvent_t, int>> side;  // begin (category -1) and end events
+  int begin_side(hipStream_t st) { return side_event(st, -1); }
+  int end_side(hipStream_t st, int cat) { return side_event(st, cat); }
+  int side_event(hipStream_t st, int cat) {
+    if (!on) return PRT_OK;
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(e, st));
+    side.push_back({e, cat});
+    return PRT_OK;
+  }
+  // After both streams have completed: adds each category's device time to
+  // cat_ms, sets *union_ms to the time during which a span of a category in
+  // union_mask (bit per category) ran on either stream, and returns the main
+  // stream's elapsed time (side spans overlap it).
+  double collect(double* cat_ms, double* union_ms = nullptr, unsigned union_mask = 0) {
+    if (union_ms) *union_ms = 0.0;
+    if (!on) return 0.0;
+    auto at = [&](hipEvent_t e) {
       float ms = 0;
-      if (hipEventElapsedTime(&ms, prev, m.first) == hipSuccess) cat_ms[m.second] += ms;
-      prev = m.first;
+      return hipEventElapsedTime(&ms, start, e) == hipSuccess ? (double)ms : -1.0;
+    };
+    std::vector<std::pair<double, double>> iv;  // spans of the union categories
+    double wall = 0.0, t_prev = 0.0;
+    for (auto& m : marks) {
+      const double t = at(m.first);
+      if (t < 0) continue;
+      cat_ms[m.second] += t - t_prev;
+      if (union_mask >> m.second & 1u) iv.push_back({t_prev, t});
+      wall = t;
+      t_prev = t;
+    }
+    for (size_t i = 0; i + 1 < side.size(); i += 2) {
+      const double t0 = at(side[i].first), t1 = at(side[i + 1].first);
+      if (t0 < 0 || t1 < 0) continue;
+      cat_ms[side[i + 1].second] += t1 - t0;
+      if (union_mask >> side[i + 1].second & 1u) iv.push_back({t0, t1});
+    }
+    if (union_ms) {
+      std::sort(iv.begin(), iv.end());
+      double end = -1.0;
+      for (auto& v : iv) {
+        const double b = std::max(v.first, end);
+        if (v.second > b) *union_ms += v.second - b;
+        end = std::max(end, v.second);
+      }
     }
     for (auto& m : marks) (void)hipEventDestroy(m.first);
+    for (auto& m : side) (void)hipEventDestroy(m.first);
     (void)hipEventDestroy(start);
     marks.clear();
+    side.clear();
+    return wall;
   }
 };
 enum { T_EXTEND = 0, T_SHADE = 1, T_SHADOW = 2, T_OTHER = 3, T_EXTEND0 = 4, T_NEE = 5, T_NCAT = 6 };
@@ -127,6 +170,8 @@ struct prt_scene {
   int device = 0;
   int cu_count = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;              // shadow traversal, overlapping the next depth
+  hipEvent_t ev_nee = nullptr, ev_shadow = nullptr, ev_shade = nullptr;
   Params P{};  // scene + sampling part; per-call fields are patched in
   prt::DevBuf nodes, tris, tri_nrm, tri_alb, ents, mats, analytic, lights, ovf;
   std::vector<DevEntity> h_ents;

@@ -20,6 +20,7 @@ enum {
   CTR_REPLAY_MISMATCH = 10,  // adjoint: samples whose replayed radiance differs from the primal
   CTR_TRACED = 11,   // closest-hit queries finished by k_extend (must equal the rays queued)
   CTR_STRACED = 12,  // shadow rays finished by k_shadow (must equal CTR_SHADOW)
+  CTR_QUEUED = 13,   // rays of the ray queues listed for k_extend (primary + extension)
   CTR_COUNT = 14
 };
 
@@ -105,7 +106,7 @@ hipError_t launch_trace(const Params& P, const prt_ray_dev* rays, int64_t n, prt
                         int max_blocks, hipStream_t s);
 hipError_t launch_gen_primary(const Params& P, const Batch& B, const RayQ& q, hipStream_t s);
 hipError_t launch_slab_index(const uint2* slabs, uint32_t n_slabs, uint2* desc, uint32_t max_chunks, uint32_t* n_chunks,
-                             uint32_t* n_entries, uint64_t* ctr, hipStream_t s);
+                             uint32_t* n_entries, uint64_t* ctr, int ctr_entries, hipStream_t s);
 hipError_t launch_extend(const Params& P, const RayQ& q, const ChunkList& cl, int64_t n_host, uint32_t* fetch,
                          double* hit_t, int32_t* hit_id, uint64_t* ctr, int max_blocks, hipStream_t s);
 hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ& qin, const ChunkList& cl,

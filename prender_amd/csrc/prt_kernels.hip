@@ -321,11 +321,13 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
 // Chunk descriptors of a slab queue (one block of 1024 threads): slab s holds
 // slabs[s].y entries from queue index slabs[s].x on, ceil(that / 64) chunks;
 // chunks are numbered slab by slab.  Also stores the number of chunks and of
-// entries.  More chunks than max_chunks (never, with the host's sizing) are
+// entries (and adds the entries to ctr[ctr_entries] unless that is < 0).
+// More chunks than max_chunks (never, with the host's sizing) are
 // dropped and counted.
 __global__ __launch_bounds__(1024) void k_slab_index(const uint2* __restrict__ slabs, uint32_t n_slabs,
                                                      uint2* __restrict__ desc, uint32_t max_chunks,
-                                                     uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr) {
+                                                     uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr,
+                                                     int ctr_entries) {
   __shared__ uint32_t s_pref[PRT_SLAB_INDEX_MAX + 1];
   __shared__ uint32_t s_wch[16], s_went[16];
   const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
@@ -383,6 +385,7 @@ __global__ __launch_bounds__(1024) void k_slab_index(const uint2* __restrict__ s
   if (tid == 0) {
     *n_chunks = total;
     *n_entries = tot_ent;
+    if (ctr_entries >= 0 && tot_ent) atomicAdd((unsigned long long*)&ctr[ctr_entries], (unsigned long long)tot_ent);
     if (tot_ch > max_chunks)
       atomicAdd((unsigned long long*)&ctr[CTR_DROP], (unsigned long long)(tot_ch - max_chunks) * 64ull);
   }
@@ -489,10 +492,10 @@ hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const Chun
   return hipGetLastError();
 }
 hipError_t launch_slab_index(const uint2* slabs, uint32_t n_slabs, uint2* desc, uint32_t max_chunks, uint32_t* n_chunks,
-                             uint32_t* n_entries, uint64_t* ctr, hipStream_t s) {
+                             uint32_t* n_entries, uint64_t* ctr, int ctr_entries, hipStream_t s) {
   if (n_slabs == 0 || n_slabs > PRT_SLAB_INDEX_MAX) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_slab_index, dim3(1), dim3(1024), 0, s, slabs, n_slabs, desc, max_chunks, n_chunks, n_entries,
-                     ctr);
+                     ctr, ctr_entries);
   return hipGetLastError();
 }
 hipError_t launch_resolve(const Params& P, const Batch& B, const uint64_t* accum, void* rgb, void* var, int f64,

@@ -86,7 +86,8 @@ class PrtStats(C.Structure):
                 ("launches_extend", C.c_int64), ("shadow_node_visits", C.c_uint64),
                 ("shadow_tri_tests", C.c_uint64), ("bytes_extend", C.c_uint64),
                 ("bytes_shadow", C.c_uint64), ("ms_extend_primary", C.c_double),
-                ("ms_nee", C.c_double), ("mixed_scene_passes", C.c_uint64)]
+                ("ms_nee", C.c_double), ("mixed_scene_passes", C.c_uint64),
+                ("ms_traversal", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
