@@ -3,7 +3,8 @@
 // choice of its tile / in-flight / stack / grid sizes (prt_tune_batch).
 //
 // The device side is prt_stream.hip; every iteration is refill, gen, extend,
-// shade, nee, shadow, resolve on the lead scene's stream, two iterations per
+// shade, nee, shadow, resolve on the lead scene's stream (nee and shadow on
+// its second stream when PRT_BATCH_OVERLAP is set), two iterations per
 // captured hipGraph.  The host only replays the graph and, once per group of
 // replays, reads the control block to learn whether the work queue is drained.
 #include <hip/hip_runtime.h>
@@ -16,6 +17,12 @@
 
 #include "prt_internal.h"
 #include "prt_stream.h"
+
+// 1: NEE and shadow rays of an iteration run on a second stream, alongside
+// the next iteration's camera-ray generation and closest-hit rays
+#ifndef PRT_BATCH_OVERLAP
+#define PRT_BATCH_OVERLAP 1
+#endif
 
 namespace prt {
 
@@ -179,7 +186,7 @@ int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, const StreamG
   for (int p = 0; p < 2; ++p)
     if ((rc = dev_alloc(w.desc[p], (size_t)g.max_desc * sizeof(ChunkDesc)))) return rc;
   if ((rc = dev_alloc(w.sh_desc, (size_t)g.max_sh_desc * sizeof(ChunkDesc)))) return rc;
-  if ((rc = dev_alloc(w.fetch, (size_t)2 * PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE * sizeof(uint32_t)))) return rc;
+  if ((rc = dev_alloc(w.fetch, (size_t)3 * PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE * sizeof(uint32_t)))) return rc;
   const int64_t ovf_threads = (int64_t)std::max(1, lead->cu_count) * kMaxTravBlocksPerCu * 256;
   if ((rc = dev_alloc(w.ovf, (size_t)ovf_threads * 2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * sizeof(uint32_t))))
     return rc;
@@ -289,27 +296,41 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   const int trav_blocks = cu * c.trav_blocks_per_cu;
   const int work_blocks = cu * 8;
 
-  hipEvent_t e0, e1, ev[2];
+  // NEE and shadow rays on the lead scene's second stream (prt_stream.h)
+  const bool overlap = PRT_BATCH_OVERLAP && nee;
+  hipStream_t s2 = overlap ? lead->stream2 : s;
+  hipEvent_t e0, e1, ev[2], ev_shade, ev_sh[2];
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventCreate(&ev[0]));
   HIPCHK(hipEventCreate(&ev[1]));
+  HIPCHK(hipEventCreateWithFlags(&ev_shade, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ev_sh[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ev_sh[1], hipEventDisableTiming));
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   auto cleanup = [&]() {
+    if (overlap) (void)hipStreamSynchronize(s2);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
+    (void)hipEventDestroy(ev_shade);
+    (void)hipEventDestroy(ev_sh[0]);
+    (void)hipEventDestroy(ev_sh[1]);
   };
   Timer tm;
+  int sh_pending = -1;  // parity of the last shadow launch not yet waited for on s
   // one iteration; with timing on, an event after each kernel (plain launches)
   auto iteration = [&](int cur) -> hipError_t {
     const int nxt = cur ^ 1;
     hipError_t e;
     auto mark = [&](int cat) { return tm.mark(cat) == PRT_OK ? hipSuccess : hipErrorUnknown; };
+    auto side = [&](int cat) {
+      return (cat < 0 ? tm.begin_side(s2) : tm.end_side(s2, cat)) == PRT_OK ? hipSuccess : hipErrorUnknown;
+    };
     if ((e = launch_stream_refill(S, cur, s)) != hipSuccess) return e;
     if ((e = launch_stream_gen(S, q[cur], cur, work_blocks, s)) != hipSuccess) return e;
     if ((e = mark(T_OTHER)) != hipSuccess) return e;
@@ -319,15 +340,36 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
     if ((e = launch_stream_shade(S, q[cur], hit_t, hit_id, q[nxt], nxt, s)) != hipSuccess) return e;
     if ((e = launch_stream_index(S, nxt, s)) != hipSuccess) return e;
     if ((e = mark(T_SHADE)) != hipSuccess) return e;
-    if (nee) {
+    if (nee && overlap) {
+      if ((e = hipEventRecord(ev_shade, s)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(s2, ev_shade, 0)) != hipSuccess) return e;
+      if ((e = side(-1)) != hipSuccess) return e;
+      if ((e = launch_stream_nee(S, q[nxt], nxt, sq, s2)) != hipSuccess) return e;
+      if ((e = launch_stream_index(S, 2, s2)) != hipSuccess) return e;
+      if ((e = side(T_NEE)) != hipSuccess) return e;
+      if ((e = side(-1)) != hipSuccess) return e;
+      if ((e = launch_stream_shadow(S, sq, cur, c.stack_lds, trav_blocks, s2)) != hipSuccess) return e;
+      if ((e = side(T_SHADOW)) != hipSuccess) return e;
+      if ((e = hipEventRecord(ev_sh[cur], s2)) != hipSuccess) return e;
+      // resolve waits for the previous iteration's shadow launch only
+      if (sh_pending == nxt && (e = hipStreamWaitEvent(s, ev_sh[nxt], 0)) != hipSuccess) return e;
+      sh_pending = cur;
+    } else if (nee) {
       if ((e = launch_stream_nee(S, q[nxt], nxt, sq, s)) != hipSuccess) return e;
       if ((e = launch_stream_index(S, 2, s)) != hipSuccess) return e;
       if ((e = mark(T_NEE)) != hipSuccess) return e;
-      if ((e = launch_stream_shadow(S, sq, c.stack_lds, trav_blocks, s)) != hipSuccess) return e;
+      if ((e = launch_stream_shadow(S, sq, cur, c.stack_lds, trav_blocks, s)) != hipSuccess) return e;
       if ((e = mark(T_SHADOW)) != hipSuccess) return e;
     }
     if ((e = launch_stream_resolve(S, cu * 2, s)) != hipSuccess) return e;
     return mark(T_OTHER);
+  };
+  // s waits for the last shadow launch (end of a captured graph, end of the render)
+  auto join = [&]() -> hipError_t {
+    if (sh_pending < 0) return hipSuccess;
+    const hipError_t e = hipStreamWaitEvent(s, ev_sh[sh_pending], 0);
+    sh_pending = -1;
+    return e;
   };
 
   hipError_t he = hipEventRecord(e0, s);
@@ -347,6 +389,7 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
     if (he == hipSuccess) {
       hipError_t e_it = iteration(0);
       if (e_it == hipSuccess) e_it = iteration(1);
+      if (e_it == hipSuccess) e_it = join();  // the capture ends on s
       he = hipStreamEndCapture(s, &graph);
       if (he == hipSuccess) he = e_it;
     }
@@ -389,7 +432,8 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
       return fail(PRT_E_INVALID, "batched render did not finish within the iteration limit");
     }
   }
-  he = hipEventRecord(e1, s);
+  he = join();
+  if (he == hipSuccess) he = hipEventRecord(e1, s);
   uint64_t ctrh[CTR_COUNT];
   StreamCtl final_ctl;
   if (he == hipSuccess) he = hipMemcpyAsync(ctrh, S.ctr, sizeof(ctrh), hipMemcpyDeviceToHost, s);

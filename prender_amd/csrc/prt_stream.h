@@ -4,7 +4,8 @@
 // is a run of consecutive pixels of one scene with all their camera samples
 // (at most T = 1 << tile_shift samples).  A fixed pool of tile slots holds
 // the tiles in flight: a slot owns T fixed-point sample accumulators and a
-// count of the tile's live rays (queued path segments and shadow rays).
+// count of the tile's live rays (queued path segments; shadow rays are not
+// counted, see below).
 //
 // Queues are slab queues, as in the per-scene renderer: the shade and NEE
 // kernels run PRT_SLAB_WAVES waves, and wave w writes its outputs into its
@@ -14,10 +15,11 @@
 // 64-entry chunks of one scene each: every wave of a consuming kernel holds
 // rays of one scene and reads that scene's parameters with scalar loads.
 //
-// Every iteration runs, on one stream:
+// Every iteration runs:
 //   refill  (1 block)  assign free slots to the next tiles of the work queue;
 //                      append the chunks of their camera rays to the current
-//                      queue's chunk list; clear the claim counters
+//                      queue's chunk list; clear the claim counters of the
+//                      extend launch and of this parity's shadow launch
 //   gen                unique sub-pixel camera rays of the new tiles, in the
 //                      primary region after the slabs
 //   extend             closest hit of every chunk of the current queue
@@ -30,9 +32,18 @@
 //   resolve            tiles with no live ray left: fold samples into their
 //                      scene's image, clear the slot, free it
 // so the GPU always holds rays from as many tiles as there are slots, of
-// whatever scenes those tiles belong to.  Two iterations (one per queue
-// parity) are captured as one hipGraph and replayed until refill reports that
-// the work queue is drained and no tile is in flight.
+// whatever scenes those tiles belong to.  The nee / index / shadow kernels
+// of iteration i may run on a second stream, after iteration i's shade and
+// alongside iteration i+1's refill / gen / extend; resolve of iteration i
+// then waits for the shadow kernel of iteration i-1 only.  That is enough:
+// a tile whose live count is zero after shade i has no ray in the next queue,
+// so NEE i gave it no shadow ray, and its last shadow rays came from NEE i-1
+// or earlier.  Shadow rays therefore stay out of the live count, and the
+// shadow claim counters have one region per queue parity (refill of
+// iteration i clears parity i's region, whose previous user, shadow i-2,
+// has finished).  Two iterations (one per queue parity) are captured as one
+// hipGraph and replayed until refill reports that the work queue is drained
+// and no tile is in flight.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -122,7 +133,8 @@ struct Stream {
   uint32_t* sh_nruns;       // [PRT_SLAB_WAVES]
   ChunkDesc* desc[2];       // [max_desc] chunk list of queue p
   ChunkDesc* sh_desc;       // [max_sh_desc] chunk list of the shadow queue
-  uint32_t* fetch;          // [2][PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE] claim counters of extend, shadow
+  uint32_t* fetch;          // [3][PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE] claim counters of extend, and of
+                            // the shadow launches of queue parity 0 and 1
   StreamCtl* ctl;
   uint64_t* accum;          // [n_slots << tile_shift][3] fixed point
   uint64_t* ctr;            // [CTR_COUNT]
@@ -145,5 +157,6 @@ hipError_t launch_stream_shade(const Stream& st, const RayQ& qin, const double* 
                                const RayQ& qout, int nxt, hipStream_t s);
 hipError_t launch_stream_index(const Stream& st, int which, hipStream_t s);  // which: 0 / 1 ray queue, 2 shadow
 hipError_t launch_stream_nee(const Stream& st, const RayQ& q, int nxt, const ShadowQ& sq, hipStream_t s);
-hipError_t launch_stream_shadow(const Stream& st, const ShadowQ& sq, int stack_lds, int blocks, hipStream_t s);
+hipError_t launch_stream_shadow(const Stream& st, const ShadowQ& sq, int par, int stack_lds, int blocks,
+                                hipStream_t s);
 hipError_t launch_stream_resolve(const Stream& st, int blocks, hipStream_t s);

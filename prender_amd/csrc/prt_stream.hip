@@ -163,7 +163,11 @@ __global__ __launch_bounds__(256) void k_stream_refill(Stream st, int cur) {
     s_hdr[2] = next;
     s_hdr[3] = c->cont_chunks[cur];
   }
-  for (uint32_t k = tid; k < 2u * PRT_FETCH_COUNTERS; k += 256) st.fetch[(size_t)k * PRT_FETCH_STRIDE] = 0u;
+  for (uint32_t k = tid; k < 2u * PRT_FETCH_COUNTERS; k += 256) {
+    // the extend launch's region, then this parity's shadow region
+    const uint32_t region = k < PRT_FETCH_COUNTERS ? 0u : 1u + (uint32_t)cur;
+    st.fetch[((size_t)region * PRT_FETCH_COUNTERS + k % PRT_FETCH_COUNTERS) * PRT_FETCH_STRIDE] = 0u;
+  }
   __syncthreads();
   const uint32_t n_want = s_hdr[0], free_n = s_hdr[1], next = s_hdr[2], cont_chunks = s_hdr[3];
 
@@ -516,7 +520,6 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
       }
     }
     emitted += n_sh;
-    live_update(st, n_sh > 0 ? (int)(slot >> shift) : -1, (int)n_sh);
   }
   log.close(min(s_cnt[wib], st.slab_sh));
   const uint64_t s_sh = wave_sum_u32(emitted), s_drop = wave_sum_u32(dropped);
@@ -531,36 +534,34 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
 // Any-hit tests of the shadow queue's chunks; unoccluded rays add their
 // contribution.
 template <int NL>
-__global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq) {
+__global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq, int par) {
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL);
-  const uint32_t shift = (uint32_t)st.tile_shift;
-  const ClaimRange cr = claim_range(st.fetch + (size_t)PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE, st.ctl->sh_chunks);
+  const ClaimRange cr =
+      claim_range(st.fetch + (size_t)(1 + par) * PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE, st.ctl->sh_chunks);
   TravStats ts{0, 0, 0};
   uint32_t c, c1;
   while (claim(cr, FETCH_CHUNKS, c, c1)) {
     for (; c < c1; ++c) {
       const ChunkDesc dc = chunk_at(st.sh_desc, c);
       const Params& P = scene_params(st.scenes, dc.scene);
-      const bool valid = lane_id() < dc.cnt;
-      uint32_t slot = 0;
-      if (valid) {
+      if (lane_id() < dc.cnt) {
         const uint32_t i = dc.base + lane_id();
-        slot = sq.slot[i];
+        const uint32_t slot = sq.slot[i];
         if (!occluded(P, mk(sq.ox[i], sq.oy[i], sq.oz[i]), mk(sq.dx[i], sq.dy[i], sq.dz[i]), sq.tmax[i], S, ts)) {
           accum_add(&st.accum[3 * (size_t)slot + 0], sq.cr[i]);
           accum_add(&st.accum[3 * (size_t)slot + 1], sq.cg[i]);
           accum_add(&st.accum[3 * (size_t)slot + 2], sq.cb[i]);
         }
       }
-      live_update(st, valid ? (int)(slot >> shift) : -1, -1);
     }
   }
   flush_stats(st.ctr, CTR_SNODES, ts);
 }
 
 // Finished tiles (busy slots whose live count is zero after this iteration's
-// kernels): fold every pixel's samples into its scene's image, clear the
-// slot's accumulators and return the slot to the free list.
+// shade; launched after the shadow kernels of the earlier iterations): fold
+// every pixel's samples into its scene's image, clear the slot's
+// accumulators and return the slot to the free list.
 __global__ __launch_bounds__(256) void k_stream_resolve(Stream st) {
   const uint32_t shift = (uint32_t)st.tile_shift;
   for (uint32_t ts = blockIdx.x; ts < (uint32_t)st.n_slots; ts += gridDim.x) {
@@ -627,13 +628,15 @@ hipError_t launch_stream_nee(const Stream& st, const RayQ& q, int nxt, const Sha
   hipLaunchKernelGGL(k_stream_nee, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, st, q, nxt, sq);
   return hipGetLastError();
 }
-hipError_t launch_stream_shadow(const Stream& st, const ShadowQ& sq, int stack_lds, int blocks, hipStream_t s) {
+hipError_t launch_stream_shadow(const Stream& st, const ShadowQ& sq, int par, int stack_lds, int blocks,
+                                hipStream_t s) {
+  if (par != 0 && par != 1) return hipErrorInvalidValue;
   if (stack_lds <= 8)
-    hipLaunchKernelGGL(k_stream_shadow<8>, dim3(blocks), dim3(256), 0, s, st, sq);
+    hipLaunchKernelGGL(k_stream_shadow<8>, dim3(blocks), dim3(256), 0, s, st, sq, par);
   else if (stack_lds <= 16)
-    hipLaunchKernelGGL(k_stream_shadow<16>, dim3(blocks), dim3(256), 0, s, st, sq);
+    hipLaunchKernelGGL(k_stream_shadow<16>, dim3(blocks), dim3(256), 0, s, st, sq, par);
   else
-    hipLaunchKernelGGL(k_stream_shadow<32>, dim3(blocks), dim3(256), 0, s, st, sq);
+    hipLaunchKernelGGL(k_stream_shadow<32>, dim3(blocks), dim3(256), 0, s, st, sq, par);
   return hipGetLastError();
 }
 hipError_t launch_stream_resolve(const Stream& st, int blocks, hipStream_t s) {
