@@ -159,7 +159,8 @@ void prt_scene_destroy(prt_scene* s) {
   s->grad_buf.release();
   s->tri_alb_rr.release();
   if (s->stream2) (void)hipStreamSynchronize(s->stream2);
-  if (s->ev_nee) (void)hipEventDestroy(s->ev_nee);
+  for (hipEvent_t e : s->ev_nee)
+    if (e) (void)hipEventDestroy(e);
   if (s->ev_shadow) (void)hipEventDestroy(s->ev_shadow);
   if (s->ev_shade) (void)hipEventDestroy(s->ev_shade);
   if (s->stream2) (void)hipStreamDestroy(s->stream2);
@@ -189,7 +190,8 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   s->cu_count = prop.multiProcessorCount;
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&s->ev_nee, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_nee[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_nee[1], hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&s->ev_shadow, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&s->ev_shade, hipEventDisableTiming));
 
@@ -645,7 +647,7 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
       HIPCHK(launch_nee(P, B, q[k], chunks(k), sq, slabs + 2 * W, ctr, st_sh));
       HIPCHK(index(2, W, slabs + 2 * W, st_sh));
       if ((rc = tm.end_side(st_sh, T_NEE))) return rc;
-      HIPCHK(hipEventRecord(s->ev_nee, st_sh));
+      HIPCHK(hipEventRecord(s->ev_nee[k], st_sh));
       if ((rc = tm.begin_side(st_sh))) return rc;
       HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st_sh));
       if ((rc = tm.end_side(st_sh, T_SHADOW))) return rc;
@@ -658,8 +660,8 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
     if ((rc = tm.mark(T_NEE))) return rc;
     if ((rc = next_fetch(&f))) return rc;
     if (overlap) {
-      HIPCHK(hipEventRecord(s->ev_nee, st));
-      HIPCHK(hipStreamWaitEvent(st_sh, s->ev_nee, 0));
+      HIPCHK(hipEventRecord(s->ev_shade, st));
+      HIPCHK(hipStreamWaitEvent(st_sh, s->ev_shade, 0));
       if ((rc = tm.begin_side(st_sh))) return rc;
     }
     HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st_sh));
@@ -706,8 +708,8 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
                          tb, st));
     if ((rc = tm.mark(T_EXTEND))) return rc;
     tot.launches++;
-    // the NEE of the previous depth read q[nxt] and its chunk list
-    if (overlap == 2 && nee) HIPCHK(hipStreamWaitEvent(st, s->ev_nee, 0));
+    // the NEE two depths back read q[nxt] and its chunk list
+    if (overlap == 2 && nee) HIPCHK(hipStreamWaitEvent(st, s->ev_nee[nxt], 0));
     HIPCHK(launch_shade(P, B, 0, q[cur], chunks(cur), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[nxt],
                         fan(depth), slabs + nxt * W, accum, A, ctr, st));
     HIPCHK(index(nxt, W, slabs + nxt * W, st));

@@ -171,7 +171,8 @@ struct prt_scene {
   int cu_count = 0;
   hipStream_t stream = nullptr;
   hipStream_t stream2 = nullptr;              // shadow traversal, overlapping the next depth
-  hipEvent_t ev_nee = nullptr, ev_shadow = nullptr, ev_shade = nullptr;
+  hipEvent_t ev_nee[2] = {nullptr, nullptr};  // last NEE that read ray queue 0 / 1
+  hipEvent_t ev_shadow = nullptr, ev_shade = nullptr;
   Params P{};  // scene + sampling part; per-call fields are patched in
   prt::DevBuf nodes, tris, tri_nrm, tri_alb, ents, mats, analytic, lights, ovf;
   std::vector<DevEntity> h_ents;
