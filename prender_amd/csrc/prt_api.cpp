@@ -23,6 +23,11 @@
 #include "prt_internal.h"
 #include "prt_kernels.h"
 
+// 256-thread blocks per CU of the traversal grids
+#ifndef PRT_TRAV_BLOCKS_PER_CU
+#define PRT_TRAV_BLOCKS_PER_CU 8
+#endif
+
 namespace prt {
 
 thread_local std::string g_err;
@@ -340,7 +345,7 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   if ((rc = dev_upload(s->lights, s->h_lights, st))) return rc;
   if (any_albedo && (rc = dev_upload(s->d_slot_of_tri, s->slot_of_tri, st))) return rc;
   // traversal stack overflow: one area per thread of the largest grid
-  s->max_trav_blocks = std::max(1, s->cu_count) * 8;
+  s->max_trav_blocks = std::max(1, s->cu_count) * PRT_TRAV_BLOCKS_PER_CU;
   const int64_t ovf_threads = (int64_t)s->max_trav_blocks * 256;
   if ((rc = dev_alloc(s->ovf, (size_t)ovf_threads * 2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * sizeof(uint32_t))))
     return rc;
