@@ -178,6 +178,8 @@ typedef struct prt_stats {
                                      ran (shadow kernels overlap the next depth's closest-hit
                                      kernel on a second stream, so this can be less than
                                      ms_extend + ms_shadow) */
+    uint64_t mlt_proposals;       /* Metropolis renders: proposals evaluated */
+    uint64_t mlt_accepts;         /* Metropolis renders: proposals accepted */
 } prt_stats;
 
 typedef struct prt_render_opts {
@@ -222,6 +224,40 @@ int prt_render(prt_scene* scene, const prt_render_opts* opts,
 /* Same, writing device buffers. */
 int prt_render_device(prt_scene* scene, const prt_render_opts* opts,
                       float* d_rgb, float* d_var, prt_stats* stats);
+
+/* ---- Metropolis light transport (replaces Render::RunMLT, render.h:360-780;
+ *      KelemenMLT, mlt.h:70-170; generate_new_path, render.h:195-356) ------- */
+/* Primary-sample-space MLT (Kelemen et al. 2002) over the same radiance():
+ * seed paths with fresh primary samples estimate the normalisation b (mean
+ * luminance) and give each Markov chain its start, drawn in proportion to
+ * luminance; then every chain proposes a large step (all primary samples
+ * fresh, probability p_large) or a small step (every primary sample moved by
+ * mlt.h:42-54's Mutate0), accepts with min(1, L_new/L_old) and splats both
+ * paths with Kelemen's weights.  Camera: a pixel from two primary samples, a
+ * 2x2 sub-pixel choice and the tent filter (render.h:262-300).  Primary
+ * samples drive Russian roulette, the BSDF choice and direction of the first
+ * sample_depths vertices; light sampling takes fresh numbers per proposal, as
+ * the reference's Next01() does (directLight_radiance.h:343-359,
+ * light.h:146-151).  The reference runs `samples` chains one after another,
+ * each of W*H*mutation steps; here `chains` chains run side by side, each a
+ * wavefront batch slot, for samples*W*H*mutation/chains steps. */
+typedef struct prt_mlt_opts {
+    uint64_t seed;               /* RNG key */
+    double mutation;             /* steps per pixel per chain of the reference (SceneEnv::mutation; <= 0: 32) */
+    double pre_sample;           /* seed paths per pixel per chain (SceneEnv::pre_sample; <= 0: 0.4) */
+    double p_large;              /* large-step probability (render.h:551; <= 0: 0.5) */
+    int64_t mutations_per_chain; /* > 0 overrides samples*W*H*mutation / chains */
+    int32_t samples_override;    /* > 0 replaces prt_sampling.samples (the reference's chain count) */
+    int32_t chains;              /* chains run side by side (0: auto, at most 2^20) */
+    int32_t sample_depths;       /* path vertices whose random numbers are primary samples (0: 8, <= 32) */
+    int32_t max_depth;           /* safety cap on path vertices (0 = 128) */
+    int32_t image_f64;           /* 1 = d_rgb is a double array (device entry point) */
+    int32_t pad_;
+} prt_mlt_opts;
+/* out_rgb: W*H*3 float in the reference's row order, already scaled by the
+ * MLT sensor response (0.25 * SENSOR_RESPONSE, scene_env.h:1945-1956). */
+int prt_render_mlt(prt_scene* scene, const prt_mlt_opts* opts, float* out_rgb, prt_stats* stats);
+int prt_render_mlt_device(prt_scene* scene, const prt_mlt_opts* opts, float* d_rgb, prt_stats* stats);
 
 /* ---- batched rendering of many scenes ------------------------------------- */
 /* No reference counterpart: the reference renders one scene per process

@@ -56,6 +56,10 @@ def lib():
                                   C.POINTER(C.c_double), C.c_int]
         L.orc_set_face_albedo.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         L.orc_set_emission.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+        L.orc_mlt_mutate0.restype = C.c_uint32
+        L.orc_mlt_mutate0.argtypes = [C.c_uint32, C.c_double]
+        L.orc_render_mlt.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                     C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
         _lib = L
     return _lib
 
@@ -113,6 +117,20 @@ class OracleScene:
         keys = ["primary", "extension", "shadow", "pickup", "nodes", "tris", "truncated"]
         return rgb, var, dict(zip(keys, [int(x) for x in stats]))
 
+    def render_mlt(self, seed: int, mutation: float = 0.0, pre_sample: float = 0.0, p_large: float = 0.0,
+                   mutations_per_chain: int = 0, samples: Optional[int] = None, chains: int = 0,
+                   sample_depths: int = 0, threads: int = 0):
+        """Metropolis render as prt_render_mlt runs it; returns (rgb[H,W,3]
+        float64, stats)."""
+        W, H = self.desc.camera.width, self.desc.camera.height
+        rgb = np.zeros((H, W, 3), dtype=np.float64)
+        stats = (C.c_uint64 * 6)()
+        rc = lib().orc_render_mlt(self.h, seed, mutation, pre_sample, p_large, mutations_per_chain, samples or 0,
+                                  chains, sample_depths, rgb.ctypes.data, stats, threads)
+        assert rc == 0, rc
+        keys = ["seeds", "chains", "steps", "proposals", "accepts", "lum_total"]
+        return rgb, dict(zip(keys, [int(x) for x in stats]))
+
     def radiance_sample(self, seed: int, x: int, y: int, sample: int):
         out = (C.c_double * 3)()
         lib().orc_radiance_sample(self.h, seed, x, y, sample, out)
@@ -154,3 +172,8 @@ class OracleScene:
         rc = lib().orc_adjoint(self.h, seed, samples or 0, d.ctypes.data, entity, g.ctypes.data, light, ge, threads)
         assert rc == 0, rc
         return g, np.array(ge[:])
+
+
+def mlt_mutate0(word: int, r: float) -> int:
+    """Kelemen's Mutate0 (mlt.h:42-54) of one primary-sample word."""
+    return int(lib().orc_mlt_mutate0(word, r))

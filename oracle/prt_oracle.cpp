@@ -120,16 +120,29 @@ enum {
   DIM_GLASS = 13
 };
 
+// Camera words of a Metropolis proposal before its per-vertex words.
+static const int MLT_CAMERA_WORDS = 8;
+
 struct PathKey {
   uint32_t key[2];
   uint32_t pixel, sample;
   uint32_t depth, branch;
+  // Metropolis renders: the chain's primary samples (null otherwise); the
+  // first mlt_depths vertices take dimensions 0..3 from them
+  const uint32_t* mlt;
+  int mlt_depths;
   // lazily filled Philox blocks for this vertex
   mutable uint32_t blk[4][4];
   mutable unsigned have;
 };
 static inline double u01_from(uint32_t w) { return (double)w * 2.3283064365386963e-10; }
 static inline double rnd(const PathKey& pk, int dim) {
+  if (pk.mlt) {
+    // the glass reflect / refract choice (brdf.h:866) is a primary sample
+    // too; it shares the vertex's BSDF-choice word, which glass leaves unused
+    if (dim == DIM_GLASS) dim = DIM_MATERIAL;
+    if (dim < 4 && (int)pk.depth < pk.mlt_depths) return u01_from(pk.mlt[MLT_CAMERA_WORDS + 4 * pk.depth + dim]);
+  }
   const int b = dim >> 2;
   if (!(pk.have & (1u << b))) {
     uint32_t c[4] = {pk.pixel, pk.sample, (pk.depth & 0xFFu) | (pk.branch << 8), (uint32_t)b};
@@ -143,6 +156,7 @@ static inline PathKey vertex_key(const uint32_t key[2], uint32_t pixel, uint32_t
   PathKey pk;
   pk.key[0] = key[0]; pk.key[1] = key[1];
   pk.pixel = pixel; pk.sample = sample; pk.depth = (uint32_t)depth; pk.branch = branch;
+  pk.mlt = nullptr; pk.mlt_depths = 0;
   pk.have = 0;
   return pk;
 }
@@ -661,6 +675,8 @@ struct Ctx {
   uint32_t pixel, sample;
   Stats st;
   Adj* ad = nullptr;
+  const uint32_t* mlt = nullptr;  // Metropolis proposal's primary samples
+  int mlt_depths = 0;
 };
 
 static V3 radiance(Ctx& C, const Ray& ray, int depth, uint32_t branch, const V3& T);
@@ -881,7 +897,9 @@ static V3 radiance(Ctx& C, const Ray& ray, int depth, uint32_t branch, const V3&
   const V3 n_orient = dot(h.normal, ray.dir) < 0.0 ? h.normal : (-1.0 * h.normal);
   double rr = rgb_max(m.emission + m.color);
   if (depth > S.depth_limit) rr *= std::pow(0.5, depth - S.depth_limit);
-  const PathKey pk = vertex_key(C.key, C.pixel, C.sample, depth, branch);
+  PathKey pk = vertex_key(C.key, C.pixel, C.sample, depth, branch);
+  pk.mlt = C.mlt;
+  pk.mlt_depths = C.mlt_depths;
   if (depth > S.depth) {
     if (rnd(pk, DIM_RR) >= rr) {
       if (C.ad && h.entity == C.ad->light) adj_emission(C, T, V3(1, 1, 1));
@@ -1303,6 +1321,235 @@ int orc_set_emission(orc_scene* s, int entity, const double emission[3]) {
   m.emission = from(emission);
   for (Light& L : S.lights) L.power = S.ents[L.entity].area * length(S.mats[S.ents[L.entity].mat].emission);
   std::stable_sort(S.lights.begin(), S.lights.end(), [](const Light& a, const Light& b) { return a.power > b.power; });
+  return PRT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- Metropolis
+// Primary-sample-space MLT as the HIP path runs it (prender_amd/csrc/prt_mlt.*):
+// Render::RunMLT (render.h:360-780), KelemenMLT (mlt.h:70-170) and
+// generate_new_path (render.h:195-356) with many chains side by side, all
+// randomness from Philox streams keyed by (chain, step, tag).
+namespace orc {
+
+static const uint32_t MLT_TAG_LARGE = 0x4C415247u, MLT_TAG_MUTATE = 0x4D555441u, MLT_TAG_FRESH = 0x46524553u,
+                      MLT_TAG_ACCEPT = 0x41434350u, MLT_TAG_INIT = 0x494E4954u, MLT_TAG_SEED = 0x53454544u;
+static const double MLT_FIX_ONE = 1048576.0, MLT_LUM_ONE = 16777216.0;
+
+static uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+static void mlt_key(uint64_t seed, uint64_t phase, uint64_t idx, uint32_t k[2]) {
+  const uint64_t h = splitmix64(splitmix64(seed ^ (phase * 0xA0761D6478BD642Full)) + idx);
+  k[0] = (uint32_t)h;
+  k[1] = (uint32_t)(h >> 32);
+}
+static void philox_at(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, const uint32_t k[2], uint32_t out[4]) {
+  const uint32_t c[4] = {c0, c1, c2, c3};
+  philox4x32_10(c, k, out);
+}
+static uint32_t word_of(double x) { return x >= 1.0 ? 0u : (uint32_t)(x * 4294967296.0); }
+// Mutate0, mlt.h:42-54
+static uint32_t mutate0(uint32_t w, double r) {
+  const double x = u01_from(w);
+  const double s1 = 1.0 / 512.0, s2 = 1.0 / 16.0;
+  const double dx = s1 / (s1 / s2 + (double)std::fabs((float)(2.0 * r - 1.0))) - s1 / (s1 / s2 + 1.0);
+  if (r < 0.5) {
+    const double x1 = x + dx;
+    return word_of((x1 < 1.0) ? x1 : x1 - 1.0);
+  }
+  const double x1 = x - dx;
+  return word_of((x1 < 0.0) ? x1 + 1.0 : x1);
+}
+// generate_new_path's camera (render.h:262-300) with the tent filter
+static Ray mlt_camera_ray(const Camera& cam, const uint32_t* u, uint32_t& pix) {
+  const int x = std::min((int)(u01_from(u[0]) * (double)cam.width), cam.width - 1);
+  const int y = std::min((int)(u01_from(u[1]) * (double)cam.height), cam.height - 1);
+  const int sx = u01_from(u[2]) < 0.5 ? 0 : 1;
+  const int sy = u01_from(u[3]) < 0.5 ? 0 : 1;
+  const double rr1 = 2.0 * u01_from(u[4]), rr2 = 2.0 * u01_from(u[5]);
+  const double dx = rr1 < 1.0 ? std::sqrt(rr1) - 1.0 : 1.0 - std::sqrt(2.0 - rr1);
+  const double dy = rr2 < 1.0 ? std::sqrt(rr2) - 1.0 : 1.0 - std::sqrt(2.0 - rr2);
+  const double r1 = (sx + 0.5 + dx) * 0.5;
+  const double r2 = (sy + 0.5 + dy) * 0.5;
+  const V3 sp = cam.center + cam.screen_x * ((r1 + x) * cam.iw - 0.5) + cam.screen_y * ((r2 + y) * cam.ih - 0.5);
+  pix = (uint32_t)((cam.height - y - 1) * cam.width + x);
+  return Ray{cam.position, normalize(sp - cam.position)};
+}
+static void mlt_splat(std::vector<uint64_t>& img, uint32_t pix, double w, const V3& f) {
+  const double c[3] = {f.x, f.y, f.z};
+  for (int k = 0; k < 3; ++k) {
+    const double v = w * c[k];
+    if (v > 0.0) img[3 * (size_t)pix + k] += (uint64_t)(v * MLT_FIX_ONE + 0.5);
+  }
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// Kelemen's Mutate0 on one primary-sample word (for the tests).
+uint32_t orc_mlt_mutate0(uint32_t w, double r) { return orc::mutate0(w, r); }
+
+// Metropolis render (prt_render_mlt's algorithm).  out_rgb: W*H*3 doubles in
+// image row order.  stats: [seed paths, chains, steps per chain, proposals,
+// accepts, luminance total (fixed point)].
+int orc_render_mlt(const orc_scene* s, uint64_t seed, double mutation, double pre_sample, double p_large,
+                   int64_t mutations_per_chain, int samples_override, int chains, int sample_depths,
+                   double* out_rgb, uint64_t* stats, int nthreads) {
+  if (!s || !out_rgb) return PRT_E_INVALID;
+  const Scene& S = s->S;
+  const int W = S.cam.width, H = S.cam.height;
+  const int64_t n_pix = (int64_t)W * H;
+  const int samples = samples_override > 0 ? samples_override : S.samples;
+  if (mutation <= 0.0) mutation = 32.0;
+  if (pre_sample <= 0.0) pre_sample = 0.4;
+  p_large = p_large > 0.0 ? std::min(p_large, 1.0) : 0.5;
+  const int depths = sample_depths > 0 ? sample_depths : 8;
+  if (depths > 32 || samples <= 0) return PRT_E_INVALID;
+  int64_t N = (int64_t)(n_pix * pre_sample) * samples;
+  if (N < 1) N = 1;
+  if (N > ((int64_t)1 << 24)) return PRT_E_INVALID;
+  const double total_steps = (double)samples * (double)n_pix * mutation;
+  const int64_t C = chains > 0 ? chains : (int64_t)std::min(std::max(total_steps / 64.0, 1024.0), (double)(1 << 20));
+  if (C > (1 << 20)) return PRT_E_INVALID;
+  const int64_t M = mutations_per_chain > 0 ? mutations_per_chain
+                                            : std::max<int64_t>(1, (int64_t)std::ceil(total_steps / (double)C));
+  const int stride = MLT_CAMERA_WORDS + 4 * depths;
+  uint32_t kc[2];
+  mlt_key(seed, 3, 0, kc);
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+  (void)nthreads;
+#endif
+  auto seed_words = [&](uint32_t sidx, uint32_t* u) {
+    for (int blk = 0; blk < stride / 4; ++blk) philox_at(sidx, 0u, MLT_TAG_SEED, (uint32_t)blk, kc, u + 4 * blk);
+  };
+  auto path = [&](const uint32_t key[2], uint32_t chain, const uint32_t* u, uint32_t& pix) {
+    Ctx Cx;
+    Cx.S = &S;
+    Cx.key[0] = key[0]; Cx.key[1] = key[1];
+    Cx.pixel = chain;
+    Cx.sample = 0;
+    Cx.mlt = u;
+    Cx.mlt_depths = depths;
+    const Ray ray = mlt_camera_ray(S.cam, u, pix);
+    return radiance(Cx, ray, 0, 1, V3(1, 1, 1));
+  };
+  // seed paths, in batches of C (render.h:479-503)
+  std::vector<V3> seed_f((size_t)N);
+  std::vector<uint64_t> seed_lum((size_t)N);
+  std::vector<uint32_t> seed_pix((size_t)N);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#endif
+  for (int64_t si = 0; si < N; ++si) {
+    uint32_t key[2];
+    mlt_key(seed, 1, (uint64_t)(si / C), key);
+    std::vector<uint32_t> u((size_t)stride);
+    seed_words((uint32_t)si, u.data());
+    uint32_t pix;
+    const V3 f = path(key, (uint32_t)(si % C), u.data(), pix);
+    seed_f[(size_t)si] = f;
+    const double l = luminance(f) * MLT_LUM_ONE + 0.5;
+    seed_lum[(size_t)si] = l < 1099511627776.0 ? (uint64_t)l : 1099511627776ull;
+    seed_pix[(size_t)si] = pix;
+  }
+  std::vector<uint64_t> cdf((size_t)N);
+  uint64_t run = 0;
+  for (int64_t i = 0; i < N; ++i) cdf[(size_t)i] = (run += seed_lum[(size_t)i]);
+  const uint64_t total = cdf[(size_t)N - 1];
+  std::vector<uint64_t> img((size_t)n_pix * 3, 0);
+  uint64_t accepts = 0;
+  int64_t steps_run = 0;
+  if (total > 0) {
+    const double b = (double)total / MLT_LUM_ONE / (double)N;
+    const double weight = 4.0 * (double)n_pix, inv_m = 1.0 / (double)M;
+    std::vector<uint32_t> cur((size_t)C * stride), prop((size_t)C * stride);
+    std::vector<V3> f_old((size_t)C);
+    std::vector<uint32_t> pix_old((size_t)C);
+    for (int64_t c = 0; c < C; ++c) {  // start paths (mlt.h:201-220)
+      uint32_t r4[4];
+      philox_at((uint32_t)c, 0u, MLT_TAG_INIT, 0u, kc, r4);
+      uint64_t r = (uint64_t)(u01_from(r4[0]) * (double)total);
+      if (r >= total) r = total - 1;
+      const size_t sidx = (size_t)(std::upper_bound(cdf.begin(), cdf.end(), r) - cdf.begin());
+      seed_words((uint32_t)sidx, &cur[(size_t)c * stride]);
+      f_old[(size_t)c] = seed_f[sidx];
+      pix_old[(size_t)c] = seed_pix[sidx];
+    }
+    for (int64_t it = 0; it < M; ++it) {
+      uint32_t key[2];
+      mlt_key(seed, 2, (uint64_t)it, key);
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+      {
+        std::vector<uint64_t> local((size_t)n_pix * 3, 0);
+        uint64_t acc_local = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (int64_t c = 0; c < C; ++c) {
+          uint32_t* pu = &prop[(size_t)c * stride];
+          const uint32_t* cu = &cur[(size_t)c * stride];
+          uint32_t r4[4];
+          philox_at((uint32_t)c, (uint32_t)it, MLT_TAG_LARGE, 0u, kc, r4);
+          const bool large = u01_from(r4[0]) < p_large;
+          for (int blk = 0; blk < stride / 4; ++blk) {
+            if (large) {
+              philox_at((uint32_t)c, (uint32_t)it, MLT_TAG_FRESH, (uint32_t)blk, kc, pu + 4 * blk);
+            } else {
+              uint32_t r[4];
+              philox_at((uint32_t)c, (uint32_t)it, MLT_TAG_MUTATE, (uint32_t)blk, kc, r);
+              for (int k = 0; k < 4; ++k) pu[4 * blk + k] = mutate0(cu[4 * blk + k], u01_from(r[k]));
+            }
+          }
+          uint32_t p_new;
+          const V3 fn = path(key, (uint32_t)c, pu, p_new);
+          const V3 fo = f_old[(size_t)c];
+          // render.h:574-630
+          const double l_new = luminance(fn), l_old = luminance(fo);
+          const double a = l_old > 1.0e-16 ? std::min(1.0, l_new / l_old) : 1.0;
+          const double lg = large ? 1.0 : 0.0;
+          const double w_new = (a + lg) / (l_new / b + p_large) * inv_m;
+          const double w_old = (1.0 - a) / (l_old / b + p_large) * inv_m;
+          mlt_splat(local, pix_old[(size_t)c], w_old * weight, fo);
+          mlt_splat(local, p_new, w_new * weight, fn);
+          philox_at((uint32_t)c, (uint32_t)it, MLT_TAG_ACCEPT, 0u, kc, r4);
+          if (u01_from(r4[0]) < a) {
+            std::memcpy(&cur[(size_t)c * stride], pu, sizeof(uint32_t) * stride);
+            f_old[(size_t)c] = fn;
+            pix_old[(size_t)c] = p_new;
+            ++acc_local;
+          }
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+          for (size_t i = 0; i < img.size(); ++i) img[i] += local[i];
+          accepts += acc_local;
+        }
+      }
+    }
+    steps_run = M;
+  }
+  const double scale = S.sensor_response * 0.25 / (double)C;
+  for (size_t i = 0; i < img.size(); ++i) out_rgb[i] = (double)img[i] * (1.0 / MLT_FIX_ONE) * scale;
+  if (stats) {
+    stats[0] = (uint64_t)N;
+    stats[1] = (uint64_t)C;
+    stats[2] = (uint64_t)steps_run;
+    stats[3] = (uint64_t)C * (uint64_t)steps_run;
+    stats[4] = accepts;
+    stats[5] = total;
+  }
   return PRT_OK;
 }
 

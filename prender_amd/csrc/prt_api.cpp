@@ -19,6 +19,7 @@
 
 #include "../../include/prt.h"
 #include "bvh.h"
+#include "prt_mlt.h"
 #include "prt_device.h"
 #include "prt_internal.h"
 #include "prt_kernels.h"
@@ -153,6 +154,7 @@ void prt_scene_destroy(prt_scene* s) {
   if (!s) return;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->stream2) (void)hipStreamSynchronize(s->stream2);
   s->nodes.release(); s->tris.release(); s->tri_nrm.release(); s->tri_alb.release();
   s->ents.release(); s->mats.release(); s->analytic.release(); s->lights.release(); s->ovf.release();
   s->ws.release();
@@ -163,7 +165,7 @@ void prt_scene_destroy(prt_scene* s) {
   s->d_slot_of_tri.release();
   s->grad_buf.release();
   s->tri_alb_rr.release();
-  if (s->stream2) (void)hipStreamSynchronize(s->stream2);
+  s->mlt.release();
   for (hipEvent_t e : s->ev_nee)
     if (e) (void)hipEventDestroy(e);
   if (s->ev_shadow) (void)hipEventDestroy(s->ev_shadow);
@@ -436,7 +438,7 @@ int prt_scene_info(const prt_scene* s, int64_t* n_nodes, int64_t* n_tris, int64_
   return PRT_OK;
 }
 
-static void fill_stats(prt_stats* st, const uint64_t* c) {
+extern "C++" void prt::fill_stats(prt_stats* st, const uint64_t* c) {
   st->node_visits += c[CTR_NODES] + c[CTR_SNODES];
   st->tri_tests += c[CTR_TRIS] + c[CTR_STRIS];
   st->shadow_node_visits += c[CTR_SNODES];
@@ -506,7 +508,7 @@ int prt_trace_closest(prt_scene* s, const prt_ray* rays, int64_t n, prt_hit* hit
 }
 
 // ---------------------------------------------------------------- render
-static int ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum) {
+extern "C++" int prt::ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum) {
   Workspace& w = s->ws;
   // A slab writer over input chunks [c0, c1) owns fan * 64 * (c1 - c0)
   // entries from fan * 64 * c0 on, so a launch's outputs span fan * 64 *
@@ -578,12 +580,6 @@ static int setup_render(prt_scene* s, const prt_render_opts* o, RenderSetup& R) 
   R.P = P;
   return PRT_OK;
 }
-
-// Counters of a call, summed over its batches.
-struct CallTotals {
-  uint64_t primary_rays = 0;
-  int64_t launches = 0;
-};
 
 // Depths enqueued between host reads of the ray count (a depth with an
 // empty queue costs only near-empty launches).
@@ -685,7 +681,8 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
   // the primary rays: one slab of np rays at the start of q[0]
   HIPCHK(launch_fill_u32((uint32_t*)slabs, 0u, 1, st));
   HIPCHK(launch_fill_u32((uint32_t*)slabs + 1, (uint32_t)np, 1, st));
-  HIPCHK(launch_gen_primary(P, B, q[0], st));
+  if (P.mlt_u) HIPCHK(launch_mlt_gen(P, (uint32_t)np, q[0], st));
+  else HIPCHK(launch_gen_primary(P, B, q[0], st));
   HIPCHK(index(0, 1, slabs, st));
   if ((rc = tm.mark(T_OTHER))) return rc;
   if ((rc = next_fetch(&f))) return rc;
@@ -726,8 +723,8 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
   return PRT_OK;
 }
 
-static int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* accum, const Adjoint& A, Timer& tm,
-                     CallTotals& tot) {
+extern "C++" int prt::run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* accum, const Adjoint& A, Timer& tm,
+                   CallTotals& tot) {
   const int rc = run_batch_on(s, P, B, accum, A, tm, tot);
   // a failed batch may have left shadow work queued on the second stream
   if (rc != PRT_OK) (void)hipStreamSynchronize(s->stream2);

@@ -85,4 +85,16 @@ struct Params {
   uint32_t* stack_ovf;       // per-thread traversal stack overflow
   int32_t ovf_threads;
   int32_t n_tris;
+  // Metropolis renders (null otherwise): the batch's "pixels" are Markov
+  // chains; chain c's primary samples of the current proposal are
+  // mlt_u[c * mlt_stride ...], MLT_CAMERA_WORDS camera words, then 4 words
+  // per path vertex for the first mlt_depths vertices; the proposal's image
+  // element goes to mlt_pix[c]
+  const uint32_t* mlt_u;
+  uint32_t* mlt_pix;
+  int32_t mlt_stride;
+  int32_t mlt_depths;
 };
+// camera words of a Metropolis proposal: pixel x, y, sub-pixel x, y, tent
+// filter x, y, two spare (render.h:262-300)
+#define MLT_CAMERA_WORDS 8

@@ -164,6 +164,32 @@ enum { T_EXTEND = 0, T_SHADE = 1, T_SHADOW = 2, T_OTHER = 3, T_EXTEND0 = 4, T_NE
 struct StreamWorkspace;
 void release_stream_workspace(StreamWorkspace* w);
 
+// Chain and seed-path state of Metropolis renders (prt_mlt.cpp).
+struct MltWorkspace {
+  DevBuf cur, prop, pix_new, pix_old, f_old, img;
+  DevBuf seed_f, seed_lum, seed_pix, seed_cdf, tile_sums;
+  void release() {
+    for (DevBuf* b : {&cur, &prop, &pix_new, &pix_old, &f_old, &img, &seed_f, &seed_lum, &seed_pix, &seed_cdf,
+                      &tile_sums})
+      b->release();
+  }
+};
+
+// Counters of a call, summed over its batches.
+struct CallTotals {
+  uint64_t primary_rays = 0;
+  int64_t launches = 0;
+};
+
+// Adds the traversal / shading counters c[CTR_*] to *st.
+void fill_stats(prt_stats* st, const uint64_t* c);
+// Sizes the scene's workspace for batches of `batch_samples` camera samples.
+int ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum);
+// One batch of camera samples (or Metropolis proposals, when P.mlt_u is set)
+// through the wavefront; sample radiance accumulates into accum (fixed point).
+int run_batch(prt_scene* s, const Params& P, const Batch& B, uint64_t* accum, const Adjoint& A, Timer& tm,
+              CallTotals& tot);
+
 }  // namespace prt
 
 struct prt_scene {
@@ -192,4 +218,5 @@ struct prt_scene {
   prt::DevBuf d_slot_of_tri;            // input triangle -> leaf slot (per-face albedo updates)
   prt::DevBuf grad_buf;                 // adjoint accumulators (double)
   prt::DevBuf tri_alb_rr;               // held albedo for Russian-roulette probabilities
+  prt::MltWorkspace mlt;                // Metropolis renders
 };

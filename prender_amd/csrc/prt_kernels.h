@@ -21,7 +21,8 @@ enum {
   CTR_TRACED = 11,   // closest-hit queries finished by k_extend (must equal the rays queued)
   CTR_STRACED = 12,  // shadow rays finished by k_shadow (must equal CTR_SHADOW)
   CTR_QUEUED = 13,   // rays of the ray queues listed for k_extend (primary + extension)
-  CTR_COUNT = 14
+  CTR_MLT_ACCEPT = 14,  // Metropolis proposals accepted
+  CTR_COUNT = 15
 };
 
 struct prt_ray_dev {

@@ -190,8 +190,13 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
       const uint32_t pl = i / P.spp, k = i % P.spp;
       const int sub = (int)(k % ss2);
       const int64_t pix = B.pixel_base + pl;
-      camera_ray(P, (int)(pix % P.W), (int)(pix / P.W), sub % P.SS, sub / P.SS, o, d);
       const uint32_t hi = pl * ss2 + sub;
+      if (P.mlt_u != nullptr) {  // a Metropolis proposal's own camera ray
+        o = mk(qin.ox[hi], qin.oy[hi], qin.oz[hi]);
+        d = mk(qin.dx[hi], qin.dy[hi], qin.dz[hi]);
+      } else {
+        camera_ray(P, (int)(pix % P.W), (int)(pix / P.W), sub % P.SS, sub / P.SS, o, d);
+      }
       t = hit_t[hi];
       id = hit_id[hi];
       T = mk(1.0, 1.0, 1.0);

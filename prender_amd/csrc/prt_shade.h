@@ -29,6 +29,24 @@ __device__ __forceinline__ void camera_ray(const Params& P, int x, int y, int sx
   d = normalize(sp - o);
 }
 
+// Random words 0..3 of a path vertex: Russian roulette, BSDF choice, the two
+// direction numbers (counter (pixel, sample, depth | branch << 8, 0)).  In a
+// Metropolis render the first mlt_depths vertices read them from the chain's
+// primary samples instead (the chain is the batch "pixel"); the reference's
+// primary samples are exactly these next01() draws (radiance.h:108, 517,
+// 534; brdf.h:511-512, 866), while light sampling draws Next01() from an
+// independent stream (directLight_radiance.h:343-359).
+__device__ __forceinline__ void vertex_words(const Params& P, uint32_t pixel, uint32_t sample, int depth,
+                                             uint32_t branch, uint32_t b[4]) {
+  if (P.mlt_u != nullptr && depth < P.mlt_depths) {
+    const uint4 v = *reinterpret_cast<const uint4*>(P.mlt_u + (size_t)pixel * (uint32_t)P.mlt_stride +
+                                                    MLT_CAMERA_WORDS + 4 * depth);
+    b[0] = v.x; b[1] = v.y; b[2] = v.z; b[3] = v.w;
+    return;
+  }
+  philox(pixel, sample, rng_c2(depth, branch), 0u, P.key0, P.key1, b);
+}
+
 // Children go to qout positions off + j, j < cap, with j from *count_out.
 struct ShadeCtx {
   RayQ qout;
@@ -135,7 +153,7 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
   if (depth > P.depth_limit) rr *= ldexp(1.0, -(depth - P.depth_limit));  // pow(0.5, k), exact
   uint32_t b0[4];
   const uint32_t c2 = rng_c2(depth, branch);
-  philox(pixel, sample, c2, 0u, P.key0, P.key1, b0);
+  vertex_words(P, pixel, sample, depth, branch, b0);
   if (depth > P.depth) {
     if (u01(b0[0]) >= rr) {  // terminate with the emission
       va.own = vmul(T, E);
@@ -216,9 +234,15 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
   const double Tr = (1.0 - Re) * nnt2;
   const double probability = 0.25 + 0.5 * Re;
   if (depth > 2) {
-    uint32_t b3[4];
-    philox(pixel, sample, c2, 3u, P.key0, P.key1, b3);
-    const bool take_refl = u01(b3[1]) < probability;  // dimension 13
+    // dimension 13; a Metropolis proposal uses the vertex's BSDF-choice word,
+    // unused by glass, so that the choice is a primary sample
+    uint32_t wg = b0[1];
+    if (P.mlt_u == nullptr) {
+      uint32_t b3[4];
+      philox(pixel, sample, c2, 3u, P.key0, P.key1, b3);
+      wg = b3[1];
+    }
+    const bool take_refl = u01(wg) < probability;
     const d3 wgt = take_refl ? color / (probability * rr) / (1.0 - 0.0)
                              : color / ((1.0 - probability) * rr) / (1.0 - 0.0);
     const d3 tw = vmul(T, wgt);

@@ -87,7 +87,8 @@ class PrtStats(C.Structure):
                 ("shadow_tri_tests", C.c_uint64), ("bytes_extend", C.c_uint64),
                 ("bytes_shadow", C.c_uint64), ("ms_extend_primary", C.c_double),
                 ("ms_nee", C.c_double), ("mixed_scene_passes", C.c_uint64),
-                ("ms_traversal", C.c_double)]
+                ("ms_traversal", C.c_double), ("mlt_proposals", C.c_uint64),
+                ("mlt_accepts", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -97,6 +98,16 @@ class PrtRenderOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("samples_override", C.c_int32),
                 ("batch_pixels", C.c_int32), ("max_depth", C.c_int32),
                 ("use_graph", C.c_int32), ("timing", C.c_int32), ("image_f64", C.c_int32)]
+
+
+class PrtMltOpts(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("mutation", C.c_double), ("pre_sample", C.c_double),
+                ("p_large", C.c_double), ("mutations_per_chain", C.c_int64), ("samples_override", C.c_int32),
+                ("chains", C.c_int32), ("sample_depths", C.c_int32), ("max_depth", C.c_int32),
+                ("image_f64", C.c_int32), ("pad_", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
 
 
 class PrtBatchOpts(C.Structure):
