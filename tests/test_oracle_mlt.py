@@ -1,0 +1,160 @@
+"""Pins the oracle's primary-sample-space Metropolis renderer (oracle
+orc_render_mlt, the algorithm of prt_render_mlt; Render::RunMLT,
+render.h:360-780, KelemenMLT, mlt.h:70-170) before the HIP path is compared
+with it.
+
+* Mutate0 against mlt.h:42-54's closed form in exact rational arithmetic and
+  its symmetry (a symmetric proposal is what makes min(1, L'/L) the right
+  acceptance);
+* a scene whose every path has the same radiance: every proposal is accepted
+  and the image total has a closed form in the number of large steps, which
+  fixes the splat weights (render.h:588-589), the path weight 4*W*H and the
+  0.25 sensor factor (scene_env.h:1955);
+* the Metropolis image against a path-traced one of the same scene: both
+  estimate the same pixel integrals (up to the tent filter's reach across
+  pixel edges), so interior means and block averages must agree.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleScene, mlt_mutate0, philox4x32_10
+from prender_amd import generators as G
+from prender_amd.scene import Camera, Material, SceneDesc, Sampling, Sphere
+
+TAG_LARGE = 0x4C415247
+
+
+def lum(a):
+    return (0.2126 * a[..., 0] + 0.0722 * a[..., 2]) + 0.7152 * a[..., 1]
+
+
+def splitmix64(x):
+    m = (1 << 64) - 1
+    x = (x + 0x9E3779B97F4A7C15) & m
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m
+    return x ^ (x >> 31)
+
+
+def control_key(seed):
+    """Key of the chain-control Philox streams (phase 3)."""
+    m = (1 << 64) - 1
+    h = splitmix64((splitmix64(seed ^ ((3 * 0xA0761D6478BD642F) & m)) + 0) & m)
+    return [h & 0xFFFFFFFF, h >> 32]
+
+
+# ---------------------------------------------------------------- Mutate0
+def mutate0_exact(x, r):
+    s1, s2 = Fraction(1, 512), Fraction(1, 16)
+    a = abs(2 * r - 1)
+    dx = s1 / (s1 / s2 + a) - s1 / (s1 / s2 + 1)
+    return (x + dx) % 1 if r < Fraction(1, 2) else (x - dx) % 1
+
+
+@pytest.mark.parametrize("x,r", [(0.5, 0.25), (0.5, 0.75), (0.1, 0.5), (0.97, 0.3), (0.02, 0.9), (0.5, 0.0),
+                                 (0.25, 0.125)])
+def test_mutate0_closed_form(x, r):
+    # x and r are exact binary fractions, so |2r-1| is exact in float too
+    w = int(x * 2 ** 32)
+    got = mlt_mutate0(w, r) / 2 ** 32
+    want = float(mutate0_exact(Fraction(w, 2 ** 32), Fraction(r)))
+    assert abs(got - want) <= 2 ** -31, (got, want)
+
+
+def test_mutate0_step_sizes_and_symmetry():
+    # largest step at r = 1/2: s2 - s1 / (s1/s2 + 1); none at r = 0 and 1
+    w = 1 << 31
+    big = mlt_mutate0(w, 0.5) / 2 ** 32 - 0.5
+    assert abs(big + (1 / 16 - (1 / 512) / (1 / 32 + 1))) < 1e-9  # r = 1/2 moves down (r >= 1/2)
+    assert mlt_mutate0(w, 0.0) == w
+    # r and 1 - r move by the same amount in opposite directions (mod 1)
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        w = int(rng.integers(0, 2 ** 32))
+        r = float(rng.integers(1, 2 ** 20)) / 2 ** 21  # r < 1/2, exact
+        up = (mlt_mutate0(w, r) - w) % 2 ** 32
+        down = (w - mlt_mutate0(w, 1.0 - r)) % 2 ** 32
+        assert abs(up - down) <= 2, (w, r, up, down)
+
+
+# ---------------------------------------------------------------- constant radiance
+def empty_scene(W=16, H=12, bg=(0.25, 0.5, 0.75), sensor=1.0):
+    # one sphere behind the camera: every camera ray misses and returns the
+    # background colour
+    return SceneDesc(camera=Camera(width=W, height=H),
+                     sampling=Sampling(samples=2, supersamples=2, luminance_cutoff=0.0, background=bg,
+                                       sensor_response=sensor),
+                     materials=[Material(color=(0.5, 0.5, 0.5))],
+                     entities=[Sphere(position=(50.0, 52.0, 400.0), radius=5.0, material=0)])
+
+
+def test_mlt_constant_radiance_closed_form():
+    W, H, C, M, seed = 16, 12, 96, 40, 11
+    bg = np.array([0.25, 0.5, 0.75])
+    o = OracleScene(empty_scene(W, H, tuple(bg), sensor=1.5))
+    img, st = o.render_mlt(seed=seed, chains=C, mutations_per_chain=M, pre_sample=1.0)
+    assert st["accepts"] == st["proposals"] == C * M  # a = 1 for every proposal
+    kc = control_key(seed)
+    n_large = sum(philox4x32_10([c, it, TAG_LARGE, 0], kc)[0] * 2.0 ** -32 < 0.5
+                  for c in range(C) for it in range(M))
+    assert 0.4 * C * M < n_large < 0.6 * C * M
+    b = st["lum_total"] / 2.0 ** 24 / st["seeds"]
+    L = float(lum(bg))
+    weight, inv_m = 4.0 * W * H, 1.0 / M
+    total = np.zeros(3)
+    for large, count in ((1.0, n_large), (0.0, C * M - n_large)):
+        w = (1.0 + large) / (L / b + 0.5) * inv_m * weight
+        for k in range(3):
+            total[k] += count * int(w * bg[k] * 2.0 ** 20 + 0.5)
+    want = total * 2.0 ** -20 * 1.5 * 0.25 / C
+    got = img.reshape(-1, 3).sum(axis=0)
+    np.testing.assert_allclose(got, want, rtol=1e-12)
+    # ... which is the background over the image, up to the large-step count
+    np.testing.assert_allclose(got / (W * H), bg * 1.5, rtol=0.05)
+
+
+def test_mlt_seed_sensitivity_and_reproducibility():
+    d = G.config1(width=16, height=12, samples=2)
+    d.sampling.luminance_cutoff = 0.0
+    o = OracleScene(d)
+    a, sa = o.render_mlt(seed=3, chains=128, mutations_per_chain=16)
+    b, sb = o.render_mlt(seed=3, chains=128, mutations_per_chain=16)
+    c, _ = o.render_mlt(seed=4, chains=128, mutations_per_chain=16)
+    assert np.array_equal(a, b) and sa == sb
+    assert not np.array_equal(a, c)
+    assert 0 < sa["accepts"] < sa["proposals"] == 128 * 16
+
+
+def test_mlt_matches_path_tracing():
+    # 48x36 cornell spheres: the Metropolis image and a 256 spp path-traced
+    # image estimate the same pixel integrals.  The tent filter (2x2
+    # sub-pixel grid +-1/2 pixel, render.h:279-285) reaches across pixel
+    # edges, so pixels are compared as 6x6 blocks.  The Metropolis image's
+    # total is b (the seed paths' mean luminance) by construction; with
+    # 100 seed paths per pixel and chain b carries ~1 % noise.
+    W, H = 48, 36
+    d = G.config1(width=W, height=H, samples=4)
+    d.sampling.luminance_cutoff = 0.0
+    o = OracleScene(d)
+    pt, _, _ = o.render(seed=5, samples=256)
+    m, st = o.render_mlt(seed=7, mutation=256, chains=4096, pre_sample=100.0)
+    assert st["seeds"] == int(W * H * 100.0) * 4
+    b = st["lum_total"] / 2.0 ** 24 / st["seeds"]
+    sr = d.sampling.sensor_response
+    assert abs(lum(m).mean() / (b * sr) - 1.0) < 0.02
+    assert abs(b * sr / lum(pt).mean() - 1.0) < 0.05
+    inner = (slice(H // 8, H - H // 8), slice(W // 8, W - W // 8))
+    assert abs(lum(m[inner]).mean() / lum(pt[inner]).mean() - 1.0) < 0.08
+    k = 6
+    bp = pt.reshape(H // k, k, W // k, k, 3).mean(axis=(1, 3))
+    bm = m.reshape(H // k, k, W // k, k, 3).mean(axis=(1, 3))
+    rel = np.abs(bm - bp).sum() / np.abs(bp).sum()
+    assert rel < 0.08, rel
+    # the same blocks with each image normalised by its mean: the shape
+    shape = np.abs(bm / lum(m).mean() - bp / lum(pt).mean()).sum() / np.abs(bp / lum(pt).mean()).sum()
+    assert shape < 0.07, shape
+    # Metropolis acceptance of this scene (about 42 %)
+    assert 0.3 < st["accepts"] / st["proposals"] < 0.6
