@@ -331,6 +331,12 @@ prt_scene_file* prt_scene_file_load(const char* path);
 const prt_scene_desc* prt_scene_file_desc(const prt_scene_file* file);
 /* OUTPUT (default "output.ppm") and gamma_offset of the file. */
 const char* prt_scene_file_output(const prt_scene_file* file, double* gamma_offset);
+/* The file's Metropolis switches (metropolisTransport, mutation, pre_sample
+ * and the METROPOLIS / MUTATION / PRESAMPLE overrides, scene_env.h:642-663,
+ * 1908-1911, 1946-1956): returns 1 when the file asks for RunMLT, else 0, and
+ * fills *opts (if not NULL) with its mutation and pre_sample, every other
+ * field zero (library defaults). */
+int prt_scene_file_mlt(const prt_scene_file* file, prt_mlt_opts* opts);
 void prt_scene_file_free(prt_scene_file* file);
 /* One numeric field as the scene-file reader evaluates it (Expr.cpp
  * expr_eval: + - * / % ^, parentheses, pi, e, sqrt ... datan2, floor, ceil,

@@ -449,6 +449,9 @@ struct prt_scene_file {
   std::vector<std::vector<int32_t>> faces;
   std::string output = "output.ppm";
   double gamma_offset = 0.0;
+  // metropolisTransport and its parameters (scene_env.h:205-208, 309-315)
+  int metropolis = 0;
+  double mutation = 2.0, pre_sample = 0.4;
 };
 
 namespace {
@@ -545,7 +548,10 @@ void load(const std::string& path, prt_scene_file& F) {
     if (R.is("participatingMedia\n")) { R.line(); media = scan_int(R.buf); continue; }
     if (R.is("metropolisTransport\n")) { R.line(); mlt = scan_int(R.buf); continue; }
     if (R.is("energyRedistributionPathTracing\n")) { R.line(); erpt = scan_int(R.buf); continue; }
-    if (R.is("mutation\n") || R.is("pre_sample\n") || R.is("mlt_r1r2\n")) { R.line(); continue; }
+    if (R.is("mutation\n")) { R.line(); F.mutation = one(R.buf); continue; }
+    if (R.is("pre_sample\n")) { R.line(); F.pre_sample = one(R.buf); continue; }
+    // mlt_r1r2 only feeds generate_new_path's disabled local mutation (render.h:207)
+    if (R.is("mlt_r1r2\n")) { R.line(); continue; }
     if (R.is("sensor_response\n")) { R.line(); sensor = std::strtod(R.buf, nullptr); continue; }
     if (R.starts("background_texture ")) unsupported("background_texture");
     if (R.starts("map_coef ") || R.starts("coef ")) continue;
@@ -735,11 +741,17 @@ void load(const std::string& path, prt_scene_file& F) {
   if (const char* s = env("LUMINANCE_CUTOFF")) cutoff = eval1(s, vars);
   if (const char* s = env("METROPOLIS")) mlt = (int)eval1(s, vars);
   if (const char* s = env("ERPT")) erpt = (int)eval1(s, vars);
+  if (const char* s = env("MUTATION")) F.mutation = eval1(s, vars);
+  if (const char* s = env("PRESAMPLE")) F.pre_sample = eval1(s, vars);
   if (const char* s = env("ENVLIGHT_R")) envl[0] = eval1(s, vars);
   if (const char* s = env("ENVLIGHT_G")) envl[1] = eval1(s, vars);
   if (const char* s = env("ENVLIGHT_B")) envl[2] = eval1(s, vars);
-  if (mlt) unsupported("metropolisTransport");
   if (erpt) unsupported("energyRedistributionPathTracing");
+  F.metropolis = mlt;
+  if (mlt) {  // scene_env.h:1946-1956 (tent filter and the 0.25 sensor factor live in prt_render_mlt)
+    if (F.pre_sample <= 0.0) F.pre_sample = 0.4;
+    if (F.mutation <= 0.0) F.mutation = 32.0;
+  }
   if (media) unsupported("participatingMedia");
   if (tent) unsupported("tentfilter");
   if (lens > 0.0) unsupported("LENS");
@@ -812,6 +824,16 @@ const char* prt_scene_file_output(const prt_scene_file* f, double* gamma_offset)
   if (!f) return nullptr;
   if (gamma_offset) *gamma_offset = f->gamma_offset;
   return f->output.c_str();
+}
+
+int prt_scene_file_mlt(const prt_scene_file* f, prt_mlt_opts* opts) {
+  if (!f) return 0;
+  if (opts) {
+    std::memset(opts, 0, sizeof(*opts));
+    opts->mutation = f->mutation;
+    opts->pre_sample = f->pre_sample;
+  }
+  return f->metropolis ? 1 : 0;
 }
 
 void prt_scene_file_free(prt_scene_file* f) { delete f; }

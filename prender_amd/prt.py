@@ -74,6 +74,7 @@ def lib():
     L.prt_scene_file_output.restype = C.c_char_p
     L.prt_scene_file_output.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.prt_scene_file_free.argtypes = [C.c_void_p]
+    L.prt_scene_file_mlt.argtypes = [C.c_void_p, C.POINTER(PrtMltOpts)]
     L.prt_eval_expression.restype = C.c_double
     L.prt_eval_expression.argtypes = [C.c_char_p, C.POINTER(C.c_int)]
     L.prt_image_to_rgb8.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_void_p]
@@ -88,7 +89,7 @@ EXPORTED = ["prt_abi_version", "prt_last_error", "prt_device_info", "prt_scene_c
             "prt_scene_info", "prt_trace_closest", "prt_trace_closest_device", "prt_render", "prt_render_device",
             "prt_render_mlt", "prt_render_mlt_device", "prt_render_batch_device", "prt_tune_batch", "prt_set_face_albedo_device", "prt_set_emission",
             "prt_hold_rr_albedo", "prt_adjoint_device", "prt_scene_file_load", "prt_scene_file_desc",
-            "prt_scene_file_output", "prt_scene_file_free", "prt_eval_expression", "prt_image_to_rgb8",
+            "prt_scene_file_output", "prt_scene_file_mlt", "prt_scene_file_free", "prt_eval_expression", "prt_image_to_rgb8",
             "prt_write_ppm", "prt_write_bmp", "prt_write_hdr"]
 
 

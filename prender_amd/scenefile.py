@@ -66,3 +66,19 @@ def load(path: str) -> SceneDesc:
         return desc
     finally:
         L.prt_scene_file_free(h)
+
+
+def mlt_settings(path: str):
+    """(metropolisTransport on?, {"mutation", "pre_sample"}) of a scene file,
+    with the reference's MLT defaults (scene_env.h:1946-1956); pass the dict
+    to ``prt.Scene.render_mlt``."""
+    L = prt.lib()
+    h = L.prt_scene_file_load(os.fsencode(path))
+    if not h:
+        raise prt.PrtError(f"prt_scene_file_load failed: {L.prt_last_error().decode(errors='replace')}")
+    try:
+        o = prt.PrtMltOpts()
+        on = L.prt_scene_file_mlt(h, C.byref(o))
+        return bool(on), {"mutation": o.mutation, "pre_sample": o.pre_sample}
+    finally:
+        L.prt_scene_file_free(h)
