@@ -41,7 +41,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5"],
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5", "mlt"],
                     help="cfg2 (default) is the metric's configuration; the others are extra measurements "
                          "(cfg4: 64-scene batch vs per-scene renders, cfg5: inverse rendering)")
     ap.add_argument("--no-tune", action="store_true", help="cfg4: skip the batch tuner (library defaults)")
@@ -76,6 +76,9 @@ def main():
         return
     if args.workload == "cfg5":
         print(json.dumps(bench_inverse(args, torch, G, prt, info)))
+        return
+    if args.workload == "mlt":
+        print(json.dumps(bench_mlt(args, torch, G, prt, info)))
         return
     desc, wl_text = workload(G, args.workload)
     W, H, spp = desc.camera.width, desc.camera.height, desc.spp
@@ -194,6 +197,65 @@ def workload(G, name):
                              "1024x768, 64 spp (16x2x2), DEPTH 5 / DEPTH_LIMIT 10, NEE")
     return G.config2(), ("cfg2: quad Cornell box (24 tris) + L6 displaced icosphere (81,920 tris), "
                          "0.25^2 area light, 512x512, 64 spp (16x2x2), DEPTH 5 / DEPTH_LIMIT 8, NEE")
+
+
+def bench_mlt(args, torch, G, prt, info):
+    """Primary-sample-space Metropolis renders (prt_render_mlt_device) of the
+    cfg2 scene: SAMPLING 16 chains of the reference's kind, mutation 4,
+    pre_sample 0.4, as 2^20 chains side by side.  Reports paths traced per
+    second (seed paths + proposals) and the RMSE of the Metropolis image and
+    of a path-traced image with as many paths, both against a 1024 spp
+    path-traced reference."""
+    desc = G.config2()
+    desc.sampling.luminance_cutoff = 0.0
+    W, H = desc.camera.width, desc.camera.height
+    scene = prt.Scene(desc)
+    dev = torch.device("cuda:0")
+    img = torch.empty((H, W, 3), dtype=torch.float32, device=dev)
+    kw = dict(mutation=4.0, pre_sample=0.4, chains=1 << 20)
+    for i in range(args.warmup):
+        scene.render_mlt_device(img.data_ptr(), seed=500 + i, **kw)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        stats.append(scene.render_mlt_device(img.data_ptr(), seed=args.seed + i, **kw))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    paths = sum(s["camera_samples"] for s in stats)
+    props = sum(s["mlt_proposals"] for s in stats)
+    out = {
+        "metric": "MLT paths/s (seed paths + proposals), 1 GPU", "value": paths / elapsed, "unit": "paths/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "mlt: cfg2 scene (512x512, 81,944 triangles, NEE), PSSMLT with SAMPLING 16 x "
+                               "W*H*4 steps over 2^20 chains, pre_sample 0.4", "chains": 1 << 20,
+                   "paths_per_render": paths // args.steps, "parallelism": "1 GPU"},
+        "proposals_per_s": props / elapsed,
+        "accept_rate": sum(s["mlt_accepts"] for s in stats) / max(1, props),
+        "mrays_per_s": sum(s["primary_rays"] + s["extension_rays"] + s["shadow_rays"] for s in stats)
+        / elapsed / 1e6,
+        "device": info["arch"],
+    }
+    if not args.no_rmse:
+        ref = torch.empty_like(img)
+        scene.render_device(ref.data_ptr(), None, seed=999_999, samples=256)
+        pt = torch.empty_like(img)
+        spp_eq = max(1, round(paths / args.steps / (W * H) / 4))  # SUPERSAMPLING 2: 4 rays per sample
+        scene.render_device(pt.data_ptr(), None, seed=77, samples=spp_eq)
+        torch.cuda.synchronize()
+        def blocks(a, k=8):
+            return a.reshape(H // k, k, W // k, k, 3).mean(dim=(1, 3))
+        out["rmse_vs_1024spp"] = {
+            "mlt": float(torch.sqrt(torch.mean((img - ref) ** 2)).item()),
+            "path_tracing_equal_paths": float(torch.sqrt(torch.mean((pt - ref) ** 2)).item()),
+            "mlt_8x8_blocks": float(torch.sqrt(torch.mean((blocks(img) - blocks(ref)) ** 2)).item()),
+            "path_tracing_8x8_blocks": float(torch.sqrt(torch.mean((blocks(pt) - blocks(ref)) ** 2)).item()),
+            "path_tracing_spp": spp_eq * 4,
+            "note": "the Metropolis image blurs pixel edges with the reference's tent filter, which the "
+                    "path-traced reference does not"}
+    return out
 
 
 def bench_batch(args, torch, G, prt, info):
