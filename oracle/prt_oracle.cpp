@@ -14,8 +14,10 @@
 // (proj/light.h:95-155), intersect_scene_ (proj/scene.cpp:83-214),
 // Sphere::intersect (proj/sphere.h:68-103), Plane/UVPlane::intersect
 // (proj/plane.h:86-110, 220-373), Triangle::CheckIntersection2
-// (proj/polygon.h:78-129) and the SAH BVH -> QBVH build and traversal
-// (proj/BVH.cpp:119-348, proj/QBVH.cpp:17-323, proj/BoundingBox.h:137-178).
+// (proj/polygon.h:78-129), the SAH BVH -> QBVH build and traversal
+// (proj/BVH.cpp:119-348, proj/QBVH.cpp:17-323, proj/BoundingBox.h:137-178)
+// and, as the HIP path runs it, primary-sample-space Metropolis light
+// transport (proj/render.h:195-356, 360-780; proj/mlt.h:26-220).
 //
 // Deliberate differences from the reference (each one also applies to the
 // HIP path, so the two stay comparable):
