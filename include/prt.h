@@ -178,8 +178,8 @@ typedef struct prt_stats {
                                      ran (shadow kernels overlap the next depth's closest-hit
                                      kernel on a second stream, so this can be less than
                                      ms_extend + ms_shadow) */
-    uint64_t mlt_proposals;       /* Metropolis renders: proposals evaluated */
-    uint64_t mlt_accepts;         /* Metropolis renders: proposals accepted */
+    uint64_t mlt_proposals;       /* Metropolis / ERPT renders: proposals evaluated */
+    uint64_t mlt_accepts;         /* Metropolis / ERPT renders: proposals accepted */
 } prt_stats;
 
 typedef struct prt_render_opts {
@@ -258,6 +258,30 @@ typedef struct prt_mlt_opts {
  * MLT sensor response (0.25 * SENSOR_RESPONSE, scene_env.h:1945-1956). */
 int prt_render_mlt(prt_scene* scene, const prt_mlt_opts* opts, float* out_rgb, prt_stats* stats);
 int prt_render_mlt_device(prt_scene* scene, const prt_mlt_opts* opts, float* d_rgb, prt_stats* stats);
+
+/* ---- energy redistribution path tracing (replaces Render::RunERPT,
+ *      render.h:786-1123; ERPTSampler, erpt.h:14-88) ------------------------ */
+/* W*H paths estimate the deposition energy ed = L(mean radiance) / mutation
+ * (render.h:831-849); then SUPERSAMPLING^2 initial paths per pixel: a path
+ * that returns a light directly with NEE on is splatted as F / samples, any
+ * other starts floor(u + L / (mutation * ed)) chains of `mutation` steps, each
+ * step a perturbation of the chain's primary samples by up to
+ * +-mutate_distance, accepted with min(1, L'/L), and a deposit of
+ * ed / samples worth of the current path's colour at its pixel (skipped after
+ * 20 deposits in a row on one pixel).  The image is scaled by
+ * 0.25 * SENSOR_RESPONSE (scene_env.h:1970).  Chains run side by side as
+ * wavefront batches of up to 2^20. */
+typedef struct prt_erpt_opts {
+    uint64_t seed;               /* RNG key */
+    double mutation;             /* chain steps (SceneEnv::mutation; <= 0: 32) */
+    double mutate_distance;      /* primary-sample perturbation (erpt.h:107; <= 0: 0.05) */
+    int32_t samples_override;    /* > 0 replaces prt_sampling.samples (the deposit divisor) */
+    int32_t sample_depths;       /* path vertices whose random numbers are primary samples (0: 8, <= 32) */
+    int32_t max_depth;           /* safety cap on path vertices (0 = 128) */
+    int32_t image_f64;           /* 1 = d_rgb is a double array (device entry point) */
+} prt_erpt_opts;
+int prt_render_erpt(prt_scene* scene, const prt_erpt_opts* opts, float* out_rgb, prt_stats* stats);
+int prt_render_erpt_device(prt_scene* scene, const prt_erpt_opts* opts, float* d_rgb, prt_stats* stats);
 
 /* ---- batched rendering of many scenes ------------------------------------- */
 /* No reference counterpart: the reference renders one scene per process

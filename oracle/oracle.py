@@ -58,6 +58,8 @@ def lib():
         L.orc_set_emission.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.orc_mlt_mutate0.restype = C.c_uint32
         L.orc_mlt_mutate0.argtypes = [C.c_uint32, C.c_double]
+        L.orc_render_erpt.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
         L.orc_render_mlt.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_int64,
                                      C.c_int, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
         _lib = L
@@ -129,6 +131,19 @@ class OracleScene:
                                   chains, sample_depths, rgb.ctypes.data, stats, threads)
         assert rc == 0, rc
         keys = ["seeds", "chains", "steps", "proposals", "accepts", "lum_total"]
+        return rgb, dict(zip(keys, [int(x) for x in stats]))
+
+    def render_erpt(self, seed: int, mutation: float = 0.0, mutate_distance: float = 0.0,
+                    samples: Optional[int] = None, sample_depths: int = 0, threads: int = 0):
+        """Energy redistribution render as prt_render_erpt runs it; returns
+        (rgb[H,W,3] float64, stats)."""
+        W, H = self.desc.camera.width, self.desc.camera.height
+        rgb = np.zeros((H, W, 3), dtype=np.float64)
+        stats = (C.c_uint64 * 4)()
+        rc = lib().orc_render_erpt(self.h, seed, mutation, mutate_distance, samples or 0, sample_depths,
+                                   rgb.ctypes.data, stats, threads)
+        assert rc == 0, rc
+        keys = ["initial", "chains", "proposals", "accepts"]
         return rgb, dict(zip(keys, [int(x) for x in stats]))
 
     def radiance_sample(self, seed: int, x: int, y: int, sample: int):

@@ -679,6 +679,7 @@ struct Ctx {
   Adj* ad = nullptr;
   const uint32_t* mlt = nullptr;  // Metropolis proposal's primary samples
   int mlt_depths = 0;
+  bool direct_hit = false;        // the camera ray's diffuse hit on a light was returned (NEE on)
 };
 
 static V3 radiance(Ctx& C, const Ray& ray, int depth, uint32_t branch, const V3& T);
@@ -784,6 +785,7 @@ static V3 brdf_diffuse(Ctx& C, const PathKey& pk, const Ray& ray, const Hit& h, 
     if (dot(h.normal, ray.dir) >= 0.0) return V3();
   }
   if (e.light && (!S.nee || depth == 0)) {
+    if (S.nee && depth == 0) C.direct_hit = true;  // *hit_direct_light = 1, brdf.h:443-452
     if (C.ad && h.entity == C.ad->light) adj_emission(C, T, V3(1.0 / rr, 1.0 / rr, 1.0 / rr));
     return m.emission / rr / (1.0 - 0.0);
   }
@@ -1338,6 +1340,7 @@ namespace orc {
 static const uint32_t MLT_TAG_LARGE = 0x4C415247u, MLT_TAG_MUTATE = 0x4D555441u, MLT_TAG_FRESH = 0x46524553u,
                       MLT_TAG_ACCEPT = 0x41434350u, MLT_TAG_INIT = 0x494E4954u, MLT_TAG_SEED = 0x53454544u;
 static const double MLT_FIX_ONE = 1048576.0, MLT_LUM_ONE = 16777216.0;
+static const uint32_t ERPT_TAG_CHAINS = 0x4E434841u, ERPT_TAG_PERTURB = 0x50455254u, ERPT_TAG_ACCEPT = 0x45414343u;
 
 static uint64_t splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;
@@ -1551,6 +1554,207 @@ int orc_render_mlt(const orc_scene* s, uint64_t seed, double mutation, double pr
     stats[3] = (uint64_t)C * (uint64_t)steps_run;
     stats[4] = accepts;
     stats[5] = total;
+  }
+  return PRT_OK;
+}
+
+// Energy redistribution path tracing as prt_render_erpt runs it: RunERPT's
+// structure (render.h:786-1123) with the published chain (perturbations of
+// the current path, deposits in its colour), in the same batches and Philox
+// streams as the HIP path.  stats: [initial paths, chains, proposals, accepts].
+int orc_render_erpt(const orc_scene* s, uint64_t seed, double mutation, double mutate_distance, int samples_override,
+                    int sample_depths, double* out_rgb, uint64_t* stats, int nthreads) {
+  if (!s || !out_rgb) return PRT_E_INVALID;
+  const Scene& S = s->S;
+  const int W = S.cam.width, H = S.cam.height;
+  const int64_t n_pix = (int64_t)W * H;
+  const int samples = samples_override > 0 ? samples_override : S.samples;
+  if (mutation <= 0.0) mutation = 32.0;
+  const int M = mutation < 1.0 ? 1 : (int)mutation;
+  const double d = mutate_distance > 0.0 ? mutate_distance : 0.05;
+  const int depths = sample_depths > 0 ? sample_depths : 8;
+  if (depths > 32 || samples <= 0) return PRT_E_INVALID;
+  const int ss2 = S.supersamples * S.supersamples;
+  const int64_t n_init = n_pix * ss2;
+  const int stride = MLT_CAMERA_WORDS + 4 * depths;
+  const int64_t cap = (int64_t)1 << 20;
+  const int64_t B = std::min(cap, std::max(n_init, n_pix));
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+  (void)nthreads;
+#endif
+  uint32_t kA[2], kB[2], kC[2];
+  mlt_key(seed, 4, 0, kA);
+  mlt_key(seed, 5, 0, kB);
+  mlt_key(seed, 6, 0, kC);
+  uint64_t launch = 0;
+  auto fresh_words = [&](const uint32_t k[2], uint32_t g, uint32_t* u) {
+    for (int blk = 0; blk < stride / 4; ++blk) philox_at(g, 0u, MLT_TAG_SEED, (uint32_t)blk, k, u + 4 * blk);
+  };
+  auto path = [&](const uint32_t key[2], uint32_t slot, const uint32_t* u, uint32_t& pix, bool* direct) {
+    Ctx Cx;
+    Cx.S = &S;
+    Cx.key[0] = key[0]; Cx.key[1] = key[1];
+    Cx.pixel = slot;
+    Cx.sample = 0;
+    Cx.mlt = u;
+    Cx.mlt_depths = depths;
+    const Ray ray = mlt_camera_ray(S.cam, u, pix);
+    const V3 f = radiance(Cx, ray, 0, 1, V3(1, 1, 1));
+    if (direct) *direct = Cx.direct_hit;
+    return f;
+  };
+  // deposition energy (render.h:831-849): sums of the paths' radiance in
+  // 2^-24 fixed point, as the device sums its 32.32 accumulators >> 8
+  uint64_t sums[3] = {0, 0, 0};
+  for (int64_t first = 0; first < n_pix; first += cap) {
+    const int64_t n = std::min(cap, n_pix - first);
+    uint32_t key[2];
+    mlt_key(seed, 7, launch++, key);
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+      uint64_t loc[3] = {0, 0, 0};
+      std::vector<uint32_t> u((size_t)stride);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+      for (int64_t i = 0; i < n; ++i) {
+        fresh_words(kA, (uint32_t)(first + i), u.data());
+        uint32_t pix;
+        const V3 f = path(key, (uint32_t)i, u.data(), pix, nullptr);
+        const double c[3] = {f.x, f.y, f.z};
+        for (int k = 0; k < 3; ++k) loc[k] += ((uint64_t)(c[k] * 4294967296.0 + 0.5)) >> 8;
+      }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+      for (int k = 0; k < 3; ++k) sums[k] += loc[k];
+    }
+  }
+  const V3 mean((double)sums[0] / MLT_LUM_ONE / (double)n_pix, (double)sums[1] / MLT_LUM_ONE / (double)n_pix,
+                (double)sums[2] / MLT_LUM_ONE / (double)n_pix);
+  const double ed = luminance(mean) / (double)M;
+  const double dep = ed / (double)samples;
+  std::vector<uint64_t> img((size_t)n_pix * 3, 0);
+  uint64_t chain_base = 0, proposals = 0, accepts = 0;
+  std::vector<uint32_t> pu((size_t)B * stride);
+  std::vector<V3> pf((size_t)B);
+  std::vector<uint32_t> ppix((size_t)B);
+  std::vector<uint64_t> first_chain((size_t)B);
+  for (int64_t first = 0; first < n_init; first += cap) {
+    const int64_t n = std::min(cap, n_init - first);
+    uint32_t key[2];
+    mlt_key(seed, 7, launch++, key);
+    std::vector<uint64_t> count((size_t)n, 0);
+    std::vector<char> direct_hit((size_t)n, 0);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+      uint32_t* u = &pu[(size_t)i * stride];
+      fresh_words(kB, (uint32_t)(first + i), u);
+      bool direct = false;
+      pf[(size_t)i] = path(key, (uint32_t)i, u, ppix[(size_t)i], &direct);
+      const double l = luminance(pf[(size_t)i]);
+      if (direct) {
+        direct_hit[(size_t)i] = 1;
+      } else if (l > 0.0 && ed > 0.0) {
+        uint32_t r[4];
+        philox_at((uint32_t)(first + i), 0u, ERPT_TAG_CHAINS, 0u, kB, r);
+        const double c = std::floor(u01_from(r[0]) + l / ((double)M * ed));
+        count[(size_t)i] = c > 0.0 ? (uint64_t)c : 0ull;
+      }
+    }
+    // direct hits: F / samples at their pixel (render.h:927-931)
+    for (int64_t i = 0; i < n; ++i)
+      if (direct_hit[(size_t)i]) mlt_splat(img, ppix[(size_t)i], 1.0 / samples, pf[(size_t)i]);
+    uint64_t run_total = 0;
+    for (int64_t i = 0; i < n; ++i) first_chain[(size_t)i] = (run_total += count[(size_t)i]);
+    const uint64_t n_chains = run_total;
+    for (uint64_t j0 = 0; j0 < n_chains; j0 += (uint64_t)B) {
+      const int64_t nc = (int64_t)std::min<uint64_t>((uint64_t)B, n_chains - j0);
+      const uint32_t base = (uint32_t)(chain_base + j0);
+      std::vector<uint32_t> cur((size_t)nc * stride), pix_y((size_t)nc), pix_now((size_t)nc), run((size_t)nc, 0);
+      std::vector<V3> f_y((size_t)nc);
+      for (int64_t j = 0; j < nc; ++j) {
+        const uint64_t J = j0 + (uint64_t)j;
+        const size_t p = (size_t)(std::upper_bound(first_chain.begin(), first_chain.begin() + n, J) -
+                                  first_chain.begin());
+        std::memcpy(&cur[(size_t)j * stride], &pu[p * stride], sizeof(uint32_t) * stride);
+        f_y[(size_t)j] = pf[p];
+        pix_y[(size_t)j] = ppix[p];
+        const int64_t lp = (first + (int64_t)p) / ss2;  // loop pixel (render.h:971)
+        pix_now[(size_t)j] = (uint32_t)((H - (int)(lp / W) - 1) * W + (int)(lp % W));
+      }
+      for (int step = 0; step < M; ++step) {
+        uint32_t skey[2];
+        mlt_key(seed, 7, launch++, skey);
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+        {
+          std::vector<uint64_t> local((size_t)n_pix * 3, 0);
+          std::vector<uint32_t> prop((size_t)stride);
+          uint64_t acc_local = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+          for (int64_t j = 0; j < nc; ++j) {
+            uint32_t* cu = &cur[(size_t)j * stride];
+            for (int blk = 0; blk < stride / 4; ++blk) {
+              uint32_t r[4];
+              philox_at(base + (uint32_t)j, (uint32_t)step, ERPT_TAG_PERTURB, (uint32_t)blk, kC, r);
+              for (int k = 0; k < 4; ++k) {
+                double x = u01_from(cu[4 * blk + k]) + d * (2.0 * u01_from(r[k]) - 1.0);  // erpt.h:29-34
+                if (x > 1.0) x -= 1.0;
+                if (x < 0.0) x += 1.0;
+                prop[(size_t)(4 * blk + k)] = word_of(x);
+              }
+            }
+            uint32_t pz;
+            const V3 fz = path(skey, (uint32_t)j, prop.data(), pz, nullptr);
+            const double lz = luminance(fz), ly = luminance(f_y[(size_t)j]);
+            const double q = std::min(1.0, ly > 1.0e-16 ? lz / ly : 0.0);
+            uint32_t r4[4];
+            philox_at(base + (uint32_t)j, (uint32_t)step, ERPT_TAG_ACCEPT, 0u, kC, r4);
+            if (q > u01_from(r4[0])) {
+              std::memcpy(cu, prop.data(), sizeof(uint32_t) * stride);
+              f_y[(size_t)j] = fz;
+              pix_y[(size_t)j] = pz;
+              ++acc_local;
+            }
+            if (pix_now[(size_t)j] == pix_y[(size_t)j]) {
+              ++run[(size_t)j];
+            } else {
+              pix_now[(size_t)j] = pix_y[(size_t)j];
+              run[(size_t)j] = 0;
+            }
+            const double l = luminance(f_y[(size_t)j]);
+            if (run[(size_t)j] < 20 && l > 0.0) mlt_splat(local, pix_y[(size_t)j], dep / l, f_y[(size_t)j]);
+          }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+          {
+            for (size_t i = 0; i < img.size(); ++i) img[i] += local[i];
+            accepts += acc_local;
+          }
+        }
+      }
+      proposals += (uint64_t)nc * (uint64_t)M;
+    }
+    chain_base += n_chains;
+  }
+  const double scale = S.sensor_response * 0.25;
+  for (size_t i = 0; i < img.size(); ++i) out_rgb[i] = (double)img[i] * (1.0 / MLT_FIX_ONE) * scale;
+  if (stats) {
+    stats[0] = (uint64_t)n_init;
+    stats[1] = chain_base;
+    stats[2] = proposals;
+    stats[3] = accepts;
   }
   return PRT_OK;
 }

@@ -92,6 +92,7 @@ struct Params {
   // element goes to mlt_pix[c]
   const uint32_t* mlt_u;
   uint32_t* mlt_pix;
+  uint32_t* mlt_flags;  // per chain: 1 if the camera ray returned a light directly (ERPT), or null
   int32_t mlt_stride;
   int32_t mlt_depths;
 };

@@ -168,9 +168,12 @@ void release_stream_workspace(StreamWorkspace* w);
 struct MltWorkspace {
   DevBuf cur, prop, pix_new, pix_old, f_old, img;
   DevBuf seed_f, seed_lum, seed_pix, seed_cdf, tile_sums;
+  // energy redistribution: initial paths of a batch and per-chain run state
+  DevBuf path_u, path_f, path_lum, path_pix, path_direct, path_count, path_first, pix_now, run, sums;
   void release() {
     for (DevBuf* b : {&cur, &prop, &pix_new, &pix_old, &f_old, &img, &seed_f, &seed_lum, &seed_pix, &seed_cdf,
-                      &tile_sums})
+                      &tile_sums, &path_u, &path_f, &path_lum, &path_pix, &path_direct, &path_count, &path_first,
+                      &pix_now, &run, &sums})
       b->release();
   }
 };

@@ -216,9 +216,11 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
       id = hit_id[i];
     }
     C.slot = slot;
+    C.direct = 0u;
     const uint32_t pl = slot / P.spp, k = slot % P.spp;
     VertexAdj va;
     const d3 add = shade_vertex(P, C, o, d, t, id, T, Pw, (uint32_t)(B.pixel_base + pl), k, depth, branch, va);
+    if (primary && P.mlt_flags != nullptr) P.mlt_flags[slot] = C.direct;
     if (ADJ && A.on) {
       const size_t px = sample_image_index(P, B, slot);
       const d3 g = mk(A.dimage[3 * px] * A.scale, A.dimage[3 * px + 1] * A.scale, A.dimage[3 * px + 2] * A.scale);

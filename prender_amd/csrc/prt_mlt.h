@@ -76,5 +76,63 @@ hipError_t launch_mlt_init(const MltChains& C, const MltSeeds& S, hipStream_t s)
 // mean seed luminance, weight = 4*W*H (tent filter pdf), inv_m = 1/steps.
 hipError_t launch_mlt_step(const MltChains& C, const uint64_t* accum, uint32_t iter, double p_large, double b,
                            double weight, double inv_m, uint64_t* ctr, hipStream_t s);
+// ---- energy redistribution path tracing (Render::RunERPT, render.h:786-1123)
+enum : uint32_t {
+  ERPT_TAG_CHAINS = 0x4E434841u,   // chain count of initial path g
+  ERPT_TAG_PERTURB = 0x50455254u,  // perturbation numbers per (chain, step, word block)
+  ERPT_TAG_ACCEPT = 0x45414343u    // acceptance number per (chain, step)
+};
+#define ERPT_MAX_STACK 20  // render.h:969
+
+// Initial paths of one batch: radiance, luminance, image element, chain count.
+struct ErptPaths {
+  double* f;         // [n][3]
+  double* lum;       // [n]
+  uint32_t* pix;     // [n] (Params::mlt_pix of the batch)
+  uint32_t* direct;  // [n] (Params::mlt_flags of the batch)
+  uint64_t* count;   // [n] chains started from the path
+  uint64_t* first;   // [n] inclusive prefix of count
+  uint64_t* tile_sums;
+  uint32_t n;
+};
+
+// One sub-batch of chains.  A chain proposes a perturbation of its current
+// path's primary samples, accepts with min(1, L_z / L_y) and after every step
+// deposits ed / samples worth of its current path's colour at that path's
+// pixel (Cline et al. 2005, as render.h:955-1008 lays the loop out).
+struct ErptChains {
+  uint32_t* cur;      // [n][stride] current path's primary samples
+  uint32_t* prop;     // [n][stride] proposal (Params::mlt_u)
+  uint32_t* pix_new;  // [n] proposal's image element (Params::mlt_pix)
+  uint32_t* pix_y;    // [n] current path's image element
+  uint32_t* pix_now;  // [n] element of the current run of deposits
+  uint32_t* run;      // [n] length of that run (stack_num)
+  double* f_y;        // [n][3] current path's radiance
+  uint64_t* img;      // [W*H*3] image, fixed point (MLT_FIX_ONE)
+  uint32_t n, stride, k0, k1;
+  uint32_t base;      // global index of chain 0 of the sub-batch (Philox counter)
+  double dep;         // ed / samples: the luminance of one deposit
+};
+
+// Sum of the batch's path radiance, fixed point 2^-24, into sums[3].
+hipError_t launch_erpt_sum(const uint64_t* accum, uint32_t n, uint64_t* sums, hipStream_t s);
+// Initial paths first..first+n-1 (global index, row-major over pixels, SS^2
+// per pixel): a path that returned a light directly is splatted as
+// F/samples; any other path with luminance > 0 starts
+// floor(u + L / (mutation * ed)) chains (render.h:927-938).
+hipError_t launch_erpt_paths(const uint64_t* accum, const ErptPaths& E, uint32_t first, uint64_t* img,
+                             double inv_samples, double ed, double mutation, uint32_t k0, uint32_t k1,
+                             hipStream_t s);
+// Chains j0..j0+C.n-1 of the batch's path list: parent path, start state;
+// the run element starts at the parent's loop pixel (render.h:971).
+hipError_t launch_erpt_chain_init(const ErptChains& C, const ErptPaths& E, const uint32_t* path_u, uint64_t j0,
+                                  uint32_t first, int ss2, int width, int height, hipStream_t s);
+// Proposals: every primary sample moved by d * (2u - 1), wrapped (erpt.h:29-34).
+hipError_t launch_erpt_perturb(const ErptChains& C, uint32_t step, double d, hipStream_t s);
+// Acceptance min(1, L_z / L_y) and the deposit, skipped once 20 deposits in a
+// row fell on one pixel (render.h:975-1007).
+hipError_t launch_erpt_step(const ErptChains& C, const uint64_t* accum, uint32_t step, uint64_t* ctr,
+                            hipStream_t s);
+
 // Final image: img * scale, float or double.
 hipError_t launch_mlt_resolve(const uint64_t* img, int64_t n, double scale, void* out, int as_f64, hipStream_t s);

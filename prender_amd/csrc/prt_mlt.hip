@@ -257,6 +257,135 @@ __global__ __launch_bounds__(256) void k_mlt_resolve(const uint64_t* __restrict_
   }
 }
 
+// ---------------------------------------------------------------- ERPT
+// Block sums of the batch's path radiance (accumulators >> 8: 2^-24 fixed
+// point), one atomic per block and channel.
+__global__ __launch_bounds__(256) void k_erpt_sum(const uint64_t* __restrict__ accum, uint32_t n, uint64_t* sums) {
+  __shared__ uint64_t s_sum[3][4];
+  uint64_t v[3] = {0, 0, 0};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    for (int k = 0; k < 3; ++k) v[k] += accum[3 * (size_t)i + k] >> 8;
+  for (int k = 0; k < 3; ++k) {
+    uint64_t x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane_id() == 0) s_sum[k][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const uint64_t t = s_sum[threadIdx.x][0] + s_sum[threadIdx.x][1] + s_sum[threadIdx.x][2] + s_sum[threadIdx.x][3];
+    if (t) atomicAdd((unsigned long long*)&sums[threadIdx.x], (unsigned long long)t);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_erpt_paths(const uint64_t* __restrict__ accum, ErptPaths E, uint32_t first,
+                                                    uint64_t* img, double inv_samples, double ed, double mutation,
+                                                    uint32_t k0, uint32_t k1) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= E.n) return;
+  const double f[3] = {accum_get(accum + 3 * (size_t)i), accum_get(accum + 3 * (size_t)i + 1),
+                       accum_get(accum + 3 * (size_t)i + 2)};
+  const double l = lum3(f);
+  E.f[3 * (size_t)i] = f[0];
+  E.f[3 * (size_t)i + 1] = f[1];
+  E.f[3 * (size_t)i + 2] = f[2];
+  E.lum[i] = l;
+  uint64_t chains = 0;
+  if (E.direct[i]) {
+    splat(img, E.pix[i], inv_samples, f);  // render.h:927-931
+  } else if (l > 0.0 && ed > 0.0) {
+    uint32_t r[4];
+    philox(first + i, 0u, ERPT_TAG_CHAINS, 0u, k0, k1, r);
+    const double c = floor(u01(r[0]) + l / (mutation * ed));  // render.h:938
+    chains = c > 0.0 ? (uint64_t)c : 0ull;
+  }
+  E.count[i] = chains;
+}
+
+__global__ __launch_bounds__(256) void k_erpt_chain_init(ErptChains C, ErptPaths E, const uint32_t* __restrict__ path_u,
+                                                         uint64_t j0, uint32_t first, int ss2, int width, int height) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= C.n) return;
+  const uint64_t J = j0 + j;
+  uint32_t lo = 0, hi = E.n - 1;  // the path whose chain range holds J
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2u;
+    if (E.first[mid] > J) hi = mid;
+    else lo = mid + 1u;
+  }
+  const uint32_t p = lo;
+  const uint4* src = reinterpret_cast<const uint4*>(path_u + (size_t)p * C.stride);
+  uint4* dst = reinterpret_cast<uint4*>(C.cur + (size_t)j * C.stride);
+  for (uint32_t blk = 0; blk < C.stride / 4u; ++blk) dst[blk] = src[blk];
+  C.f_y[3 * (size_t)j] = E.f[3 * (size_t)p];
+  C.f_y[3 * (size_t)j + 1] = E.f[3 * (size_t)p + 1];
+  C.f_y[3 * (size_t)j + 2] = E.f[3 * (size_t)p + 2];
+  C.pix_y[j] = E.pix[p];
+  // the loop pixel of initial path g = (y*W + x)*SS^2 + i (render.h:897-917)
+  const uint32_t lp = (first + p) / (uint32_t)ss2;
+  const int x = (int)(lp % (uint32_t)width), y = (int)(lp / (uint32_t)width);
+  C.pix_now[j] = (uint32_t)((height - y - 1) * width + x);
+  C.run[j] = 0u;
+}
+
+__global__ __launch_bounds__(256) void k_erpt_perturb(ErptChains C, uint32_t step, double d) {
+  const uint32_t blocks = C.stride / 4u;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)C.n * blocks) return;
+  const uint32_t j = (uint32_t)(t / blocks), blk = (uint32_t)(t % blocks);
+  const uint4 v = reinterpret_cast<const uint4*>(C.cur + (size_t)j * C.stride)[blk];
+  uint32_t r[4];
+  philox(C.base + j, step, ERPT_TAG_PERTURB, blk, C.k0, C.k1, r);
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  uint32_t o[4];
+  for (int k = 0; k < 4; ++k) {
+    double x = u01(w[k]) + d * (2.0 * u01(r[k]) - 1.0);  // erpt.h:29-34
+    if (x > 1.0) x -= 1.0;
+    if (x < 0.0) x += 1.0;
+    o[k] = word_of(x);
+  }
+  reinterpret_cast<uint4*>(C.prop + (size_t)j * C.stride)[blk] = make_uint4(o[0], o[1], o[2], o[3]);
+}
+
+__global__ __launch_bounds__(256) void k_erpt_step(ErptChains C, const uint64_t* __restrict__ accum, uint32_t step,
+                                                   uint64_t* ctr) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  bool acc = false;
+  if (j < C.n) {
+    const double fz[3] = {accum_get(accum + 3 * (size_t)j), accum_get(accum + 3 * (size_t)j + 1),
+                          accum_get(accum + 3 * (size_t)j + 2)};
+    double fy[3] = {C.f_y[3 * (size_t)j], C.f_y[3 * (size_t)j + 1], C.f_y[3 * (size_t)j + 2]};
+    const double lz = lum3(fz), ly = lum3(fy);
+    const double q = fmin(1.0, ly > 1.0e-16 ? lz / ly : 0.0);  // render.h:978-980
+    uint32_t r[4];
+    philox(C.base + j, step, ERPT_TAG_ACCEPT, 0u, C.k0, C.k1, r);
+    uint32_t py = C.pix_y[j];
+    if (q > u01(r[0])) {
+      acc = true;
+      const uint4* src = reinterpret_cast<const uint4*>(C.prop + (size_t)j * C.stride);
+      uint4* dst = reinterpret_cast<uint4*>(C.cur + (size_t)j * C.stride);
+      for (uint32_t blk = 0; blk < C.stride / 4u; ++blk) dst[blk] = src[blk];
+      for (int k = 0; k < 3; ++k) {
+        fy[k] = fz[k];
+        C.f_y[3 * (size_t)j + k] = fz[k];
+      }
+      py = C.pix_new[j];
+      C.pix_y[j] = py;
+    }
+    uint32_t run = C.run[j];
+    if (C.pix_now[j] == py) {
+      ++run;
+    } else {
+      C.pix_now[j] = py;
+      run = 0;
+    }
+    C.run[j] = run;
+    const double l = lum3(fy);
+    if (run < ERPT_MAX_STACK && l > 0.0) splat(C.img, py, C.dep / l, fy);
+  }
+  const uint64_t n_acc = __popcll(__ballot(acc));
+  if (lane_id() == 0 && n_acc) atomicAdd((unsigned long long*)&ctr[CTR_MLT_ACCEPT], (unsigned long long)n_acc);
+}
+
 inline unsigned grid_of(uint64_t n, unsigned block = 256) { return (unsigned)((n + block - 1) / block); }
 
 }  // namespace
@@ -314,5 +443,45 @@ hipError_t launch_mlt_resolve(const uint64_t* img, int64_t n, double scale, void
   if (n <= 0) return hipSuccess;
   const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);
   hipLaunchKernelGGL(k_mlt_resolve, dim3(g), dim3(256), 0, s, img, n, scale, out, as_f64);
+  return hipGetLastError();
+}
+
+hipError_t launch_erpt_sum(const uint64_t* accum, uint32_t n, uint64_t* sums, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const unsigned g = (unsigned)std::min<uint64_t>(grid_of(n), 2048);
+  hipLaunchKernelGGL(k_erpt_sum, dim3(g), dim3(256), 0, s, accum, n, sums);
+  return hipGetLastError();
+}
+
+hipError_t launch_erpt_paths(const uint64_t* accum, const ErptPaths& E, uint32_t first, uint64_t* img,
+                             double inv_samples, double ed, double mutation, uint32_t k0, uint32_t k1,
+                             hipStream_t s) {
+  if (E.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_erpt_paths, dim3(grid_of(E.n)), dim3(256), 0, s, accum, E, first, img, inv_samples, ed,
+                     mutation, k0, k1);
+  return hipGetLastError();
+}
+
+hipError_t launch_erpt_chain_init(const ErptChains& C, const ErptPaths& E, const uint32_t* path_u, uint64_t j0,
+                                  uint32_t first, int ss2, int width, int height, hipStream_t s) {
+  if (C.n == 0) return hipSuccess;
+  if (E.n == 0 || C.stride % 4u != 0 || ss2 <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_erpt_chain_init, dim3(grid_of(C.n)), dim3(256), 0, s, C, E, path_u, j0, first, ss2, width,
+                     height);
+  return hipGetLastError();
+}
+
+hipError_t launch_erpt_perturb(const ErptChains& C, uint32_t step, double d, hipStream_t s) {
+  if (C.stride % 4u != 0) return hipErrorInvalidValue;
+  const uint64_t threads = (uint64_t)C.n * (C.stride / 4u);
+  if (threads == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_erpt_perturb, dim3(grid_of(threads)), dim3(256), 0, s, C, step, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_erpt_step(const ErptChains& C, const uint64_t* accum, uint32_t step, uint64_t* ctr,
+                            hipStream_t s) {
+  if (C.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_erpt_step, dim3(grid_of(C.n)), dim3(256), 0, s, C, accum, step, ctr);
   return hipGetLastError();
 }

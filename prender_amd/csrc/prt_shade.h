@@ -54,6 +54,8 @@ struct ShadeCtx {
   uint32_t off, cap;
   uint32_t slot;
   uint32_t truncated, n_ext, pickups, dropped;
+  uint32_t direct;  // set when a camera ray's diffuse hit on a light is returned with NEE on
+                    // (BRDF_Diffuse's *hit_direct_light = 1, brdf.h:443-452)
 };
 
 // What the adjoint needs to know about one shaded vertex.
@@ -173,6 +175,7 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
     // BRDF_Diffuse, brdf.h:408-700
     if (e.light && e.type == ENT_UVPLANE && dot(n, d) >= 0.0) return add;
     if (e.light) {
+      if (P.nee && depth == 0) C.direct = 1u;
       if (!P.nee || depth == 0) {
         add = add + vmul(T, E / rr / (1.0 - 0.0));
         va.own = vmul(T, E / rr);

@@ -310,6 +310,7 @@ __global__ __launch_bounds__(256) void k_stream_shade(Stream st, RayQ qin, const
   C.off = wave * st.slab_ray;
   C.cap = st.slab_ray;
   C.truncated = C.n_ext = C.pickups = C.dropped = 0;
+  C.direct = 0;
   RunLog log{st.ray_runs + (size_t)wave * st.max_runs, st.max_runs, 0u, 0u, C.off, 0u, -1};
   uint32_t c, c_end;
   wave_range(n_chunks, c, c_end);

@@ -15,7 +15,7 @@ from typing import Optional
 
 import numpy as np
 
-from .scene import (CDesc, HIT_DTYPE, PrtBatchOpts, PrtMltOpts, PrtRenderOpts, PrtSceneDesc, PrtStats, RAY_DTYPE,
+from .scene import (CDesc, HIT_DTYPE, PrtBatchOpts, PrtErptOpts, PrtMltOpts, PrtRenderOpts, PrtSceneDesc, PrtStats, RAY_DTYPE,
                     SceneDesc)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -56,6 +56,8 @@ def lib():
     L.prt_render.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_void_p, C.POINTER(PrtStats)]
     L.prt_render_device.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_void_p,
                                     C.POINTER(PrtStats)]
+    L.prt_render_erpt.argtypes = [C.c_void_p, C.POINTER(PrtErptOpts), C.c_void_p, C.POINTER(PrtStats)]
+    L.prt_render_erpt_device.argtypes = [C.c_void_p, C.POINTER(PrtErptOpts), C.c_void_p, C.POINTER(PrtStats)]
     L.prt_render_mlt.argtypes = [C.c_void_p, C.POINTER(PrtMltOpts), C.c_void_p, C.POINTER(PrtStats)]
     L.prt_render_mlt_device.argtypes = [C.c_void_p, C.POINTER(PrtMltOpts), C.c_void_p, C.POINTER(PrtStats)]
     L.prt_render_batch_device.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(PrtBatchOpts),
@@ -87,7 +89,7 @@ def lib():
 
 EXPORTED = ["prt_abi_version", "prt_last_error", "prt_device_info", "prt_scene_create", "prt_scene_destroy",
             "prt_scene_info", "prt_trace_closest", "prt_trace_closest_device", "prt_render", "prt_render_device",
-            "prt_render_mlt", "prt_render_mlt_device", "prt_render_batch_device", "prt_tune_batch", "prt_set_face_albedo_device", "prt_set_emission",
+            "prt_render_mlt", "prt_render_mlt_device", "prt_render_erpt", "prt_render_erpt_device", "prt_render_batch_device", "prt_tune_batch", "prt_set_face_albedo_device", "prt_set_emission",
             "prt_hold_rr_albedo", "prt_adjoint_device", "prt_scene_file_load", "prt_scene_file_desc",
             "prt_scene_file_output", "prt_scene_file_mlt", "prt_scene_file_free", "prt_eval_expression", "prt_image_to_rgb8",
             "prt_write_ppm", "prt_write_bmp", "prt_write_hdr"]
@@ -129,6 +131,19 @@ def mlt_opts(seed: int = 0, mutation: float = 0.0, pre_sample: float = 0.0, p_la
     o.mutations_per_chain = int(mutations_per_chain)
     o.samples_override = int(samples or 0)
     o.chains = int(chains)
+    o.sample_depths = int(sample_depths)
+    o.max_depth = int(max_depth)
+    o.image_f64 = 1 if image_f64 else 0
+    return o
+
+
+def erpt_opts(seed: int = 0, mutation: float = 0.0, mutate_distance: float = 0.0, samples: Optional[int] = None,
+              sample_depths: int = 0, max_depth: int = 0, image_f64: bool = False) -> PrtErptOpts:
+    o = PrtErptOpts()
+    o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    o.mutation = float(mutation)
+    o.mutate_distance = float(mutate_distance)
+    o.samples_override = int(samples or 0)
     o.sample_depths = int(sample_depths)
     o.max_depth = int(max_depth)
     o.image_f64 = 1 if image_f64 else 0
@@ -223,6 +238,24 @@ class Scene:
         o = mlt_opts(seed, **kw)
         check(lib().prt_render_mlt_device(self.h, C.byref(o), C.c_void_p(rgb_ptr), C.byref(st)),
               "prt_render_mlt_device")
+        return st.as_dict()
+
+    # ---- energy redistribution path tracing (Render::RunERPT, render.h:786-1123)
+    def render_erpt(self, seed: int = 0, **kw):
+        """Returns (rgb[H,W,3], stats); keyword arguments are the
+        prt_erpt_opts fields (erpt_opts)."""
+        H, W = self.height, self.width
+        rgb = np.zeros((H, W, 3), dtype=np.float32)
+        st = PrtStats()
+        o = erpt_opts(seed, **kw)
+        check(lib().prt_render_erpt(self.h, C.byref(o), rgb.ctypes.data, C.byref(st)), "prt_render_erpt")
+        return rgb, st.as_dict()
+
+    def render_erpt_device(self, rgb_ptr: int, seed: int = 0, **kw):
+        st = PrtStats()
+        o = erpt_opts(seed, **kw)
+        check(lib().prt_render_erpt_device(self.h, C.byref(o), C.c_void_p(rgb_ptr), C.byref(st)),
+              "prt_render_erpt_device")
         return st.as_dict()
 
     # ---- differentiable parameters (per-face albedo, light emission)

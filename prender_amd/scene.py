@@ -110,6 +110,12 @@ class PrtMltOpts(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
 
 
+class PrtErptOpts(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("mutation", C.c_double), ("mutate_distance", C.c_double),
+                ("samples_override", C.c_int32), ("sample_depths", C.c_int32), ("max_depth", C.c_int32),
+                ("image_f64", C.c_int32)]
+
+
 class PrtBatchOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("tile_samples", C.c_int32), ("inflight_tiles", C.c_int32),
                 ("stack_lds", C.c_int32), ("trav_blocks_per_cu", C.c_int32), ("use_graph", C.c_int32),

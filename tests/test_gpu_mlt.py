@@ -122,3 +122,66 @@ def test_mlt_device_entry_and_errors(prt):
         s.render_mlt(seed=3, chains=(1 << 20) + 1)
     with pytest.raises(prt.PrtError):
         s.render_mlt(seed=3, sample_depths=33)
+
+
+# ---------------------------------------------------------------- ERPT
+def test_erpt_constant_radiance_equals_oracle(prt):
+    d = empty_scene(24, 16, (0.25, 0.5, 0.75), 2.0)
+    print("erpt constant 24x16 mutation 12 samples 3 seed 5", flush=True)
+    g, sg = prt.Scene(d).render_erpt(seed=5, mutation=12, samples=3)
+    o, so = OracleScene(d).render_erpt(seed=5, mutation=12, samples=3)
+    assert sg["mlt_proposals"] == so["proposals"] == 24 * 16 * 4 * 12
+    assert sg["mlt_accepts"] == so["accepts"]
+    np.testing.assert_array_equal(g, o.astype(np.float32))
+
+
+def test_erpt_common_random_numbers_match_oracle(prt):
+    W = H = 32
+    d = G.config2(width=W, height=H, samples=4, level=3)
+    d.sampling.luminance_cutoff = 0.0
+    kw = dict(seed=12, mutation=16, samples=1)
+    print("erpt CRN cfg2 32x32 L3", kw, flush=True)
+    g, sg = prt.Scene(d).render_erpt(**kw)
+    o, so = OracleScene(d).render_erpt(**kw)
+    close = np.abs(g - o) <= 1e-5 * np.maximum(np.abs(o), 1e-3)
+    rel = np.abs(g - o).sum() / np.abs(o).sum()
+    print("ERPT CRN: pixels within 1e-5:", close.mean(), "total rel diff", rel, "chains", sg["mlt_proposals"] // 16,
+          so["chains"], "accepts", sg["mlt_accepts"], so["accepts"], flush=True)
+    assert sg["mlt_proposals"] == so["proposals"]
+    assert abs(sg["mlt_accepts"] / so["accepts"] - 1.0) < 1e-3
+    assert close.mean() > 0.99
+    assert rel < 1e-4
+
+
+def test_erpt_matches_path_tracing(prt):
+    W = H = 64
+    d = G.config2(width=W, height=H, samples=4, level=4)
+    d.sampling.luminance_cutoff = 0.0
+    d.sampling.supersamples = 6
+    s = prt.Scene(d)
+    print("erpt vs pt cfg2 64x64 L4 SS6", flush=True)
+    pt, _, _ = s.render(seed=5, samples=16)
+    e, st = s.render_erpt(seed=8, mutation=32, samples=9)
+    k = 8
+    bp = pt.reshape(H // k, k, W // k, k, 3).mean(axis=(1, 3))
+    be = e.reshape(H // k, k, W // k, k, 3).mean(axis=(1, 3))
+    rel = np.abs(be - bp).sum() / np.abs(bp).sum()
+    shape = np.abs(be / lum(e).mean() - bp / lum(pt).mean()).sum() / np.abs(bp / lum(pt).mean()).sum()
+    print("erpt vs pt: mean", lum(e).mean(), lum(pt).mean(), "block rel", rel, "shape", shape, st, flush=True)
+    assert abs(lum(e).mean() / lum(pt).mean() - 1.0) < 0.12
+    assert rel < 0.15 and shape < 0.12
+    assert st["camera_samples"] == W * H + W * H * 36 + st["mlt_proposals"]
+
+
+def test_erpt_device_entry(prt):
+    import torch
+    d = G.config1(width=16, height=12, samples=2)
+    d.sampling.luminance_cutoff = 0.0
+    s = prt.Scene(d)
+    h, _ = s.render_erpt(seed=3, mutation=8)
+    out = torch.empty((12, 16, 3), dtype=torch.float32, device="cuda")
+    s.render_erpt_device(out.data_ptr(), seed=3, mutation=8)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), h)
+    with pytest.raises(prt.PrtError):
+        s.render_erpt(seed=3, sample_depths=40)

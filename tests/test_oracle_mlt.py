@@ -158,3 +158,49 @@ def test_mlt_matches_path_tracing():
     assert shape < 0.07, shape
     # Metropolis acceptance of this scene (about 42 %)
     assert 0.3 < st["accepts"] / st["proposals"] < 0.6
+
+
+# ---------------------------------------------------------------- ERPT
+def test_erpt_constant_radiance_closed_form():
+    # every path returns the background: ed = L(bg) / mutation, each initial
+    # path starts exactly one chain (floor(u + 1)), every proposal is
+    # accepted and every step deposits ed / samples worth of bg (runs stay
+    # below 20 with 12 steps), so the image total is closed-form
+    W, H, M, samples = 16, 12, 12, 3
+    bg = np.array([0.25, 0.5, 0.75])
+    d = empty_scene(W, H, tuple(bg), sensor=2.0)
+    o = OracleScene(d)
+    img, st = o.render_erpt(seed=5, mutation=M, samples=samples)
+    n_init = W * H * 4
+    assert st["initial"] == n_init and st["chains"] == n_init
+    assert st["proposals"] == st["accepts"] == n_init * M
+    L = float(lum(bg))
+    ed = L / M
+    total = np.array([n_init * M * int(ed / samples / L * bg[k] * 2.0 ** 20 + 0.5) for k in range(3)], np.float64)
+    want = total * 2.0 ** -20 * 2.0 * 0.25
+    np.testing.assert_allclose(img.reshape(-1, 3).sum(axis=0), want, rtol=1e-12)
+    # = W*H * bg * SS^2 * 0.25 * sensor / samples, up to the 2^-20 rounding of each deposit
+    np.testing.assert_allclose(img.reshape(-1, 3).sum(axis=0), W * H * bg * 4 * 0.25 * 2.0 / samples, rtol=1e-3)
+
+
+def test_erpt_matches_path_tracing():
+    # SUPERSAMPLING 6 gives 36 initial paths per pixel; the reference divides
+    # deposits by SAMPLING, so SAMPLING 9 (= 36 * 0.25) makes the image the
+    # pixel radiance.  Chains are capped at 20 deposits in a row on one pixel
+    # (render.h:969-1007), a small downward bias on flat regions.
+    W = H = 48
+    d = G.config2(width=W, height=H, samples=4, level=3)
+    d.sampling.luminance_cutoff = 0.0
+    d.sampling.supersamples = 6
+    o = OracleScene(d)
+    pt, _, _ = o.render(seed=5, samples=16)
+    e, st = o.render_erpt(seed=8, mutation=32, samples=9)
+    assert st["initial"] == W * H * 36 and st["proposals"] == st["chains"] * 32
+    assert abs(lum(e).mean() / lum(pt).mean() - 1.0) < 0.12
+    k = 8
+    bp = pt.reshape(H // k, k, W // k, k, 3).mean(axis=(1, 3))
+    be = e.reshape(H // k, k, W // k, k, 3).mean(axis=(1, 3))
+    assert np.abs(be - bp).sum() / np.abs(bp).sum() < 0.15
+    shape = np.abs(be / lum(e).mean() - bp / lum(pt).mean()).sum() / np.abs(bp / lum(pt).mean()).sum()
+    assert shape < 0.12, shape
+    assert 0.3 < st["accepts"] / st["proposals"] < 0.8
