@@ -355,11 +355,13 @@ prt_scene_file* prt_scene_file_load(const char* path);
 const prt_scene_desc* prt_scene_file_desc(const prt_scene_file* file);
 /* OUTPUT (default "output.ppm") and gamma_offset of the file. */
 const char* prt_scene_file_output(const prt_scene_file* file, double* gamma_offset);
-/* The file's Metropolis switches (metropolisTransport, mutation, pre_sample
- * and the METROPOLIS / MUTATION / PRESAMPLE overrides, scene_env.h:642-663,
- * 1908-1911, 1946-1956): returns 1 when the file asks for RunMLT, else 0, and
- * fills *opts (if not NULL) with its mutation and pre_sample, every other
- * field zero (library defaults). */
+/* The file's Markov-chain switches (metropolisTransport,
+ * energyRedistributionPathTracing, mutation, pre_sample and the METROPOLIS /
+ * ERPT / MUTATION / PRESAMPLE overrides, scene_env.h:642-672, 1908-1911,
+ * 1946-1968): returns 2 when the file asks for RunERPT (tested first, as
+ * main.cpp:189-196 does), 1 for RunMLT, else 0, and fills *opts (if not
+ * NULL) with its mutation and pre_sample, every other field zero (library
+ * defaults); an ERPT render takes opts->mutation as its chain length. */
 int prt_scene_file_mlt(const prt_scene_file* file, prt_mlt_opts* opts);
 void prt_scene_file_free(prt_scene_file* file);
 /* One numeric field as the scene-file reader evaluates it (Expr.cpp

@@ -450,7 +450,7 @@ struct prt_scene_file {
   std::string output = "output.ppm";
   double gamma_offset = 0.0;
   // metropolisTransport and its parameters (scene_env.h:205-208, 309-315)
-  int metropolis = 0;
+  int metropolis = 0, erpt = 0;
   double mutation = 2.0, pre_sample = 0.4;
 };
 
@@ -746,9 +746,9 @@ void load(const std::string& path, prt_scene_file& F) {
   if (const char* s = env("ENVLIGHT_R")) envl[0] = eval1(s, vars);
   if (const char* s = env("ENVLIGHT_G")) envl[1] = eval1(s, vars);
   if (const char* s = env("ENVLIGHT_B")) envl[2] = eval1(s, vars);
-  if (erpt) unsupported("energyRedistributionPathTracing");
   F.metropolis = mlt;
-  if (mlt) {  // scene_env.h:1946-1956 (tent filter and the 0.25 sensor factor live in prt_render_mlt)
+  F.erpt = erpt;
+  if (mlt || erpt) {  // scene_env.h:1946-1968 (tent filter and the 0.25 sensor factor live in the renderers)
     if (F.pre_sample <= 0.0) F.pre_sample = 0.4;
     if (F.mutation <= 0.0) F.mutation = 32.0;
   }
@@ -833,7 +833,7 @@ int prt_scene_file_mlt(const prt_scene_file* f, prt_mlt_opts* opts) {
     opts->mutation = f->mutation;
     opts->pre_sample = f->pre_sample;
   }
-  return f->metropolis ? 1 : 0;
+  return f->erpt ? 2 : (f->metropolis ? 1 : 0);  // main.cpp:189-196 tests ERPT first
 }
 
 void prt_scene_file_free(prt_scene_file* f) { delete f; }

@@ -69,16 +69,18 @@ def load(path: str) -> SceneDesc:
 
 
 def mlt_settings(path: str):
-    """(metropolisTransport on?, {"mutation", "pre_sample"}) of a scene file,
-    with the reference's MLT defaults (scene_env.h:1946-1956); pass the dict
-    to ``prt.Scene.render_mlt``."""
+    """(mode, {"mutation", "pre_sample"}) of a scene file: mode 2 = ERPT
+    (energyRedistributionPathTracing), 1 = MLT (metropolisTransport), 0 =
+    path tracing, with the reference's defaults (scene_env.h:1946-1968);
+    pass the dict to ``prt.Scene.render_mlt`` (or its mutation to
+    ``render_erpt``)."""
     L = prt.lib()
     h = L.prt_scene_file_load(os.fsencode(path))
     if not h:
         raise prt.PrtError(f"prt_scene_file_load failed: {L.prt_last_error().decode(errors='replace')}")
     try:
         o = prt.PrtMltOpts()
-        on = L.prt_scene_file_mlt(h, C.byref(o))
-        return bool(on), {"mutation": o.mutation, "pre_sample": o.pre_sample}
+        mode = L.prt_scene_file_mlt(h, C.byref(o))
+        return int(mode), {"mutation": o.mutation, "pre_sample": o.pre_sample}
     finally:
         L.prt_scene_file_free(h)

@@ -205,7 +205,6 @@ def test_obj_mesh_faces_transforms_and_colours(tmp_path):
 
 
 @pytest.mark.parametrize("text,word", [
-    ("energyRedistributionPathTracing\n1\n", "energyRedistributionPathTracing"),
     ("participatingMedia\n1\n", "participatingMedia"),
     ("LENS\n1 100\n", "LENS"),
     ("OBJECT\nplane 0 0 0\nnormal 0 1 0\ncolor 1 1 1\n\n", "plane"),
@@ -225,12 +224,14 @@ def test_metropolis_keywords(tmp_path, monkeypatch):
     monkeypatch.delenv("PRESAMPLE", raising=False)
     monkeypatch.delenv("METROPOLIS", raising=False)
     on = write(tmp_path, "m.txt", "metropolisTransport\n1\nmutation\n0\npre_sample\n0.25*2\nmlt_r1r2\n0.1 0.2\n")
-    assert scenefile.mlt_settings(on) == (True, {"mutation": 32.0, "pre_sample": 0.5})
+    assert scenefile.mlt_settings(on) == (1, {"mutation": 32.0, "pre_sample": 0.5})
     off = write(tmp_path, "n.txt", "mutation\n5\n")
-    assert scenefile.mlt_settings(off) == (False, {"mutation": 5.0, "pre_sample": 0.4})
+    assert scenefile.mlt_settings(off) == (0, {"mutation": 5.0, "pre_sample": 0.4})
+    both = write(tmp_path, "e.txt", "metropolisTransport\n1\nenergyRedistributionPathTracing\n1\n")
+    assert scenefile.mlt_settings(both) == (2, {"mutation": 2.0, "pre_sample": 0.4})
     monkeypatch.setenv("METROPOLIS", "1")
     monkeypatch.setenv("MUTATION", "8")
-    assert scenefile.mlt_settings(off) == (True, {"mutation": 8.0, "pre_sample": 0.4})
+    assert scenefile.mlt_settings(off) == (1, {"mutation": 8.0, "pre_sample": 0.4})
 
 
 def test_commented_object_blocks_are_skipped(tmp_path):
