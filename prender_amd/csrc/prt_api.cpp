@@ -28,6 +28,9 @@
 #ifndef PRT_TRAV_BLOCKS_PER_CU
 #define PRT_TRAV_BLOCKS_PER_CU 8
 #endif
+#ifndef PRT_SHADOW_BLOCKS_PER_CU
+#define PRT_SHADOW_BLOCKS_PER_CU PRT_TRAV_BLOCKS_PER_CU
+#endif
 
 namespace prt {
 
@@ -349,7 +352,7 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   // traversal stack overflow: one area per thread of the largest grid
   s->max_trav_blocks = std::max(1, s->cu_count) * PRT_TRAV_BLOCKS_PER_CU;
   const int64_t ovf_threads = (int64_t)s->max_trav_blocks * 256;
-  if ((rc = dev_alloc(s->ovf, (size_t)ovf_threads * 2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * sizeof(uint32_t))))
+  if ((rc = dev_alloc(s->ovf, (size_t)PRT_OVF_REGIONS * ovf_threads * PRT_OVF_WORDS * sizeof(uint32_t))))
     return rc;
   HIPCHK(hipStreamSynchronize(st));
 
@@ -616,6 +619,8 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
   RayQ q[2] = {ray_queue(w.q[0], w.queue_cap), ray_queue(w.q[1], w.queue_cap)};
   ShadowQ sq = shadow_queue(w.sq, w.shadow_cap);
   const int tb = s->max_trav_blocks;
+  // the shadow grid (on the second stream) may leave room on each CU
+  const int tb_sh = std::max(1, s->cu_count) * PRT_SHADOW_BLOCKS_PER_CU;
   const bool nee = P.nee && P.n_lights > 0;
   const uint32_t W = PRT_SLAB_WAVES;
   const size_t fetch_launch = (size_t)PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE;
@@ -650,7 +655,7 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
       if ((rc = tm.end_side(st_sh, T_NEE))) return rc;
       HIPCHK(hipEventRecord(s->ev_nee[k], st_sh));
       if ((rc = tm.begin_side(st_sh))) return rc;
-      HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st_sh));
+      HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb_sh, st_sh));
       if ((rc = tm.end_side(st_sh, T_SHADOW))) return rc;
       HIPCHK(hipEventRecord(s->ev_shadow, st_sh));
       return PRT_OK;
@@ -665,7 +670,7 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
       HIPCHK(hipStreamWaitEvent(st_sh, s->ev_shade, 0));
       if ((rc = tm.begin_side(st_sh))) return rc;
     }
-    HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb, st_sh));
+    HIPCHK(launch_shadow(P, B, sq, chunks(2), f, accum, A, ctr, tb_sh, st_sh));
     if (overlap) {
       if ((rc = tm.end_side(st_sh, T_SHADOW))) return rc;
       HIPCHK(hipEventRecord(s->ev_shadow, st_sh));

@@ -188,7 +188,7 @@ int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, const StreamG
   if ((rc = dev_alloc(w.sh_desc, (size_t)g.max_sh_desc * sizeof(ChunkDesc)))) return rc;
   if ((rc = dev_alloc(w.fetch, (size_t)3 * PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE * sizeof(uint32_t)))) return rc;
   const int64_t ovf_threads = (int64_t)std::max(1, lead->cu_count) * kMaxTravBlocksPerCu * 256;
-  if ((rc = dev_alloc(w.ovf, (size_t)ovf_threads * 2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * sizeof(uint32_t))))
+  if ((rc = dev_alloc(w.ovf, (size_t)PRT_OVF_REGIONS * ovf_threads * PRT_OVF_WORDS * sizeof(uint32_t))))
     return rc;
   w.ovf_threads = ovf_threads;
   if (!w.h_ctl) HIPCHK(hipHostMalloc((void**)&w.h_ctl, 2 * sizeof(StreamCtl), hipHostMallocDefault));

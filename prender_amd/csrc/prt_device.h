@@ -23,6 +23,10 @@
 // smallest LDS stack depth any kernel variant uses (sizes the overflow area)
 #define PRT_STACK_LDS_MIN 8
 #define PRT_STACK_TOTAL 128
+// words of one thread's traversal-stack overflow area, and the number of
+// such areas per thread (closest-hit and shadow kernels run concurrently)
+#define PRT_OVF_WORDS (2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN))
+#define PRT_OVF_REGIONS 2
 
 struct __attribute__((aligned(16))) DevNode {
   float lo_x[4], lo_y[4], lo_z[4];

@@ -29,11 +29,16 @@
 #define PRT_WAVES_NEE 0
 #endif
 #define PRT_WAVES_ATTR(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1)))
+// threads of the one-block chunk-list kernel (256 fits beside the other
+// stream's traversal blocks sooner than 1024)
+#ifndef PRT_INDEX_THREADS
+#define PRT_INDEX_THREADS 1024
+#endif
 
 // ------------------------------------------------------------ trace API kernel
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_trace(
     Params P, const prt_ray_dev* __restrict__ rays, int64_t n, prt_hit_dev* __restrict__ hits, uint64_t* ctr) {
-  PRT_DECLARE_STACK(S, P);
+  PRT_DECLARE_STACK(S, P, 0);
   TravStats ts{0, 0, 0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const prt_ray_dev r = rays[i];
@@ -84,7 +89,7 @@ __device__ __forceinline__ size_t sample_image_index(const Params& P, const Batc
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_extend(
     Params P, RayQ q, ChunkList cl, uint32_t* fetch, double* __restrict__ hit_t, int32_t* __restrict__ hit_id,
     uint64_t* ctr) {
-  PRT_DECLARE_STACK(S, P);
+  PRT_DECLARE_STACK(S, P, 0);
   const uint32_t n_chunks = *cl.n;
   TravStats ts{0, 0, 0};
   traverse_pool<false>(
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_exte
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_shadow(
     Params P, Batch B, ShadowQ q, ChunkList cl, uint32_t* fetch, uint64_t* __restrict__ accum, Adjoint A,
     uint64_t* ctr) {
-  PRT_DECLARE_STACK(S, P);
+  PRT_DECLARE_STACK(S, P, 1);
   const uint32_t n_chunks = *cl.n;
   TravStats ts{0, 0, 0};
   double ge[3] = {0.0, 0.0, 0.0};
@@ -331,14 +336,16 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
 // entries (and adds the entries to ctr[ctr_entries] unless that is < 0).
 // More chunks than max_chunks (never, with the host's sizing) are
 // dropped and counted.
-__global__ __launch_bounds__(1024) void k_slab_index(const uint2* __restrict__ slabs, uint32_t n_slabs,
-                                                     uint2* __restrict__ desc, uint32_t max_chunks,
-                                                     uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr,
-                                                     int ctr_entries) {
+template <int NT>
+__global__ __launch_bounds__(NT) void k_slab_index(const uint2* __restrict__ slabs, uint32_t n_slabs,
+                                                   uint2* __restrict__ desc, uint32_t max_chunks,
+                                                   uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr,
+                                                   int ctr_entries) {
+  constexpr uint32_t NW = NT / 64;
   __shared__ uint32_t s_pref[PRT_SLAB_INDEX_MAX + 1];
-  __shared__ uint32_t s_wch[16], s_went[16];
+  __shared__ uint32_t s_wch[NW], s_went[NW];
   const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
-  const uint32_t per = (n_slabs + 1023u) / 1024u;
+  const uint32_t per = (n_slabs + NT - 1u) / NT;
   const uint32_t b = min(tid * per, n_slabs), e = min(b + per, n_slabs);
   uint32_t ch = 0, ent = 0;
   for (uint32_t s = b; s < e; ++s) {
@@ -346,7 +353,7 @@ __global__ __launch_bounds__(1024) void k_slab_index(const uint2* __restrict__ s
     ch += (c + 63u) >> 6;
     ent += c;
   }
-  // block exclusive scan of (ch, ent): within waves, then over the 16 waves
+  // block exclusive scan of (ch, ent): within waves, then over the NW waves
   uint32_t ich = ch, ient = ent;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -362,7 +369,7 @@ __global__ __launch_bounds__(1024) void k_slab_index(const uint2* __restrict__ s
   }
   __syncthreads();
   uint32_t wb_ch = 0, wb_ent = 0, tot_ch = 0, tot_ent = 0;
-  for (uint32_t k = 0; k < 16; ++k) {
+  for (uint32_t k = 0; k < NW; ++k) {
     if (k < wv) {
       wb_ch += s_wch[k];
       wb_ent += s_went[k];
@@ -378,7 +385,7 @@ __global__ __launch_bounds__(1024) void k_slab_index(const uint2* __restrict__ s
   if (tid == 0) s_pref[n_slabs] = tot_ch;
   __syncthreads();
   const uint32_t total = min(tot_ch, max_chunks);
-  for (uint32_t c = tid; c < total; c += 1024u) {
+  for (uint32_t c = tid; c < total; c += (uint32_t)NT) {
     uint32_t lo = 0, hi = n_slabs;  // the slab s with s_pref[s] <= c < s_pref[s + 1]
     while (hi - lo > 1u) {
       const uint32_t mid = (lo + hi) >> 1;
@@ -501,8 +508,8 @@ hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const Chun
 hipError_t launch_slab_index(const uint2* slabs, uint32_t n_slabs, uint2* desc, uint32_t max_chunks, uint32_t* n_chunks,
                              uint32_t* n_entries, uint64_t* ctr, int ctr_entries, hipStream_t s) {
   if (n_slabs == 0 || n_slabs > PRT_SLAB_INDEX_MAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_slab_index, dim3(1), dim3(1024), 0, s, slabs, n_slabs, desc, max_chunks, n_chunks, n_entries,
-                     ctr, ctr_entries);
+  hipLaunchKernelGGL(k_slab_index<PRT_INDEX_THREADS>, dim3(1), dim3(PRT_INDEX_THREADS), 0, s, slabs, n_slabs, desc,
+                     max_chunks, n_chunks, n_entries, ctr, ctr_entries);
   return hipGetLastError();
 }
 hipError_t launch_resolve(const Params& P, const Batch& B, const uint64_t* accum, void* rgb, void* var, int f64,

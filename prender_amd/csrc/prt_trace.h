@@ -46,14 +46,18 @@ struct StackN {
 typedef StackN<PRT_STACK_LDS> Stack;
 
 // Blocks of 256 threads; a thread's overflow area is picked by its global
-// index modulo ovf_threads (the host sizes the area for the largest grid).
-#define PRT_DECLARE_STACK_N(S, P, NL)                                                                        \
-  __shared__ uint32_t s_node_[(NL) * 256];                                                                   \
-  __shared__ float s_t_[(NL) * 256];                                                                         \
-  StackN<NL> S{s_node_ + threadIdx.x, s_t_ + threadIdx.x,                                                    \
-               (P).stack_ovf + (size_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % (P).ovf_threads) * \
-                                   2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN)}
-#define PRT_DECLARE_STACK(S, P) PRT_DECLARE_STACK_N(S, P, PRT_STACK_LDS)
+// index modulo ovf_threads (the host sizes the area for the largest grid)
+// within one of PRT_OVF_REGIONS regions: closest-hit kernels use region 0,
+// shadow kernels region 1, because the two run at the same time on two
+// streams.
+#define PRT_DECLARE_STACK_N(S, P, NL, REGION)                                                           \
+  __shared__ uint32_t s_node_[(NL) * 256];                                                              \
+  __shared__ float s_t_[(NL) * 256];                                                                    \
+  StackN<NL> S{s_node_ + threadIdx.x, s_t_ + threadIdx.x,                                               \
+               (P).stack_ovf + ((size_t)(REGION) * (size_t)(P).ovf_threads +                             \
+                                (size_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % (P).ovf_threads)) * \
+                                   PRT_OVF_WORDS}
+#define PRT_DECLARE_STACK(S, P, REGION) PRT_DECLARE_STACK_N(S, P, PRT_STACK_LDS, REGION)
 
 // a float >= t (t > 0): the culling bound never drops below the double hit
 __device__ __forceinline__ float fup(double t) {
