@@ -185,3 +185,33 @@ def test_erpt_device_entry(prt):
     np.testing.assert_array_equal(out.cpu().numpy(), h)
     with pytest.raises(prt.PrtError):
         s.render_erpt(seed=3, sample_depths=40)
+
+
+def test_mlt_from_reference_scene_file(prt, monkeypatch):
+    # the reference's proj/test/CornellBox.txt with METROPOLIS=1 (its
+    # environment switch, scene_env.h:1908): the file's mutation / pre_sample
+    # settings drive a Metropolis render, whose image (no luminance clamp in
+    # RunMLT) must match a path-traced one without the clamp
+    import os
+    from prender_amd import scenefile
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes", "CornellBox.txt")
+    monkeypatch.setenv("METROPOLIS", "1")
+    mode, kw = scenefile.mlt_settings(path)
+    assert mode == 1 and kw["mutation"] > 0 and kw["pre_sample"] > 0
+    d = scenefile.load(path)
+    W, H = 64, 48
+    d.camera.width, d.camera.height = W, H
+    d.sampling.luminance_cutoff = 0.0
+    s = prt.Scene(d)
+    kw.update(mutation=64.0, pre_sample=25.0)
+    print("mlt CornellBox.txt 64x48", kw, flush=True)
+    m, st = s.render_mlt(seed=4, samples=4, chains=16384, **kw)
+    pt, _, _ = s.render(seed=6, samples=128)
+    k = 8
+    bp = pt.reshape(H // k, k, W // k, k, 3).mean(axis=(1, 3))
+    bm = m.reshape(H // k, k, W // k, k, 3).mean(axis=(1, 3))
+    rel = np.abs(bm - bp).sum() / np.abs(bp).sum()
+    print("CornellBox.txt mlt vs pt: mean", lum(m).mean(), lum(pt).mean(), "block rel", rel, flush=True)
+    # the oracle on the same settings: mean within 1.6 %, block error 0.044-0.059
+    assert abs(lum(m).mean() / lum(pt).mean() - 1.0) < 0.06
+    assert rel < 0.10
