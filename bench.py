@@ -134,7 +134,8 @@ def main():
                                "nee": sum_("ms_nee") / args.steps, "shadow": ms_sh / args.steps,
                                "other": sum_("ms_other") / args.steps},
         "mrays_per_s_by_class": {
-            "primary": sum_("primary_rays") / max(1e-9, sum_("ms_extend_primary")) / 1e3,
+            "primary": (sum_("primary_rays") / sum_("ms_extend_primary") / 1e3
+                        if sum_("ms_extend_primary") > 0 else None),
             "secondary": sum_("extension_rays") / max(1e-9, ms_ext - sum_("ms_extend_primary")) / 1e3,
             "shadow": sum_("shadow_rays") / max(1e-9, ms_sh) / 1e3,
             "note": "rays / time of their traversal kernels"},

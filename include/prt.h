@@ -190,6 +190,11 @@ typedef struct prt_render_opts {
     int32_t use_graph;         /* reserved (per-scene renders launch the depth loop directly) */
     int32_t timing;            /* 1 = per-kernel device timing into prt_stats */
     int32_t image_f64;         /* 1 = out_rgb / out_var are double arrays (device entry points) */
+    int32_t engine;            /* 0 = auto: scenes with glass stream through the batched
+                                  renderer's tile queue, others run depth-synchronous
+                                  batches; 1 = depth-synchronous batches; 2 = streaming
+                                  (float images, no samples_override / batch_pixels).
+                                  The image is bit-identical either way. */
 } prt_render_opts;
 
 /* ---- version / device ---------------------------------------------------- */

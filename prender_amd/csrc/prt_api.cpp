@@ -169,6 +169,7 @@ void prt_scene_destroy(prt_scene* s) {
   s->grad_buf.release();
   s->tri_alb_rr.release();
   s->mlt.release();
+  s->timer_events.release();
   for (hipEvent_t e : s->ev_nee)
     if (e) (void)hipEventDestroy(e);
   if (s->ev_shadow) (void)hipEventDestroy(s->ev_shadow);
@@ -778,6 +779,27 @@ static int finish_stats(prt_scene* s, const RenderSetup& R, Timer& tm, const Cal
 
 static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, float* d_var, prt_stats* stats) {
   HIPCHK(hipSetDevice(s->device));
+  // Scenes with glass stream through the batched renderer's tile queue: a
+  // depth-synchronous batch of them drains through long, thin tails after
+  // the glass splits, which the stream keeps filled with new tiles.  Same
+  // image bits either way (fixed-point sums, counter-based random numbers).
+  const int engine = o ? o->engine : 0;
+  const bool stream_ok = !o || (o->image_f64 == 0 && o->samples_override <= 0 && o->batch_pixels <= 0);
+  if (engine == 2 && !stream_ok)
+    return fail(PRT_E_INVALID, "the streaming engine writes float images without samples_override / batch_pixels");
+  if (engine == 2 || (engine == 0 && s->has_glass && stream_ok)) {
+    prt_batch_opts b{};
+    b.seed = o ? o->seed : 0;
+    b.max_depth = o ? o->max_depth : 0;
+    b.timing = o ? o->timing : 0;
+    b.use_graph = -1;            // plain launches (timed renders need them)
+    b.tile_samples = 4096;
+    b.inflight_tiles = 1024;     // 2^22 samples in flight
+    prt_scene* const one[1] = {s};
+    float* const rgb[1] = {d_rgb};
+    float* const var[1] = {d_var};
+    return prt_render_batch_device(one, 1, &b, rgb, var, stats);
+  }
   RenderSetup R;
   int rc;
   if ((rc = setup_render(s, o, R))) return rc;
@@ -785,7 +807,7 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   hipStream_t st = s->stream;
   HIPCHK(launch_fill_u64((uint64_t*)s->ws.ctr.p, 0ull, CTR_COUNT, st));
   Timer tm;
-  if ((rc = tm.begin(o && o->timing != 0, st))) return rc;
+  if ((rc = tm.begin(o && o->timing != 0, st, &s->timer_events))) return rc;
   CallTotals tot;
   Adjoint none{};
   for (int64_t pb = 0; pb < R.n_pix; pb += R.bp) {
@@ -912,7 +934,7 @@ int prt_adjoint_device(prt_scene* s, const prt_render_opts* o, const float* d_di
   HIPCHK(launch_fill_u64((uint64_t*)g_alb, 0ull, 3 * nf + 3, st));
   HIPCHK(launch_fill_u64((uint64_t*)s->ws.ctr.p, 0ull, CTR_COUNT, st));
   Timer tm;
-  if ((rc = tm.begin(o && o->timing != 0, st))) return rc;
+  if ((rc = tm.begin(o && o->timing != 0, st, &s->timer_events))) return rc;
   CallTotals tot;
   Adjoint none{};
   Adjoint A{};

@@ -380,7 +380,7 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
     cleanup();
     return fail(PRT_E_HIP, std::string("stream init: ") + hipGetErrorString(he));
   }
-  if ((rc = tm.begin(c.timing, s))) {
+  if ((rc = tm.begin(c.timing, s, &lead->timer_events))) {
     cleanup();
     return rc;
   }

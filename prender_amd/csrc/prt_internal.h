@@ -80,24 +80,57 @@ struct Workspace {
 // launches whose claim counters one fill clears
 enum { FETCH_GROUP = 64, FETCH_LAUNCHES = 1024 };
 
+// HIP events kept for reuse by timed renders (creating an event per mark
+// costs more host time than the late, short path depths leave).
+struct EventPool {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  int get(hipEvent_t* e) {
+    if (used == ev.size()) {
+      hipEvent_t x;
+      HIPCHK(hipEventCreate(&x));
+      ev.push_back(x);
+    }
+    *e = ev[used++];
+    return PRT_OK;
+  }
+  void release() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    used = 0;
+  }
+  ~EventPool() { release(); }
+};
+
 // Device-time spans between successive marks, summed per category.
 struct Timer {
   bool on = false;
   hipStream_t s = nullptr;
+  EventPool own;
+  EventPool* pool = &own;
   std::vector<std::pair<hipEvent_t, int>> marks;  // event + category of the span that ends here
   hipEvent_t start{};
-  int begin(bool enable, hipStream_t st) {
+  Timer() = default;
+  Timer(const Timer&) = delete;
+  Timer& operator=(const Timer&) = delete;
+  // events come from `events` (kept by the caller across calls) or, if null,
+  // from a pool of this timer's own
+  int begin(bool enable, hipStream_t st, EventPool* events = nullptr) {
     on = enable;
     s = st;
+    if (events) pool = events;
+    pool->used = 0;
     if (!on) return PRT_OK;
-    HIPCHK(hipEventCreate(&start));
+    int rc;
+    if ((rc = pool->get(&start))) return rc;
     HIPCHK(hipEventRecord(start, s));
     return PRT_OK;
   }
   int mark(int cat) {
     if (!on) return PRT_OK;
     hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
+    int rc;
+    if ((rc = pool->get(&e))) return rc;
     HIPCHK(hipEventRecord(e, s));
     marks.push_back({e, cat});
     return PRT_OK;
@@ -109,7 +142,8 @@ struct Timer {
   int side_event(hipStream_t st, int cat) {
     if (!on) return PRT_OK;
     hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
+    int rc;
+    if ((rc = pool->get(&e))) return rc;
     HIPCHK(hipEventRecord(e, st));
     side.push_back({e, cat});
     return PRT_OK;
@@ -150,11 +184,9 @@ struct Timer {
         end = std::max(end, v.second);
       }
     }
-    for (auto& m : marks) (void)hipEventDestroy(m.first);
-    for (auto& m : side) (void)hipEventDestroy(m.first);
-    (void)hipEventDestroy(start);
     marks.clear();
     side.clear();
+    pool->used = 0;  // the events are complete and free for the next call
     return wall;
   }
 };
@@ -222,4 +254,5 @@ struct prt_scene {
   prt::DevBuf grad_buf;                 // adjoint accumulators (double)
   prt::DevBuf tri_alb_rr;               // held albedo for Russian-roulette probabilities
   prt::MltWorkspace mlt;                // Metropolis renders
+  prt::EventPool timer_events;          // per-kernel timing of this scene's renders
 };
