@@ -28,6 +28,13 @@
 #ifndef PRT_TRAV_BLOCKS_PER_CU
 #define PRT_TRAV_BLOCKS_PER_CU 8
 #endif
+// camera samples per depth-synchronous batch: 2^PRT_BATCH_LOG2 (half with
+// glass).  2^24 holds a 512x512x64 frame whole: one drain of the depth tail
+// per frame instead of eight (cfg2 216 -> 294 M samples/s against 2^21), for
+// about 7.5 GB of queues, accumulators and hit records.
+#ifndef PRT_BATCH_LOG2
+#define PRT_BATCH_LOG2 24
+#endif
 #ifndef PRT_SHADOW_BLOCKS_PER_CU
 #define PRT_SHADOW_BLOCKS_PER_CU PRT_TRAV_BLOCKS_PER_CU
 #endif
@@ -353,7 +360,8 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   // traversal stack overflow: one area per thread of the largest grid
   s->max_trav_blocks = std::max(1, s->cu_count) * PRT_TRAV_BLOCKS_PER_CU;
   const int64_t ovf_threads = (int64_t)s->max_trav_blocks * 256;
-  if ((rc = dev_alloc(s->ovf, (size_t)PRT_OVF_REGIONS * ovf_threads * PRT_OVF_WORDS * sizeof(uint32_t))))
+  const int ovf_words = ovf_words_for(s->bvh_depth);
+  if ((rc = dev_alloc(s->ovf, (size_t)PRT_OVF_REGIONS * ovf_threads * ovf_words * sizeof(uint32_t))))
     return rc;
   HIPCHK(hipStreamSynchronize(st));
 
@@ -411,6 +419,7 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   P.lights = (const int32_t*)s->lights.p;
   P.stack_ovf = (uint32_t*)s->ovf.p;
   P.ovf_threads = (int32_t)ovf_threads;
+  P.ovf_words = ovf_words;
   P.n_tris = (int32_t)s->n_tris;
   return PRT_OK;
 }
@@ -572,9 +581,9 @@ static int setup_render(prt_scene* s, const prt_render_opts* o, RenderSetup& R) 
   P.key0 = (uint32_t)opts.seed;
   P.key1 = (uint32_t)(opts.seed >> 32);
   R.n_pix = (int64_t)P.W * P.H;
-  // 2^21 camera samples per batch; 2^20 with glass, whose queues are sized
-  // for up to 8 path segments per sample
-  const int64_t batch_samples = (int64_t)1 << (s->has_glass ? 20 : 21);
+  // 2^PRT_BATCH_LOG2 camera samples per batch; half with glass, whose queues
+  // are sized for up to 8 path segments per sample
+  const int64_t batch_samples = (int64_t)1 << (s->has_glass ? PRT_BATCH_LOG2 - 1 : PRT_BATCH_LOG2);
   int64_t bp = opts.batch_pixels > 0 ? opts.batch_pixels : std::max<int64_t>(1, batch_samples / P.spp);
   bp = std::min(bp, R.n_pix);
   // queues hold up to 16x (glass splits into slabs) and 24x (shadow rays)
@@ -793,8 +802,15 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
     b.max_depth = o ? o->max_depth : 0;
     b.timing = o ? o->timing : 0;
     b.use_graph = -1;            // plain launches (timed renders need them)
+    // samples in flight: 2^22 for large frames (cfg3's 50 M: 398 against 352
+    // M samples/s at 2^21), a whole frame of up to 2^21, else 2^20.  A scene
+    // keeps its workspace (about 3 KB per sample in flight with glass) for
+    // later calls, and a caller may hold many scenes.
+    const int64_t frame = (int64_t)s->P.W * s->P.H * s->P.spp;
+    const int64_t inflight = frame >= ((int64_t)1 << 25) ? ((int64_t)1 << 22)
+                           : frame <= ((int64_t)1 << 21) ? frame : ((int64_t)1 << 20);
     b.tile_samples = 4096;
-    b.inflight_tiles = 1024;     // 2^22 samples in flight
+    b.inflight_tiles = (int32_t)std::max<int64_t>(1, (inflight + 4095) / 4096);
     prt_scene* const one[1] = {s};
     float* const rgb[1] = {d_rgb};
     float* const var[1] = {d_var};

@@ -32,6 +32,7 @@ struct StreamWorkspace {
   DevBuf accum, ctr, ctl, slot_info, live, free_list, new_tiles;
   DevBuf scenes, tiles, outs, ovf, runs, nruns, desc[2], sh_desc, fetch;
   int64_t ovf_threads = 0;
+  int ovf_words = 0;
   StreamCtl* h_ctl = nullptr;  // pinned, two copies (double-buffered status reads)
   void release() {
     for (auto& a : q)
@@ -44,6 +45,7 @@ struct StreamWorkspace {
     if (h_ctl) (void)hipHostFree(h_ctl);
     h_ctl = nullptr;
     n_slots = tile_shift = q_cap = sq_cap = ovf_threads = 0;
+    ovf_words = 0;
   }
 };
 
@@ -150,7 +152,7 @@ StreamGeometry stream_geometry(const BatchConfig& c, bool glass) {
 }
 
 int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, const StreamGeometry& g, int n_scenes,
-                            int64_t n_tiles) {
+                            int64_t n_tiles, int ovf_words) {
   if (!lead->sws) lead->sws = new (std::nothrow) StreamWorkspace();
   if (!lead->sws) return fail(PRT_E_NOMEM, "out of host memory");
   StreamWorkspace& w = *lead->sws;
@@ -188,9 +190,10 @@ int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, const StreamG
   if ((rc = dev_alloc(w.sh_desc, (size_t)g.max_sh_desc * sizeof(ChunkDesc)))) return rc;
   if ((rc = dev_alloc(w.fetch, (size_t)3 * PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE * sizeof(uint32_t)))) return rc;
   const int64_t ovf_threads = (int64_t)std::max(1, lead->cu_count) * kMaxTravBlocksPerCu * 256;
-  if ((rc = dev_alloc(w.ovf, (size_t)PRT_OVF_REGIONS * ovf_threads * PRT_OVF_WORDS * sizeof(uint32_t))))
+  if ((rc = dev_alloc(w.ovf, (size_t)PRT_OVF_REGIONS * ovf_threads * ovf_words * sizeof(uint32_t))))
     return rc;
   w.ovf_threads = ovf_threads;
+  w.ovf_words = ovf_words;
   if (!w.h_ctl) HIPCHK(hipHostMalloc((void**)&w.h_ctl, 2 * sizeof(StreamCtl), hipHostMallocDefault));
   w.n_slots = c.n_slots;
   w.tile_shift = c.tile_shift;
@@ -233,7 +236,9 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   if (tiles.size() >= ((size_t)1 << 31)) return fail(PRT_E_INVALID, "too many tiles");
   int rc;
   const StreamGeometry geo = stream_geometry(c, glass);
-  if ((rc = ensure_stream_workspace(lead, c, geo, n, (int64_t)tiles.size()))) return rc;
+  int ovf_words = 2;  // the deepest BVH of the batch sizes the shared overflow area
+  for (int i = 0; i < n; ++i) ovf_words = std::max(ovf_words, (int)scenes[i]->P.ovf_words);
+  if ((rc = ensure_stream_workspace(lead, c, geo, n, (int64_t)tiles.size(), ovf_words))) return rc;
   StreamWorkspace& w = *lead->sws;
   hipStream_t s = lead->stream;
 
@@ -246,6 +251,7 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
     P.max_vertices = c.max_vertices;
     P.stack_ovf = (uint32_t*)w.ovf.p;
     P.ovf_threads = (int32_t)w.ovf_threads;
+    P.ovf_words = w.ovf_words;
     params[i] = P;
     outs[i] = d_rgb[i];
     outs[n + i] = d_var ? d_var[i] : nullptr;

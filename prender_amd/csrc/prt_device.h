@@ -23,10 +23,14 @@
 // smallest LDS stack depth any kernel variant uses (sizes the overflow area)
 #define PRT_STACK_LDS_MIN 8
 #define PRT_STACK_TOTAL 128
-// words of one thread's traversal-stack overflow area, and the number of
-// such areas per thread (closest-hit and shadow kernels run concurrently)
-#define PRT_OVF_WORDS (2 * (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN))
+// traversal-stack overflow areas per thread (closest-hit and shadow kernels
+// run concurrently); each holds the entries beyond the smallest LDS stack
+// that the scene's BVH depth can need (ovf_words_for)
 #define PRT_OVF_REGIONS 2
+inline int ovf_words_for(int bvh_depth) {
+  const int entries = 3 * bvh_depth + 1 - PRT_STACK_LDS_MIN;  // deepest stack of a 4-wide BVH
+  return 2 * (entries > 1 ? entries : 1);
+}
 
 struct __attribute__((aligned(16))) DevNode {
   float lo_x[4], lo_y[4], lo_z[4];
@@ -99,6 +103,8 @@ struct Params {
   uint32_t* mlt_flags;  // per chain: 1 if the camera ray returned a light directly (ERPT), or null
   int32_t mlt_stride;
   int32_t mlt_depths;
+  int32_t ovf_words;    // words of one thread's stack overflow area
+  int32_t pad2_;
 };
 // camera words of a Metropolis proposal: pixel x, y, sub-pixel x, y, tent
 // filter x, y, two spare (render.h:262-300)

@@ -17,13 +17,13 @@ struct TravStats {
 
 // Per-lane traversal stack: the first NL entries in LDS (lane interleaved,
 // bank-conflict free), deeper ones in a per-thread global area of
-// (PRT_STACK_TOTAL - PRT_STACK_LDS_MIN) * 2 words.
+// Params::ovf_words words (enough for the scene's BVH depth).
 template <int NL>
 struct StackN {
   static constexpr int stride = 256;  // threads per block of every traversal kernel
   uint32_t* lds_node;  // [NL][256]
   float* lds_t;
-  uint32_t* ovf;       // (PRT_STACK_TOTAL - NL) * 2 words used
+  uint32_t* ovf;       // (3 * BVH depth + 1 - NL) * 2 words used at most
   __device__ __forceinline__ void push(int sp, int32_t node, float t) {
     if (sp < NL) {
       lds_node[sp * stride] = (uint32_t)node;
@@ -56,7 +56,7 @@ typedef StackN<PRT_STACK_LDS> Stack;
   StackN<NL> S{s_node_ + threadIdx.x, s_t_ + threadIdx.x,                                               \
                (P).stack_ovf + ((size_t)(REGION) * (size_t)(P).ovf_threads +                             \
                                 (size_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % (P).ovf_threads)) * \
-                                   PRT_OVF_WORDS}
+                                   (size_t)(P).ovf_words}
 #define PRT_DECLARE_STACK(S, P, REGION) PRT_DECLARE_STACK_N(S, P, PRT_STACK_LDS, REGION)
 
 // a float >= t (t > 0): the culling bound never drops below the double hit
