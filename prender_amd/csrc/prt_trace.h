@@ -15,32 +15,39 @@ struct TravStats {
   uint32_t rays;  // queries finished (traverse_pool)
 };
 
-// Per-lane traversal stack: the first NL entries in LDS (lane interleaved,
-// bank-conflict free), deeper ones in a per-thread global area of
-// Params::ovf_words words (enough for the scene's BVH depth).
+// Per-lane traversal stack of (node, t) entries: the first NL in LDS (lane
+// interleaved, one 64-bit entry per level, bank-conflict free), deeper ones
+// in a per-thread global area of Params::ovf_words words (enough for the
+// scene's BVH depth).  The overflow accesses are volatile so that the
+// compiler keeps them apart from the LDS ones: merged, both would become
+// flat-address loads and stores, which wait on every outstanding memory
+// operation.
 template <int NL>
 struct StackN {
   static constexpr int stride = 256;  // threads per block of every traversal kernel
-  uint32_t* lds_node;  // [NL][256]
-  float* lds_t;
-  uint32_t* ovf;       // (3 * BVH depth + 1 - NL) * 2 words used at most
+  uint2* lds;                         // [NL][256]
+  uint2* ovf;                         // 3 * BVH depth + 1 - NL entries used at most
   __device__ __forceinline__ void push(int sp, int32_t node, float t) {
+    const uint2 e = make_uint2((uint32_t)node, __float_as_uint(t));
     if (sp < NL) {
-      lds_node[sp * stride] = (uint32_t)node;
-      lds_t[sp * stride] = t;
+      lds[sp * stride] = e;
     } else {
-      ovf[2 * (sp - NL)] = (uint32_t)node;
-      ovf[2 * (sp - NL) + 1] = __float_as_uint(t);
+      volatile uint32_t* g = reinterpret_cast<volatile uint32_t*>(ovf + (sp - NL));
+      g[0] = e.x;
+      g[1] = e.y;
     }
   }
   __device__ __forceinline__ void pop(int sp, int32_t& node, float& t) {
+    uint2 e;
     if (sp < NL) {
-      node = (int32_t)lds_node[sp * stride];
-      t = lds_t[sp * stride];
+      e = lds[sp * stride];
     } else {
-      node = (int32_t)ovf[2 * (sp - NL)];
-      t = __uint_as_float(ovf[2 * (sp - NL) + 1]);
+      const volatile uint32_t* g = reinterpret_cast<const volatile uint32_t*>(ovf + (sp - NL));
+      e.x = g[0];
+      e.y = g[1];
     }
+    node = (int32_t)e.x;
+    t = __uint_as_float(e.y);
   }
 };
 typedef StackN<PRT_STACK_LDS> Stack;
@@ -50,13 +57,13 @@ typedef StackN<PRT_STACK_LDS> Stack;
 // within one of PRT_OVF_REGIONS regions: closest-hit kernels use region 0,
 // shadow kernels region 1, because the two run at the same time on two
 // streams.
-#define PRT_DECLARE_STACK_N(S, P, NL, REGION)                                                           \
-  __shared__ uint32_t s_node_[(NL) * 256];                                                              \
-  __shared__ float s_t_[(NL) * 256];                                                                    \
-  StackN<NL> S{s_node_ + threadIdx.x, s_t_ + threadIdx.x,                                               \
-               (P).stack_ovf + ((size_t)(REGION) * (size_t)(P).ovf_threads +                             \
-                                (size_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % (P).ovf_threads)) * \
-                                   (size_t)(P).ovf_words}
+#define PRT_DECLARE_STACK_N(S, P, NL, REGION)                                                                \
+  __shared__ uint2 s_stack_[(NL) * 256];                                                                     \
+  StackN<NL> S{s_stack_ + threadIdx.x,                                                                       \
+               reinterpret_cast<uint2*>(                                                                     \
+                   (P).stack_ovf + ((size_t)(REGION) * (size_t)(P).ovf_threads +                              \
+                                    (size_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % (P).ovf_threads)) * \
+                                       (size_t)(P).ovf_words)}
 #define PRT_DECLARE_STACK(S, P, REGION) PRT_DECLARE_STACK_N(S, P, PRT_STACK_LDS, REGION)
 
 // a float >= t (t > 0): the culling bound never drops below the double hit
