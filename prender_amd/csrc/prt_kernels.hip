@@ -274,6 +274,10 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
 // origin, throughput T*colour/rr and orienting normal), with the parent
 // vertex's random numbers.  Shadow rays go to the wave's slab of the shadow
 // queue.
+// ONE_LIGHT: the scene has exactly one light, whose entity record is read
+// once into wave-uniform registers (a separate instance keeps the general
+// path's registers out of it).
+template <bool ONE_LIGHT>
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
     Params P, Batch B, RayQ q, ChunkList cl, ShadowQ sq, uint2* slabs, uint64_t* ctr) {
   __shared__ uint32_t s_cnt[4];  // shadow rays appended by each wave of the block
@@ -285,22 +289,29 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
   wave_range(*cl.n, c, c_end);
   // this wave's slab: three shadow rays per entry of its chunks
   const uint32_t off = 3u * 64u * c, slab_cap = 3u * 64u * (c_end - c);
+  UniformLight U{};
+  if (ONE_LIGHT && c < c_end) U = uniform_light(P, P.lights[0]);
+  // the next chunk's descriptor is read one chunk ahead, and an entry's
+  // fields are all read before its NEE flag is tested: one memory round
+  // trip per chunk instead of three dependent ones
+  uint2 dn = c < c_end ? cl.desc[c] : make_uint2(0u, 0u);
   for (; c < c_end; ++c) {
-    const uint2 dc = cl.desc[c];
+    const uint2 dc = dn;
+    if (c + 1 < c_end) dn = cl.desc[c + 1];
     bool have = false;
     uint32_t slot = 0, w[9];
     d3 x, nrm, ct;
     if (lane_id() < dc.y) {
       const uint32_t i = dc.x + lane_id();
       const uint32_t meta = q.meta[i];
+      slot = q.slot[i];
+      x = mk(q.ox[i], q.oy[i], q.oz[i]);
+      nrm = mk(q.nx[i], q.ny[i], q.nz[i]);
+      ct = mk(q.tr[i], q.tg[i], q.tb[i]);
       if (meta & META_NEE) {
         have = true;
-        slot = q.slot[i];
         const uint32_t pl = slot / P.spp, k = slot % P.spp;
         nee_words(P, (uint32_t)(B.pixel_base + pl), k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta), w);
-        x = mk(q.ox[i], q.oy[i], q.oz[i]);
-        nrm = mk(q.nx[i], q.ny[i], q.nz[i]);
-        ct = mk(q.tr[i], q.tg[i], q.tb[i]);
       }
     }
     // one light sample at a time, appended through the LDS count: only one
@@ -309,7 +320,8 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
     for (int m = 0; m < 3; ++m) {
       ShadowRay sr;
       int32_t l = -1;
-      const bool v = have && nee_one(P, w, m, x, nrm, ct, sr, l);
+      const bool v = have && (ONE_LIGHT ? nee_one_single(U, w, m, x, nrm, ct, sr, l)
+                                        : nee_one(P, w, m, x, nrm, ct, sr, l));
       const uint32_t j = wave_append_bit(&s_cnt[wib], v);
       if (v) {
         if (j < slab_cap) {
@@ -502,7 +514,10 @@ hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ
 }
 hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const ChunkList& cl, const ShadowQ& sq,
                       uint2* slabs, uint64_t* ctr, hipStream_t s) {
-  hipLaunchKernelGGL(k_nee, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slabs, ctr);
+  if (P.n_lights == 1)
+    hipLaunchKernelGGL(k_nee<true>, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slabs, ctr);
+  else
+    hipLaunchKernelGGL(k_nee<false>, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slabs, ctr);
   return hipGetLastError();
 }
 hipError_t launch_slab_index(const uint2* slabs, uint32_t n_slabs, uint2* desc, uint32_t max_chunks, uint32_t* n_chunks,

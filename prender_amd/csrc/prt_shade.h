@@ -271,9 +271,9 @@ struct ShadowRay {
 // only when the light's own first hit along it lies within 1e-3 of the
 // sampled distance (the reference's acceptance test, line 141); the shadow
 // kernel then checks that nothing nearer blocks it.
-__device__ __forceinline__ bool nee_sample(const Params& P, int li, double ra, double rb, d3 x, d3 n_orient, d3 ct,
-                                           ShadowRay& s) {
-  const DevEntity& le = P.ents[li];
+// le: the light entity, E: its material's emission.
+__device__ __forceinline__ bool nee_sample_e(const DevEntity& le, const d3 E, double ra, double rb, d3 x,
+                                             d3 n_orient, d3 ct, ShadowRay& s) {
   d3 light_pos, light_normal;
   if (le.type == ENT_SPHERE) {
     // r1 = 2 pi ra
@@ -301,7 +301,6 @@ __device__ __forceinline__ bool nee_sample(const Params& P, int li, double ra, d
   if (!analytic_t(le, o2, light_dir, tl)) return false;
   if (!(fabs(sqrt(dist2) - tl) < 1e-3)) return false;
   const double G = dot0 * dot1 / dist2;
-  const d3 E = ld3(P.mats[le.material].emission);
   // (E * colour/pi) * G / pdf, averaged over the 3 samples, / rr, times T;
   // ct already holds T * colour / rr
   const d3 rad = E * (PS_INV_PI * G / pdf / 3.0);
@@ -312,6 +311,52 @@ __device__ __forceinline__ bool nee_sample(const Params& P, int li, double ra, d
   s.c[1] = (float)(ct.y * rad.y);
   s.c[2] = (float)(ct.z * rad.z);
   return true;
+}
+
+__device__ __forceinline__ bool nee_sample(const Params& P, int li, double ra, double rb, d3 x, d3 n_orient, d3 ct,
+                                           ShadowRay& s) {
+  const DevEntity& le = P.ents[li];
+  return nee_sample_e(le, ld3(P.mats[le.material].emission), ra, rb, x, n_orient, ct, s);
+}
+
+__device__ __forceinline__ double uniform_f64(double v) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// The entity and emission of light entity li (the same for every lane), read
+// into wave-uniform registers: with a single light, NEE samples use these
+// instead of re-reading the entity record per sample.
+struct UniformLight {
+  DevEntity e;
+  d3 E;
+  int32_t li;
+};
+__device__ __forceinline__ UniformLight uniform_light(const Params& P, int li) {
+  UniformLight u;
+  u.li = __builtin_amdgcn_readfirstlane(li);
+  const DevEntity& g = P.ents[u.li];
+  u.e.type = __builtin_amdgcn_readfirstlane(g.type);
+  u.e.material = __builtin_amdgcn_readfirstlane(g.material);
+  u.e.ninv = __builtin_amdgcn_readfirstlane(g.ninv);
+  u.e.light = 0;
+  u.e.has_albedo = 0;
+  u.e.pad_ = 0;
+  for (int k = 0; k < 3; ++k) {
+    u.e.p0[k] = uniform_f64(g.p0[k]);
+    u.e.u_axis[k] = uniform_f64(g.u_axis[k]);
+    u.e.v_axis[k] = uniform_f64(g.v_axis[k]);
+    u.e.normal[k] = uniform_f64(g.normal[k]);
+  }
+  u.e.radius = uniform_f64(g.radius);
+  u.e.u_len = uniform_f64(g.u_len);
+  u.e.v_len = uniform_f64(g.v_len);
+  u.e.area = uniform_f64(g.area);
+  const DevMaterial& m = P.mats[u.e.material];
+  u.E = mk(uniform_f64(m.emission[0]), uniform_f64(m.emission[1]), uniform_f64(m.emission[2]));
+  return u;
 }
 
 __device__ __forceinline__ void put_shadow(ShadowQ& q, uint32_t j, const ShadowRay& s, uint32_t slot, int32_t light) {
@@ -345,10 +390,17 @@ __device__ __forceinline__ void nee_words(const Params& P, uint32_t pixel, uint3
 __device__ __forceinline__ bool nee_one(const Params& P, const uint32_t w[9], int m, d3 x, d3 nrm, d3 ct,
                                         ShadowRay& s, int32_t& l) {
   const int NL = P.n_lights;
+  // with one light the index is (int)(u * 0 + 0.5) = 0 for every u
   int i = (int)(u01(w[3 * m]) * (double)(NL - 1) + 0.5);
   i = i >= NL ? NL - 1 : i;
   l = P.lights[i];
   return nee_sample(P, l, u01(w[3 * m + 1]), u01(w[3 * m + 2]), x, nrm, ct, s);
+}
+// nee_one for a scene with exactly one light, whose entity U holds.
+__device__ __forceinline__ bool nee_one_single(const UniformLight& U, const uint32_t w[9], int m, d3 x, d3 nrm,
+                                               d3 ct, ShadowRay& s, int32_t& l) {
+  l = U.li;
+  return nee_sample_e(U.e, U.E, u01(w[3 * m + 1]), u01(w[3 * m + 2]), x, nrm, ct, s);
 }
 
 // Luminance of mlt.h:171-174 with the Vector3d dot pairing.

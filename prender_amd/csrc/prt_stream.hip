@@ -479,6 +479,8 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
   uint32_t emitted = 0, dropped = 0;
   uint32_t c, c_end;
   wave_range(st.ctl->cont_chunks[nxt], c, c_end);
+  // an entry's fields are all read before its NEE flag is tested (one
+  // memory round trip less per chunk)
   for (; c < c_end; ++c) {
     const ChunkDesc dc = chunk_at(desc, c);
     log.enter(dc.scene, min(s_cnt[wib], st.slab_sh));
@@ -489,17 +491,17 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
     if (lane_id() < dc.cnt) {
       const uint32_t i = dc.base + lane_id();
       const uint32_t meta = q.meta[i];
+      slot = q.slot[i];
+      x = mk(q.ox[i], q.oy[i], q.oz[i]);
+      nrm = mk(q.nx[i], q.ny[i], q.nz[i]);
+      ct = mk(q.tr[i], q.tg[i], q.tb[i]);
       if (meta & META_NEE) {
         have = true;
-        slot = q.slot[i];
         const SlotInfo si = st.slot_info[slot >> shift];
         const uint32_t r = slot & tmask;
         const uint32_t k = r % (uint32_t)si.spp;
         const int64_t pixel = si.pixel_base + r / (uint32_t)si.spp;
         nee_words(P, (uint32_t)pixel, k, (int)(meta & 0xFFu) - 1, META_BRANCH(meta), w);
-        x = mk(q.ox[i], q.oy[i], q.oz[i]);
-        nrm = mk(q.nx[i], q.ny[i], q.nz[i]);
-        ct = mk(q.tr[i], q.tg[i], q.tb[i]);
       }
     }
     // one light sample at a time, appended through the LDS count: only one
