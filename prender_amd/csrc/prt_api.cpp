@@ -195,6 +195,7 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   if ((int64_t)smp.samples * smp.supersamples * smp.supersamples > (1 << 20))
     return fail(PRT_E_INVALID, "samples per pixel above 2^20");
   if (d->n_materials <= 0 || !d->materials) return fail(PRT_E_INVALID, "no materials");
+  if ((uint32_t)d->n_materials > TRI_SHADE_MAT_MASK + 1u) return fail(PRT_E_INVALID, "more than 2^24 materials");
   if (d->n_entities < 0 || (d->n_entities > 0 && !d->entities)) return fail(PRT_E_INVALID, "bad entity list");
 
   int dev = 0;
@@ -337,10 +338,13 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
     s->slot_of_tri[t] = (uint32_t)slot;
     float* r = &tri4[12 * slot];
     const float* v = &v9[9 * (size_t)t];
-    int32_t ent = t_ent[t], face = t_face[t], zero = 0;
+    int32_t ent = t_ent[t], face = t_face[t];
+    const DevEntity& de = s->h_ents[ent];
+    const uint32_t shade = ((uint32_t)de.material & TRI_SHADE_MAT_MASK) | (de.ninv < 0 ? TRI_SHADE_NINV : 0u) |
+                           (de.light ? TRI_SHADE_LIGHT : 0u) | (de.has_albedo ? TRI_SHADE_ALBEDO : 0u);
     r[0] = v[0]; r[1] = v[1]; r[2] = v[2]; std::memcpy(&r[3], &ent, 4);
     r[4] = v[3]; r[5] = v[4]; r[6] = v[5]; std::memcpy(&r[7], &face, 4);
-    r[8] = v[6]; r[9] = v[7]; r[10] = v[8]; std::memcpy(&r[11], &zero, 4);
+    r[8] = v[6]; r[9] = v[7]; r[10] = v[8]; std::memcpy(&r[11], &shade, 4);
     for (int k = 0; k < 3; ++k) nrm[3 * slot + k] = t_nrm[3 * (size_t)t + k];
     if (any_albedo)
       for (int k = 0; k < 3; ++k) alb[3 * slot + k] = t_alb[3 * (size_t)t + k];

@@ -58,6 +58,13 @@ struct DevEntity {
   double u_len, v_len, area;
 };
 
+// Fourth word of a triangle record's third float4: the entity's material
+// index and flags (the first two hold the entity and face indices).
+#define TRI_SHADE_MAT_MASK 0x00FFFFFFu
+#define TRI_SHADE_NINV 0x01000000u    // normal_vector_inverse < 0
+#define TRI_SHADE_LIGHT 0x02000000u   // material isLight()
+#define TRI_SHADE_ALBEDO 0x04000000u  // per-face albedo in tri_alb
+
 struct DevMaterial {
   double color[3];
   double emission[3];

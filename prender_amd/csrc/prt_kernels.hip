@@ -175,13 +175,16 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
   C.cap = fan * 64u * (c_end - c);
   C.truncated = C.n_ext = C.pickups = C.dropped = 0;
   double ge[3] = {0.0, 0.0, 0.0};
+  // chunk descriptors are read one chunk ahead
+  uint2 dn = (!primary && c < c_end) ? cl.desc[c] : make_uint2(0u, 0u);
   for (; c < c_end; ++c) {
     uint32_t i;
     if (primary) {
       i = (c << 6) + lane_id();
       if (i >= n_samples) continue;
     } else {
-      const uint2 dc = cl.desc[c];
+      const uint2 dc = dn;
+      if (c + 1 < c_end) dn = cl.desc[c + 1];
       if (lane_id() >= dc.y) continue;
       i = dc.x + lane_id();
     }

@@ -110,14 +110,30 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
   const d3 o2 = o + d * 0.000001;
   d3 x, n;
   int ei;
+  // the hit entity's material and flags: a triangle carries them in its
+  // record (TRI_SHADE_*), so its material is read without the entity record
+  uint32_t mat;
+  bool ninv_neg, is_light, has_alb, uvplane;
   if (id >= 0) {
     const float4* tr = reinterpret_cast<const float4*>(P.tris) + (size_t)id * 3;
     ei = __float_as_int(tr[0].w);
+    va.face = __float_as_int(tr[1].w);
+    const uint32_t sw = __float_as_uint(tr[2].w);
+    mat = sw & TRI_SHADE_MAT_MASK;
+    ninv_neg = (sw & TRI_SHADE_NINV) != 0;
+    is_light = (sw & TRI_SHADE_LIGHT) != 0;
+    has_alb = (sw & TRI_SHADE_ALBEDO) != 0;
+    uvplane = false;
     x = o2 + d * t;  // polygon.h:170
     n = mk(P.tri_nrm[3 * (size_t)id], P.tri_nrm[3 * (size_t)id + 1], P.tri_nrm[3 * (size_t)id + 2]);
   } else {
     ei = -2 - id;
     const DevEntity& e = P.ents[ei];
+    mat = (uint32_t)e.material;
+    ninv_neg = e.ninv < 0;
+    is_light = e.light != 0;
+    has_alb = false;
+    uvplane = e.type == ENT_UVPLANE;
     if (e.type == ENT_SPHERE) {
       x = o2 + t * d;  // sphere.h:89-95
       n = normalize(x - ld3(e.p0));
@@ -126,19 +142,17 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
       n = ld3(e.normal);
     }
   }
-  const DevEntity& e = P.ents[ei];
   va.entity = ei;
-  if (id >= 0) va.face = __float_as_int(reinterpret_cast<const float4*>(P.tris)[(size_t)id * 3 + 1].w);
-  if (e.ninv < 0) n = n * -1.0;
-  const DevMaterial& m = P.mats[e.material];
+  if (ninv_neg) n = n * -1.0;
+  const DevMaterial& m = P.mats[mat];
   d3 color;
-  if (e.has_albedo && id >= 0)
+  if (has_alb)
     color = mk(P.tri_alb[3 * (size_t)id], P.tri_alb[3 * (size_t)id + 1], P.tri_alb[3 * (size_t)id + 2]);
   else
     color = ld3(m.color);
   const d3 E = ld3(m.emission);
   // pickup of the parent's mirror / glass vertex (directLight_radiance.h:516-556)
-  if (e.light && (Pw.x != 0.0 || Pw.y != 0.0 || Pw.z != 0.0)) {
+  if (is_light && (Pw.x != 0.0 || Pw.y != 0.0 || Pw.z != 0.0)) {
     add = add + vmul(Pw, E);
     va.pre = vmul(Pw, E);
     va.add_e = Pw;
@@ -147,7 +161,7 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
   const d3 n_orient = dot(n, d) < 0.0 ? n : (-1.0 * n);
   // RGB_MAX = std::max(x, std::max(y, z)) (def.h:19); RR (radiance.h:77-126)
   d3 color_rr = color;
-  if (P.tri_alb_rr && e.has_albedo && id >= 0)
+  if (P.tri_alb_rr && has_alb)
     color_rr = mk(P.tri_alb_rr[3 * (size_t)id], P.tri_alb_rr[3 * (size_t)id + 1], P.tri_alb_rr[3 * (size_t)id + 2]);
   const d3 rw = E + color_rr;
   const double myz = (rw.y < rw.z) ? rw.z : rw.y;
@@ -173,8 +187,8 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
   const double pk = P.nee ? 1.0 : 0.0;
   if (kind == 0) {
     // BRDF_Diffuse, brdf.h:408-700
-    if (e.light && e.type == ENT_UVPLANE && dot(n, d) >= 0.0) return add;
-    if (e.light) {
+    if (is_light && uvplane && dot(n, d) >= 0.0) return add;
+    if (is_light) {
       if (P.nee && depth == 0) C.direct = 1u;
       if (!P.nee || depth == 0) {
         add = add + vmul(T, E / rr / (1.0 - 0.0));
