@@ -284,6 +284,8 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   S.accum = (uint64_t*)w.accum.p;
   S.ctr = (uint64_t*)w.ctr.p;
   S.n_scenes = n;
+  S.nee_max_lights = 0;
+  for (int i = 0; i < n; ++i) S.nee_max_lights = std::max(S.nee_max_lights, (int32_t)scenes[i]->P.n_lights);
   S.n_tiles = (int32_t)tiles.size();
   S.n_slots = c.n_slots;
   S.tile_shift = c.tile_shift;

@@ -277,13 +277,16 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
 // origin, throughput T*colour/rr and orienting normal), with the parent
 // vertex's random numbers.  Shadow rays go to the wave's slab of the shadow
 // queue.
-// ONE_LIGHT: the scene has exactly one light, whose entity record is read
-// once into wave-uniform registers (a separate instance keeps the general
-// path's registers out of it).
-template <bool ONE_LIGHT>
+// MODE (NeeMode): NEE_ONE, the scene has exactly one light, whose entity
+// record is read once into wave-uniform registers; NEE_LDS, each wave copies
+// the scene's lights (at most NEE_LDS_LIGHTS) into its LDS slots; NEE_GLOBAL,
+// light records are read from global memory per sample.  Separate instances
+// keep each path's registers out of the others.
+template <int MODE>
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
     Params P, Batch B, RayQ q, ChunkList cl, ShadowQ sq, uint2* slabs, uint64_t* ctr) {
   __shared__ uint32_t s_cnt[4];  // shadow rays appended by each wave of the block
+  __shared__ LightSlot s_lights[MODE == NEE_LDS ? 4 * NEE_LDS_LIGHTS : 1];
   const uint32_t wib = threadIdx.x >> 6;
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (lane_id() == 0) s_cnt[wib] = 0;
@@ -293,7 +296,9 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
   // this wave's slab: three shadow rays per entry of its chunks
   const uint32_t off = 3u * 64u * c, slab_cap = 3u * 64u * (c_end - c);
   UniformLight U{};
-  if (ONE_LIGHT && c < c_end) U = uniform_light(P, P.lights[0]);
+  LightSlot* slots = s_lights + (MODE == NEE_LDS ? wib * NEE_LDS_LIGHTS : 0);
+  if (MODE == NEE_ONE && c < c_end) U = uniform_light(P, P.lights[0]);
+  if (MODE == NEE_LDS) load_light_slots(P, slots);
   // the next chunk's descriptor is read one chunk ahead, and an entry's
   // fields are all read before its NEE flag is tested: one memory round
   // trip per chunk instead of three dependent ones
@@ -323,8 +328,9 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
     for (int m = 0; m < 3; ++m) {
       ShadowRay sr;
       int32_t l = -1;
-      const bool v = have && (ONE_LIGHT ? nee_one_single(U, w, m, x, nrm, ct, sr, l)
-                                        : nee_one(P, w, m, x, nrm, ct, sr, l));
+      const bool v = have && (MODE == NEE_ONE   ? nee_one_single(U, w, m, x, nrm, ct, sr, l)
+                              : MODE == NEE_LDS ? nee_one_slots(P, slots, w, m, x, nrm, ct, sr, l)
+                                                : nee_one(P, w, m, x, nrm, ct, sr, l));
       const uint32_t j = wave_append_bit(&s_cnt[wib], v);
       if (v) {
         if (j < slab_cap) {
@@ -518,9 +524,11 @@ hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ
 hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const ChunkList& cl, const ShadowQ& sq,
                       uint2* slabs, uint64_t* ctr, hipStream_t s) {
   if (P.n_lights == 1)
-    hipLaunchKernelGGL(k_nee<true>, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slabs, ctr);
+    hipLaunchKernelGGL(k_nee<NEE_ONE>, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slabs, ctr);
+  else if (P.n_lights <= NEE_LDS_LIGHTS)
+    hipLaunchKernelGGL(k_nee<NEE_LDS>, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slabs, ctr);
   else
-    hipLaunchKernelGGL(k_nee<false>, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slabs, ctr);
+    hipLaunchKernelGGL(k_nee<NEE_GLOBAL>, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, P, B, q, cl, sq, slabs, ctr);
   return hipGetLastError();
 }
 hipError_t launch_slab_index(const uint2* slabs, uint32_t n_slabs, uint2* desc, uint32_t max_chunks, uint32_t* n_chunks,

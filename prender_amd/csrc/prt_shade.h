@@ -410,6 +410,43 @@ __device__ __forceinline__ bool nee_one(const Params& P, const uint32_t w[9], in
   l = P.lights[i];
   return nee_sample(P, l, u01(w[3 * m + 1]), u01(w[3 * m + 2]), x, nrm, ct, s);
 }
+// Light tables of the NEE kernels: up to NEE_LDS_LIGHTS lights of a scene
+// (entity record, emission, entity index) copied into a wave's LDS slots, so
+// that a light sample reads its light with one LDS access instead of three
+// dependent global loads (light list, entity record, material).
+enum NeeMode { NEE_GLOBAL = 0, NEE_ONE = 1, NEE_LDS = 2 };
+#define NEE_LDS_LIGHTS 16
+struct LightSlot {
+  DevEntity e;
+  d3 E;
+  int32_t li, pad_;
+};
+// Lanes < n_lights of the calling wave (which must all be active) copy
+// light `lane` of P; the slots are readable by the whole wave afterwards.
+__device__ __forceinline__ void load_light_slots(const Params& P, LightSlot* slots) {
+  const int k = (int)lane_id();
+  if (k < P.n_lights) {
+    const int32_t li = P.lights[k];
+    const DevEntity e = P.ents[li];
+    const DevMaterial& m = P.mats[e.material];
+    slots[k].e = e;
+    slots[k].E = ld3(m.emission);
+    slots[k].li = li;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// nee_one with the scene's lights in LDS slots.
+__device__ __forceinline__ bool nee_one_slots(const Params& P, const LightSlot* slots, const uint32_t w[9], int m,
+                                              d3 x, d3 nrm, d3 ct, ShadowRay& s, int32_t& l) {
+  const int NL = P.n_lights;
+  int i = (int)(u01(w[3 * m]) * (double)(NL - 1) + 0.5);
+  i = i >= NL ? NL - 1 : i;
+  const LightSlot& L = slots[i];
+  l = L.li;
+  return nee_sample_e(L.e, L.E, u01(w[3 * m + 1]), u01(w[3 * m + 2]), x, nrm, ct, s);
+}
 // nee_one for a scene with exactly one light, whose entity U holds.
 __device__ __forceinline__ bool nee_one_single(const UniformLight& U, const uint32_t w[9], int m, d3 x, d3 nrm,
                                                d3 ct, ShadowRay& s, int32_t& l) {

@@ -145,7 +145,7 @@ struct Stream {
   uint32_t prim_base;       // start of the primary region (= PRT_SLAB_WAVES * slab_ray)
   uint32_t prim_cap;        // its entries
   uint32_t max_desc, max_sh_desc;
-  uint32_t pad_;
+  int32_t nee_max_lights;   // most lights of any scene (picks how k_stream_nee reads light records)
 };
 
 hipError_t launch_stream_init(const Stream& st, hipStream_t s);

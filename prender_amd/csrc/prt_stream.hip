@@ -467,8 +467,15 @@ __global__ __launch_bounds__(1024) void k_stream_index(Stream st, int which) {
 
 // Light samples of the diffuse vertices behind the rays of queue nxt's slab
 // chunks; shadow rays go to this wave's shadow slab, logged as runs.
+// MODE (NeeMode): NEE_ONE, no scene of the batch has more than one light and
+// a wave reads the light of its chunk's scene into wave-uniform registers;
+// NEE_LDS, no scene has more than NEE_LDS_LIGHTS lights and a wave copies its
+// chunk's scene's lights into its LDS slots; NEE_GLOBAL, light records are
+// read per sample.  Either copy is refreshed when the chunk's scene changes.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, ShadowQ sq) {
   __shared__ uint32_t s_cnt[4];  // entries in each wave's shadow slab
+  __shared__ LightSlot s_lights[MODE == NEE_LDS ? 4 * NEE_LDS_LIGHTS : 1];
   const uint32_t wib = threadIdx.x >> 6;
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (lane_id() == 0) s_cnt[wib] = 0;
@@ -479,12 +486,20 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
   uint32_t emitted = 0, dropped = 0;
   uint32_t c, c_end;
   wave_range(st.ctl->cont_chunks[nxt], c, c_end);
+  UniformLight U{};
+  LightSlot* slots = s_lights + (MODE == NEE_LDS ? wib * NEE_LDS_LIGHTS : 0);
+  int32_t u_scene = -1;
   // an entry's fields are all read before its NEE flag is tested (one
   // memory round trip less per chunk)
   for (; c < c_end; ++c) {
     const ChunkDesc dc = chunk_at(desc, c);
     log.enter(dc.scene, min(s_cnt[wib], st.slab_sh));
     const Params& P = scene_params(st.scenes, dc.scene);
+    if (MODE != NEE_GLOBAL && (int32_t)dc.scene != u_scene) {
+      u_scene = (int32_t)dc.scene;
+      if (MODE == NEE_ONE && P.n_lights == 1) U = uniform_light(P, P.lights[0]);
+      if (MODE == NEE_LDS) load_light_slots(P, slots);
+    }
     bool have = false;
     uint32_t slot = 0, w[9];
     d3 x, nrm, ct;
@@ -511,7 +526,9 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
     for (int m = 0; m < 3; ++m) {
       ShadowRay sr;
       int32_t l = -1;
-      const bool v = have && nee_one(P, w, m, x, nrm, ct, sr, l);
+      const bool v = have && (MODE == NEE_ONE   ? nee_one_single(U, w, m, x, nrm, ct, sr, l)
+                              : MODE == NEE_LDS ? nee_one_slots(P, slots, w, m, x, nrm, ct, sr, l)
+                                                : nee_one(P, w, m, x, nrm, ct, sr, l));
       const uint32_t j = wave_append_bit(&s_cnt[wib], v);
       if (v) {
         if (j < st.slab_sh) {
@@ -628,7 +645,12 @@ hipError_t launch_stream_index(const Stream& st, int which, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_stream_nee(const Stream& st, const RayQ& q, int nxt, const ShadowQ& sq, hipStream_t s) {
-  hipLaunchKernelGGL(k_stream_nee, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, st, q, nxt, sq);
+  if (st.nee_max_lights <= 1)
+    hipLaunchKernelGGL(k_stream_nee<NEE_ONE>, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, st, q, nxt, sq);
+  else if (st.nee_max_lights <= NEE_LDS_LIGHTS)
+    hipLaunchKernelGGL(k_stream_nee<NEE_LDS>, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, st, q, nxt, sq);
+  else
+    hipLaunchKernelGGL(k_stream_nee<NEE_GLOBAL>, dim3(PRT_SLAB_WAVES / 4), dim3(256), 0, s, st, q, nxt, sq);
   return hipGetLastError();
 }
 hipError_t launch_stream_shadow(const Stream& st, const ShadowQ& sq, int par, int stack_lds, int blocks,
