@@ -186,6 +186,39 @@ struct Collapser {
   }
 };
 
+// Renumbers the nodes so that those of the top PRT_TOP_LEVELS levels come
+// first, breadth first (the traversal kernels keep these in LDS); the other
+// nodes keep their depth-first order.  Internal child references (>= 0) are
+// remapped; leaf codes (< 0) are unchanged.
+void number_top_levels(BvhOutput& out) {
+  const size_t n = out.nodes.size();
+  std::vector<int32_t> top;
+  top.push_back(0);
+  for (size_t head = 0, level_end = 1, level = 1; head < top.size() && level < PRT_TOP_LEVELS;) {
+    const DevNode& d = out.nodes[(size_t)top[head]];
+    for (int i = 0; i < 4; ++i)
+      if (d.child[i] > 0 && d.lo_x[i] != std::numeric_limits<float>::infinity()) top.push_back(d.child[i]);
+    if (++head == level_end) {
+      level_end = top.size();
+      ++level;
+    }
+  }
+  std::vector<int32_t> perm(n, -1);  // old index -> new index
+  for (size_t k = 0; k < top.size(); ++k) perm[(size_t)top[k]] = (int32_t)k;
+  int32_t next = (int32_t)top.size();
+  for (size_t i = 0; i < n; ++i)
+    if (perm[i] < 0) perm[i] = next++;
+  std::vector<DevNode> nodes(n);
+  for (size_t i = 0; i < n; ++i) {
+    DevNode d = out.nodes[i];
+    for (int c = 0; c < 4; ++c)
+      if (d.child[c] >= 0) d.child[c] = perm[(size_t)d.child[c]];
+    nodes[(size_t)perm[i]] = d;
+  }
+  out.nodes.swap(nodes);
+  out.n_top = (int)top.size();
+}
+
 }  // namespace
 
 bool prt_build_bvh(const BvhInput& in, BvhOutput& out, const char** err) {
@@ -248,6 +281,7 @@ bool prt_build_bvh(const BvhInput& in, BvhOutput& out, const char** err) {
     C.emit(0, 1);
     out.depth = C.max_depth;
   }
+  number_top_levels(out);
   out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (3 * out.depth + 1 > PRT_STACK_TOTAL) {
     *err = "BVH too deep for the traversal stack";

@@ -23,6 +23,7 @@ struct BvhOutput {
   std::vector<DevNode> nodes;
   std::vector<uint32_t> order;  // leaf-order slot -> input triangle index
   int depth = 0;                // levels of 4-wide nodes
+  int n_top = 0;                // nodes 0..n_top-1: the top PRT_TOP_LEVELS levels, breadth first
   float pad = 0.0f;             // box margin used
   double build_ms = 0.0;
 };

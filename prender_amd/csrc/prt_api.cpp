@@ -329,6 +329,7 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
     if (!prt_build_bvh(bi, bo, &err)) return fail(PRT_E_UNSUPPORTED, err ? err : "bvh build failed");
   }
   s->bvh_depth = bo.depth;
+  s->n_top = bo.n_top;
   s->bvh_ms = bo.build_ms;
   std::vector<float> tri4((size_t)s->n_tris * 12);
   std::vector<float> nrm((size_t)s->n_tris * 3), alb(any_albedo ? (size_t)s->n_tris * 3 : 0);
@@ -424,6 +425,7 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   P.stack_ovf = (uint32_t*)s->ovf.p;
   P.ovf_threads = (int32_t)ovf_threads;
   P.ovf_words = ovf_words;
+  P.n_top = P.n_nodes > 0 ? std::min(bo.n_top, PRT_TOP_NODES) : 0;
   P.n_tris = (int32_t)s->n_tris;
   return PRT_OK;
 }

@@ -20,6 +20,10 @@
 #ifndef PRT_STACK_LDS
 #define PRT_STACK_LDS 16
 #endif
+// the BVH's top levels (at most 1 + 4 + 16 nodes) are numbered first and
+// copied into LDS by every traversal block
+#define PRT_TOP_LEVELS 3
+#define PRT_TOP_NODES 21
 // smallest LDS stack depth any kernel variant uses (sizes the overflow area)
 #define PRT_STACK_LDS_MIN 8
 #define PRT_STACK_TOTAL 128
@@ -111,7 +115,7 @@ struct Params {
   int32_t mlt_stride;
   int32_t mlt_depths;
   int32_t ovf_words;    // words of one thread's stack overflow area
-  int32_t pad2_;
+  int32_t n_top;        // nodes 0..n_top-1 are the top PRT_TOP_LEVELS BVH levels (kept in LDS)
 };
 // camera words of a Metropolis proposal: pixel x, y, sub-pixel x, y, tent
 // filter x, y, two spare (render.h:262-300)

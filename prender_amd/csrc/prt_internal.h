@@ -244,6 +244,7 @@ struct prt_scene {
   std::vector<int64_t> ent_n_tris;
   int64_t n_tris = 0;
   int bvh_depth = 0;
+  int n_top = 0;  // BVH nodes of the top levels (numbered first)
   double bvh_ms = 0;
   bool has_glass = false;
   int max_trav_blocks = 0;

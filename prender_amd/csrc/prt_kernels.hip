@@ -39,6 +39,7 @@
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_trace(
     Params P, const prt_ray_dev* __restrict__ rays, int64_t n, prt_hit_dev* __restrict__ hits, uint64_t* ctr) {
   PRT_DECLARE_STACK(S, P, 0);
+  PRT_TOP_CACHE(S, P.nodes, P.n_top);
   TravStats ts{0, 0, 0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const prt_ray_dev r = rays[i];
@@ -90,6 +91,7 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_exte
     Params P, RayQ q, ChunkList cl, uint32_t* fetch, double* __restrict__ hit_t, int32_t* __restrict__ hit_id,
     uint64_t* ctr) {
   PRT_DECLARE_STACK(S, P, 0);
+  PRT_TOP_CACHE(S, P.nodes, P.n_top);
   const uint32_t n_chunks = *cl.n;
   TravStats ts{0, 0, 0};
   traverse_pool<false>(
@@ -112,6 +114,7 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_shad
     Params P, Batch B, ShadowQ q, ChunkList cl, uint32_t* fetch, uint64_t* __restrict__ accum, Adjoint A,
     uint64_t* ctr) {
   PRT_DECLARE_STACK(S, P, 1);
+  PRT_TOP_CACHE(S, P.nodes, P.n_top);
   const uint32_t n_chunks = *cl.n;
   TravStats ts{0, 0, 0};
   double ge[3] = {0.0, 0.0, 0.0};

@@ -269,6 +269,8 @@ template <int NL>
 __global__ __launch_bounds__(256) void k_stream_extend(Stream st, RayQ q, int cur, double* __restrict__ hit_t,
                                                        int32_t* __restrict__ hit_id) {
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL, 0);
+  // the top-level node copy serves a batch of one scene only
+  PRT_TOP_CACHE(S, st.scenes[0].nodes, st.n_scenes == 1 ? st.scenes[0].n_top : 0);
   const ChunkDesc* desc = st.desc[cur];
   const ClaimRange cr = claim_range(st.fetch, st.ctl->ext_chunks);
   TravStats ts{0, 0, 0};
@@ -556,6 +558,7 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
 template <int NL>
 __global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq, int par) {
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL, 1);
+  PRT_TOP_CACHE(S, st.scenes[0].nodes, st.n_scenes == 1 ? st.scenes[0].n_top : 0);
   const ClaimRange cr =
       claim_range(st.fetch + (size_t)(1 + par) * PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE, st.ctl->sh_chunks);
   TravStats ts{0, 0, 0};

@@ -22,11 +22,18 @@ struct TravStats {
 // compiler keeps them apart from the LDS ones: merged, both would become
 // flat-address loads and stores, which wait on every outstanding memory
 // operation.
+// four floats as a native vector (copyable out of an address-space-qualified
+// object, unlike float4)
+typedef float prt_f4 __attribute__((ext_vector_type(4)));
+
 template <int NL>
 struct StackN {
   static constexpr int stride = 256;  // threads per block of every traversal kernel
   uint2* lds;                         // [NL][256]
   uint2* ovf;                         // 3 * BVH depth + 1 - NL entries used at most
+  // the block's LDS copy of BVH nodes 0..n_top-1 (PRT_TOP_CACHE)
+  const __attribute__((address_space(3))) prt_f4* top = nullptr;
+  int n_top = 0;
   __device__ __forceinline__ void push(int sp, int32_t node, float t) {
     const uint2 e = make_uint2((uint32_t)node, __float_as_uint(t));
     if (sp < NL) {
@@ -66,6 +73,20 @@ typedef StackN<PRT_STACK_LDS> Stack;
                                        (size_t)(P).ovf_words)}
 #define PRT_DECLARE_STACK(S, P, REGION) PRT_DECLARE_STACK_N(S, P, PRT_STACK_LDS, REGION)
 
+// Copies BVH nodes 0..NTOP-1 (the top levels, numbered first by the builder;
+// NTOP <= PRT_TOP_NODES) into the block's LDS for stack S.  Every thread of
+// the block must reach it.
+#define PRT_TOP_CACHE(S, NODES, NTOP)                                                        \
+  __shared__ float4 s_top_[PRT_TOP_NODES * 8];                                               \
+  {                                                                                          \
+    const int nt_ = (NTOP);                                                                  \
+    const float4* g_ = reinterpret_cast<const float4*>(NODES);                               \
+    for (int k_ = (int)threadIdx.x; k_ < nt_ * 8; k_ += (int)blockDim.x) s_top_[k_] = g_[k_]; \
+    __syncthreads();                                                                         \
+    S.top = (const __attribute__((address_space(3))) prt_f4*)s_top_;                         \
+    S.n_top = nt_;                                                                           \
+  }
+
 // a float >= t (t > 0): the culling bound never drops below the double hit
 __device__ __forceinline__ float fup(double t) {
   const float f = (float)t;
@@ -79,12 +100,27 @@ __device__ __forceinline__ float fup(double t) {
 // child slot in its two low bits, and a min/max network sorts the keys.  The
 // distance read back from a key is rounded down by at most 3 ulp, so culling
 // against it stays conservative; +inf stays +inf.
-__device__ __forceinline__ void node_children(const DevNode* nodes, int32_t cur, float ix, float iy, float iz,
-                                              float oix, float oiy, float oiz, float tb, float tn[4],
+// Nodes of the top levels come from the block's LDS copy.
+template <class Stk>
+__device__ __forceinline__ void node_children(const Stk& S, const DevNode* nodes, int32_t cur, float ix, float iy,
+                                              float iz, float oix, float oiy, float oiz, float tb, float tn[4],
                                               int32_t cc[4]) {
-  const float4* nd = reinterpret_cast<const float4*>(nodes + cur);
+  float4 nd[7];
+  if (cur < S.n_top) {
+    const __attribute__((address_space(3))) prt_f4* t = S.top + (size_t)cur * 8;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const prt_f4 v = t[k];
+      nd[k] = make_float4(v.x, v.y, v.z, v.w);
+    }
+  } else {
+    const float4* g = reinterpret_cast<const float4*>(nodes + cur);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) nd[k] = g[k];
+  }
   const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
-  const int4 ch = *reinterpret_cast<const int4*>(nd + 6);
+  const int4 ch = make_int4(__float_as_int(nd[6].x), __float_as_int(nd[6].y), __float_as_int(nd[6].z),
+                            __float_as_int(nd[6].w));
   const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
   const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
   uint32_t key[4];
@@ -168,7 +204,7 @@ __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double&
     while ((uint32_t)cur < (uint32_t)PRT_TRAV_DONE) {
       float tn[4];
       int32_t cc[4];
-      node_children(P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
+      node_children(S, P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
       ts.nodes++;
       push_children(S, sp, tn, cc);
       cur = (tn[0] != __builtin_inff()) ? cc[0] : pop_next();
@@ -409,7 +445,7 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
     while ((uint32_t)cur < (uint32_t)PRT_TRAV_DONE) {
       float tn[4];
       int32_t cc[4];
-      node_children(P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
+      node_children(S, P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
       ts.nodes++;
       push_children(S, sp, tn, cc);
       cur = (tn[0] != __builtin_inff()) ? cc[0] : pop_next();
