@@ -22,8 +22,12 @@
 #endif
 // the BVH's top levels (at most 1 + 4 + 16 nodes) are numbered first and
 // copied into LDS by every traversal block
+#ifndef PRT_TOP_LEVELS
 #define PRT_TOP_LEVELS 3
-#define PRT_TOP_NODES 21
+#endif
+#ifndef PRT_TOP_NODES
+#define PRT_TOP_NODES 21  // 1 + 4 + 16
+#endif
 // smallest LDS stack depth any kernel variant uses (sizes the overflow area)
 #define PRT_STACK_LDS_MIN 8
 #define PRT_STACK_TOTAL 128

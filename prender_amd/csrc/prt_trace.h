@@ -218,9 +218,17 @@ __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double&
       const uint32_t code = ~(uint32_t)lf;
       const uint32_t first = code & 0x07FFFFFFu;
       const uint32_t count = (code >> 27) + 1u;
+      // triangle k + 1 is loaded before triangle k is tested
+      const float4* tr0 = tris4 + (size_t)first * 3;
+      float4 na = tr0[0], nb = tr0[1], nc = tr0[2];
       for (uint32_t k = 0; k < count; ++k) {
-        const float4* tr = tris4 + (size_t)(first + k) * 3;
-        const float4 a = tr[0], b = tr[1], c = tr[2];
+        const float4 a = na, b = nb, c = nc;
+        if (k + 1 < count) {
+          const float4* tr = tris4 + (size_t)(first + k + 1) * 3;
+          na = tr[0];
+          nb = tr[1];
+          nc = tr[2];
+        }
         ts.tris++;
         double t, u, v;
         if (mt_test(a, b, c, o, d, t, u, v) && t < t_best) {
@@ -455,14 +463,23 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
       }
       if (!__any(lf >= 0)) break;
     }
-    // ---- test the parked leaves
+    // ---- test the parked leaves (triangle k + 1 is loaded before triangle k
+    // is tested, so a leaf costs one memory round trip rather than one per
+    // triangle)
     while (lf < 0) {
       const uint32_t code = ~(uint32_t)lf;
       const uint32_t first = code & 0x07FFFFFFu;
       const uint32_t count = (code >> 27) + 1u;
+      const float4* tr0 = tris4 + (size_t)first * 3;
+      float4 na = tr0[0], nb = tr0[1], nc = tr0[2];
       for (uint32_t k = 0; k < count; ++k) {
-        const float4* tr = tris4 + (size_t)(first + k) * 3;
-        const float4 a = tr[0], b = tr[1], c = tr[2];
+        const float4 a = na, b = nb, c = nc;
+        if (k + 1 < count) {
+          const float4* tr = tris4 + (size_t)(first + k + 1) * 3;
+          na = tr[0];
+          nb = tr[1];
+          nc = tr[2];
+        }
         ts.tris++;
         double t, u, v;
         if (mt_test(a, b, c, o, d, t, u, v) && t < t_best) {
