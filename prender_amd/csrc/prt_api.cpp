@@ -407,6 +407,7 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   P.nee = smp.next_event_estimation ? 1 : 0;
   P.max_vertices = 128;
   P.n_lights = (int32_t)s->h_lights.size();
+  P.n_materials = (int32_t)s->h_mats.size();
   P.n_analytic = (int32_t)analytic.size();
   P.n_nodes = (int32_t)bo.nodes.size();
   P.smp = 1.0 / smp.samples / (smp.supersamples * smp.supersamples);

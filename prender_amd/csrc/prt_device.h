@@ -93,7 +93,7 @@ struct Params {
   uint32_t key0, key1;
   double smp, sensor_response;
   float cutoff;
-  int32_t pad_;
+  int32_t n_materials;
   double background[3], env_light[3];
   // scene arrays
   const DevNode* nodes;

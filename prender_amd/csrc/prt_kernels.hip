@@ -156,15 +156,24 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_shad
 // against the hit of its sub-pixel's primary ray).  ADJ: with the adjoint
 // bookkeeping of a path replay (a separate instance, so forward renders
 // carry none of its registers).
-template <bool ADJ>
+// MLDS: the scene's materials (at most PRT_LDS_MATERIALS) are copied into
+// the block's LDS and read from there.
+#define PRT_LDS_MATERIALS 64
+template <bool ADJ, bool MLDS>
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
     Params P, Batch B, int primary, RayQ qin, ChunkList cl, const double* __restrict__ hit_t,
     const int32_t* __restrict__ hit_id, RayQ qout, uint32_t fan, uint2* slabs, uint64_t* __restrict__ accum,
     Adjoint A, uint64_t* ctr) {
   __shared__ uint32_t s_cnt[4];  // children appended by each wave of the block
+  __shared__ DevMaterial s_mats[MLDS ? PRT_LDS_MATERIALS : 1];
   const uint32_t wib = threadIdx.x >> 6;
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (lane_id() == 0) s_cnt[wib] = 0;
+  if (MLDS) {
+    for (int k = (int)threadIdx.x; k < P.n_materials; k += (int)blockDim.x) s_mats[k] = P.mats[k];
+    __syncthreads();
+  }
+  const DevMaterial* mats = MLDS ? s_mats : P.mats;
   const uint32_t n_samples = (uint32_t)(B.n_pixels * P.spp);
   const uint32_t n_chunks = primary ? (n_samples + 63u) >> 6 : *cl.n;
   const int ss2 = P.SS * P.SS;
@@ -230,7 +239,7 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
     C.direct = 0u;
     const uint32_t pl = slot / P.spp, k = slot % P.spp;
     VertexAdj va;
-    const d3 add = shade_vertex(P, C, o, d, t, id, T, Pw, (uint32_t)(B.pixel_base + pl), k, depth, branch, va);
+    const d3 add = shade_vertex(P, mats, C, o, d, t, id, T, Pw, (uint32_t)(B.pixel_base + pl), k, depth, branch, va);
     if (primary && P.mlt_flags != nullptr) P.mlt_flags[slot] = C.direct;
     if (ADJ && A.on) {
       const size_t px = sample_image_index(P, B, slot);
@@ -516,12 +525,16 @@ hipError_t launch_shade(const Params& P, const Batch& B, int primary, const RayQ
                         const double* hit_t, const int32_t* hit_id, const RayQ& qout, uint32_t fan, uint2* slabs,
                         uint64_t* accum, const Adjoint& A, uint64_t* ctr, hipStream_t s) {
   const dim3 grid(PRT_SLAB_WAVES / 4);
+  const bool mlds = P.n_materials <= PRT_LDS_MATERIALS;
   if (A.on)
-    hipLaunchKernelGGL(k_shade<true>, grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, fan, slabs,
-                       accum, A, ctr);
+    hipLaunchKernelGGL((k_shade<true, false>), grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, fan,
+                       slabs, accum, A, ctr);
+  else if (mlds)
+    hipLaunchKernelGGL((k_shade<false, true>), grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, fan,
+                       slabs, accum, A, ctr);
   else
-    hipLaunchKernelGGL(k_shade<false>, grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, fan, slabs,
-                       accum, A, ctr);
+    hipLaunchKernelGGL((k_shade<false, false>), grid, dim3(256), 0, s, P, B, primary, qin, cl, hit_t, hit_id, qout, fan,
+                       slabs, accum, A, ctr);
   return hipGetLastError();
 }
 hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const ChunkList& cl, const ShadowQ& sq,

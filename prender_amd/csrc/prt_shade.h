@@ -92,9 +92,10 @@ __device__ __forceinline__ void emit_child(ShadeCtx& C, d3 o, d3 d, d3 ct, d3 pw
 // Returns the radiance added to the camera sample; child rays go to the
 // output queue, and a diffuse vertex marks its child for next-event
 // estimation.
-__device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d3 d, double t, int32_t id, d3 T,
-                                           d3 Pw, uint32_t pixel, uint32_t sample, int depth, uint32_t branch,
-                                           VertexAdj& va) {
+// mats: the material table (P.mats, or a block's LDS copy of it).
+__device__ __forceinline__ d3 shade_vertex(const Params& P, const DevMaterial* mats, ShadeCtx& C, d3 o, d3 d,
+                                           double t, int32_t id, d3 T, d3 Pw, uint32_t pixel, uint32_t sample,
+                                           int depth, uint32_t branch, VertexAdj& va) {
   d3 add = mk(0.0, 0.0, 0.0);
   const d3 zero = mk(0.0, 0.0, 0.0);
   va.entity = -1;
@@ -144,7 +145,7 @@ __device__ __forceinline__ d3 shade_vertex(const Params& P, ShadeCtx& C, d3 o, d
   }
   va.entity = ei;
   if (ninv_neg) n = n * -1.0;
-  const DevMaterial& m = P.mats[mat];
+  const DevMaterial& m = mats[mat];
   d3 color;
   if (has_alb)
     color = mk(P.tri_alb[3 * (size_t)id], P.tri_alb[3 * (size_t)id + 1], P.tri_alb[3 * (size_t)id + 2]);

@@ -366,7 +366,7 @@ __global__ __launch_bounds__(256) void k_stream_shade(Stream st, RayQ qin, const
       C.slot = slot;
       const uint32_t e0 = C.n_ext;
       VertexAdj va;
-      const d3 add = shade_vertex(P, C, mk(qin.ox[ray], qin.oy[ray], qin.oz[ray]),
+      const d3 add = shade_vertex(P, P.mats, C, mk(qin.ox[ray], qin.oy[ray], qin.oz[ray]),
                                   mk(qin.dx[ray], qin.dy[ray], qin.dz[ray]), hit_t[ray], hit_id[ray], T, Pw,
                                   (uint32_t)pixel, k, depth, branch, va);
       delta = (int)(C.n_ext - e0) - 1;
