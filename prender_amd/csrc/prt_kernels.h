@@ -94,7 +94,9 @@ struct Batch {
 // slabs, which the consuming kernels walk: k_shade and k_nee in static
 // per-wave ranges, k_extend and k_shadow through PRT_FETCH_COUNTERS claim
 // counters (each 128 bytes apart, one sub-range of the chunks each).
-#define PRT_SLAB_WAVES 4096
+#ifndef PRT_SLAB_WAVES
+#define PRT_SLAB_WAVES 4096  // a multiple of 4, at most PRT_SLAB_INDEX_MAX
+#endif
 #define PRT_SLAB_INDEX_MAX 8192
 #define PRT_FETCH_COUNTERS 64
 #define PRT_FETCH_STRIDE 32

@@ -303,11 +303,22 @@ struct ClaimRange {
   uint32_t* counter;
   uint32_t b, e;
 };
+#ifndef PRT_XCD_CLAIMS
+#define PRT_XCD_CLAIMS 0
+#endif
 __device__ __forceinline__ ClaimRange claim_range(uint32_t* fetch, uint32_t n_chunks) {
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
   const uint32_t n_sub = n_waves < (uint32_t)PRT_FETCH_COUNTERS ? n_waves : (uint32_t)PRT_FETCH_COUNTERS;
-  const uint32_t sub = wave % n_sub;
+  uint32_t sub = wave % n_sub;
+  if (PRT_XCD_CLAIMS && n_sub == (uint32_t)PRT_FETCH_COUNTERS && (gridDim.x & 7u) == 0) {
+    // blocks go round-robin to the 8 XCDs: the waves of XCD x draw from
+    // sub-ranges x * n_sub / 8 .. (x + 1) * n_sub / 8 - 1, one eighth of
+    // the queue, so each XCD's L2 sees the rays of one part of the image
+    const uint32_t per = n_sub / 8u, xcd = blockIdx.x & 7u;
+    const uint32_t local = (blockIdx.x >> 3) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    sub = (uint32_t)__builtin_amdgcn_readfirstlane((int)(xcd * per + local % per));
+  }
   ClaimRange r;
   r.counter = fetch + (size_t)sub * PRT_FETCH_STRIDE;
   r.b = (uint32_t)((uint64_t)sub * n_chunks / n_sub);
