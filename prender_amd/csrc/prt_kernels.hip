@@ -187,16 +187,43 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
   C.cap = fan * 64u * (c_end - c);
   C.truncated = C.n_ext = C.pickups = C.dropped = 0;
   double ge[3] = {0.0, 0.0, 0.0};
-  // chunk descriptors are read one chunk ahead
-  uint2 dn = (!primary && c < c_end) ? cl.desc[c] : make_uint2(0u, 0u);
+  // chunk descriptors are read two chunks ahead and (forward renders: the
+  // adjoint instance has no registers to spare) the hits (t, id) of a ray one
+  // chunk ahead, so that a chunk's triangle reads, which depend on the hit,
+  // go out with its ray reads instead of after them
+  constexpr bool PF = !ADJ;
+  uint2 dn1 = make_uint2(0u, 0u), dn2 = make_uint2(0u, 0u);
+  double hn_t = 0.0;
+  int32_t hn_id = -1;
+  if (!primary && c < c_end) {
+    dn1 = cl.desc[c];
+    if (PF && lane_id() < dn1.y) {
+      hn_t = hit_t[dn1.x + lane_id()];
+      hn_id = hit_id[dn1.x + lane_id()];
+    }
+    if (PF && c + 1 < c_end) dn2 = cl.desc[c + 1];
+  }
   for (; c < c_end; ++c) {
     uint32_t i;
+    double hc_t = 0.0;
+    int32_t hc_id = -1;
     if (primary) {
       i = (c << 6) + lane_id();
       if (i >= n_samples) continue;
     } else {
-      const uint2 dc = dn;
-      if (c + 1 < c_end) dn = cl.desc[c + 1];
+      const uint2 dc = dn1;
+      if (PF) {
+        hc_t = hn_t;
+        hc_id = hn_id;
+        dn1 = dn2;
+        if (c + 1 < c_end && lane_id() < dn1.y) {
+          hn_t = hit_t[dn1.x + lane_id()];
+          hn_id = hit_id[dn1.x + lane_id()];
+        }
+        if (c + 2 < c_end) dn2 = cl.desc[c + 2];
+      } else if (c + 1 < c_end) {
+        dn1 = cl.desc[c + 1];
+      }
       if (lane_id() >= dc.y) continue;
       i = dc.x + lane_id();
     }
@@ -232,8 +259,8 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_SHADE) void k_shade(
       const uint32_t meta = qin.meta[i];
       depth = (int)(meta & 0xFFu);
       branch = META_BRANCH(meta);
-      t = hit_t[i];
-      id = hit_id[i];
+      t = PF ? hc_t : hit_t[i];
+      id = PF ? hc_id : hit_id[i];
     }
     C.slot = slot;
     C.direct = 0u;
