@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-rows", type=int, default=48, help="rows of the workload rendered by the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-min-s", type=float, default=2.0, help="least time the CPU baseline is timed over")
     ap.add_argument("--no-rmse", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5", "mlt"],
@@ -439,13 +440,24 @@ def cpu_baseline(desc, args, gpu_step_s, ref_np):
     H, W = desc.camera.height, desc.camera.width
     r0 = H // 2 - args.cpu_rows // 2
     r1 = r0 + args.cpu_rows
-    t0 = time.perf_counter()
-    rgb, var, st = o.render(seed=args.seed, rows=(r0, r1), threads=threads)
-    dt = time.perf_counter() - t0
-    n = W * (r1 - r0) * desc.spp
+    # the band is rendered again with further seeds until at least
+    # --cpu-min-s seconds are timed (a short band alone is too brief to time)
+    dt, reps, rays, rgb = 0.0, 0, 0, None
+    while True:
+        t0 = time.perf_counter()
+        img, var, st = o.render(seed=args.seed + reps, rows=(r0, r1), threads=threads)
+        dt += time.perf_counter() - t0
+        rays += st["primary"] + st["extension"] + st["shadow"] + st["pickup"]
+        if rgb is None:
+            rgb = img
+        reps += 1
+        if dt >= args.cpu_min_s or reps >= 64:
+            break
+    n = W * (r1 - r0) * desc.spp * reps
     res = {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-           "sample": f"rows {r0}..{r1 - 1} of the {W}x{H}x{desc.spp}spp workload ({n} camera samples, {dt:.1f} s)",
-           "mrays_per_s": (st["primary"] + st["extension"] + st["shadow"] + st["pickup"]) / dt / 1e6}
+           "sample": f"rows {r0}..{r1 - 1} of the {W}x{H}x{desc.spp}spp workload, {reps} render(s) "
+                     f"({n} camera samples, {dt:.1f} s)",
+           "mrays_per_s": rays / dt / 1e6}
     if ref_np is not None:
         rows = slice(H - r1, H - r0)  # image rows are stored top-down (render.h:1234)
         res["rmse_vs_16x_spp_same_rows"] = float(np.sqrt(np.mean((rgb[rows] - ref_np[rows]) ** 2)))
