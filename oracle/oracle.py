@@ -45,6 +45,9 @@ def lib():
                                  C.c_void_p, C.POINTER(C.c_uint64), C.c_int]
         L.orc_radiance_sample.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_int,
                                           C.POINTER(C.c_double)]
+        L.orc_radiance_ray.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.c_int, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_uint64)]
         L.orc_camera_rays.argtypes = [C.c_void_p, C.c_void_p]
         L.orc_qbvh_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int64),
                                     C.POINTER(C.c_int64), C.POINTER(C.c_int)]
@@ -150,6 +153,18 @@ class OracleScene:
         out = (C.c_double * 3)()
         lib().orc_radiance_sample(self.h, seed, x, y, sample, out)
         return np.array(out[:])
+
+    def radiance_ray(self, org, direction, depth: int = 0, n: int = 1, seed: int = 1):
+        """radiance() of one ray entering at path depth ``depth``, over n
+        independent paths: (mean[3], variance of the mean[3], stats)."""
+        o = (C.c_double * 3)(*[float(x) for x in org])
+        d = (C.c_double * 3)(*[float(x) for x in direction])
+        mean, var = (C.c_double * 3)(), (C.c_double * 3)()
+        stats = (C.c_uint64 * 7)()
+        rc = lib().orc_radiance_ray(self.h, seed, o, d, depth, n, mean, var, stats)
+        assert rc == 0, rc
+        keys = ["primary", "extension", "shadow", "pickup", "nodes", "tris", "truncated"]
+        return np.array(mean[:]), np.array(var[:]), dict(zip(keys, [int(x) for x in stats]))
 
     def camera_rays(self) -> np.ndarray:
         W, H = self.desc.camera.width, self.desc.camera.height

@@ -1161,6 +1161,39 @@ int orc_radiance_sample(const orc_scene* s, uint64_t seed, int x, int y, int sam
   return PRT_OK;
 }
 
+// radiance() of one given ray entering at path depth `depth` (for the
+// closed-form tests: a ray aimed at a glass interface, a mirror or a light at
+// a chosen depth).  Evaluates n independent paths (sample indices 0..n-1 of
+// pixel 0) and returns their mean and the unbiased variance of that mean.
+int orc_radiance_ray(const orc_scene* s, uint64_t seed, const double org[3], const double dir[3], int depth,
+                     int64_t n, double mean[3], double var_of_mean[3], uint64_t* stats) {
+  if (!s || !org || !dir || !mean || n <= 0 || depth < 0) return PRT_E_INVALID;
+  const Scene& S = s->S;
+  double s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+  Stats total;
+  for (int64_t i = 0; i < n; ++i) {
+    Ctx C;
+    C.S = &S;
+    C.key[0] = (uint32_t)seed; C.key[1] = (uint32_t)(seed >> 32);
+    C.pixel = 0;
+    C.sample = (uint32_t)i;
+    const V3 v = radiance(C, Ray{from(org), from(dir)}, depth, 1, V3(1, 1, 1));
+    const double c[3] = {v.x, v.y, v.z};
+    for (int k = 0; k < 3; ++k) { s1[k] += c[k]; s2[k] += c[k] * c[k]; }
+    total.add(C.st);
+  }
+  for (int k = 0; k < 3; ++k) {
+    mean[k] = s1[k] / (double)n;
+    if (var_of_mean)
+      var_of_mean[k] = n > 1 ? std::max(0.0, (s2[k] - s1[k] * s1[k] / (double)n) / (double)(n - 1)) / (double)n : 0.0;
+  }
+  if (stats) {
+    stats[0] = total.rays_primary; stats[1] = total.rays_ext; stats[2] = total.rays_shadow;
+    stats[3] = total.rays_pickup; stats[4] = total.nodes; stats[5] = total.tris; stats[6] = total.truncated;
+  }
+  return PRT_OK;
+}
+
 // Render::Run (render.h:1125-1455) over rows [row_begin, row_end) of the
 // reference's y numbering.  out_rgb / out_var: W*H*3 doubles in the image's
 // row order (image_index = (H-1-y)*W + x); rows outside the range are left
