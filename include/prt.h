@@ -220,6 +220,18 @@ int prt_trace_closest(prt_scene* scene, const prt_ray* rays, int64_t n,
  * and returns after the kernel completes. */
 int prt_trace_closest_device(prt_scene* scene, const prt_ray* d_rays, int64_t n,
                              prt_hit* d_hits, prt_stats* stats);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same query and the same hits as
+ * prt_trace_closest_device, with no hipMalloc, hipFree or event creation in
+ * the call.  Its traversal counters live in a fixed 256-byte device array of
+ * the library (one per GPU, held until the process ends), cleared on every
+ * call; no argument size changes what it needs, so no call is refused for
+ * size.  Calls that overlap on one GPU (two threads or two streams) share
+ * that array: their hits are exact, the counts in *stats are not.  stats may
+ * be NULL; no timings are reported.  Returns after the kernel completes. */
+int prt_trace_closest_device_noalloc(prt_scene* scene, const prt_ray* d_rays, int64_t n,
+                                     prt_hit* d_hits, prt_stats* stats);
 
 /* ---- render (replaces Render::Run, render.h:1125-1455) ------------------- */
 /* out_rgb: W*H*3 float, reference row order (row 0 = top, render.h:1234).

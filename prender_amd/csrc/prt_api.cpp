@@ -505,6 +505,35 @@ int prt_trace_closest_device(prt_scene* s, const prt_ray* d_rays, int64_t n, prt
   return trace_device(s, d_rays, n, d_hits, stats);
 }
 
+// The same traversal as prt_trace_closest_device with nothing allocated in
+// the call: the counters are the library's per-GPU scratch array (cleared by
+// a fill kernel, so every call starts from zero whatever the last call left)
+// and no timing events are created.  Calls that overlap on one GPU share
+// those counters (their hits stay exact; their counts mix).
+int prt_trace_closest_device_noalloc(prt_scene* s, const prt_ray* d_rays, int64_t n, prt_hit* d_hits,
+                                     prt_stats* stats) {
+  if (!s || n < 0 || (n > 0 && (!d_rays || !d_hits))) return fail(PRT_E_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(s->device));
+  uint64_t* ctr = nullptr;
+  HIPCHK(noalloc_scratch(s->device, &ctr));
+  HIPCHK(launch_fill_u64(ctr, 0, CTR_COUNT, s->stream));
+  if (n > 0)
+    HIPCHK(launch_trace(s->P, (const prt_ray_dev*)d_rays, n, (prt_hit_dev*)d_hits, ctr, s->max_trav_blocks,
+                        s->stream));
+  uint64_t c[CTR_COUNT];
+  HIPCHK(hipMemcpyAsync(c, ctr, sizeof(c), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    fill_stats(stats, c);
+    stats->extension_rays = (uint64_t)n;
+    stats->launches_extend = n > 0 ? 1 : 0;
+    stats->bytes_extend = 128 * stats->node_visits + 48 * stats->tri_tests + 48 * (uint64_t)n;
+    stats->bytes_algorithmic = stats->bytes_extend;
+  }
+  return PRT_OK;
+}
+
 int prt_trace_closest(prt_scene* s, const prt_ray* rays, int64_t n, prt_hit* hits, prt_stats* stats) {
   if (!s || n < 0 || (n > 0 && (!rays || !hits))) return fail(PRT_E_INVALID, "bad arguments");
   HIPCHK(hipSetDevice(s->device));

@@ -107,6 +107,14 @@ struct ChunkList {
 
 hipError_t launch_trace(const Params& P, const prt_ray_dev* rays, int64_t n, prt_hit_dev* hits, uint64_t* ctr,
                         int max_blocks, hipStream_t s);
+// Words of the library's file-scope device array that holds the counters of
+// the allocation-free trace entry point (one copy per GPU).
+#define PRT_NOALLOC_SCRATCH_WORDS 32
+#define PRT_NOALLOC_MAX_DEVICES 64
+static_assert(CTR_COUNT <= PRT_NOALLOC_SCRATCH_WORDS, "trace counters must fit the allocation-free scratch");
+// Device address of that array on `device`, which must be current: looked
+// up with hipGetSymbolAddress on the first call for a device and kept.
+hipError_t noalloc_scratch(int device, uint64_t** out);
 hipError_t launch_gen_primary(const Params& P, const Batch& B, const RayQ& q, hipStream_t s);
 hipError_t launch_slab_index(const uint2* slabs, uint32_t n_slabs, uint2* desc, uint32_t max_chunks, uint32_t* n_chunks,
                              uint32_t* n_entries, uint64_t* ctr, int ctr_entries, hipStream_t s);

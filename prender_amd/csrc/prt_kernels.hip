@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "prt_device.h"
 #include "prt_kernels.h"
 #include "prt_math.h"
@@ -533,6 +535,31 @@ hipError_t launch_trace(const Params& P, const prt_ray_dev* rays, int64_t n, prt
   hipLaunchKernelGGL(k_trace, dim3(grid_for(n, 256, max_blocks)), dim3(256), 0, s, P, rays, n, hits, ctr);
   return hipGetLastError();
 }
+
+// Counters of prt_trace_closest_device_noalloc: a fixed device array instead
+// of a per-scene allocation, so that the call allocates nothing.  Each GPU
+// holds its own copy until the process ends.  In host code `scratch` names a
+// host-side stand-in, not a device address: host code uses only the address
+// hipGetSymbolAddress returns for the current device (noalloc_scratch), for
+// kernel arguments and copies alike.
+static __device__ __align__(256) uint64_t scratch[PRT_NOALLOC_SCRATCH_WORDS];
+
+hipError_t noalloc_scratch(int device, uint64_t** out) {
+  static std::mutex mu;
+  static uint64_t* addr[PRT_NOALLOC_MAX_DEVICES] = {};
+  if (device < 0 || device >= PRT_NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[device] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (p == nullptr) return hipErrorInvalidSymbol;
+    addr[device] = static_cast<uint64_t*>(p);
+  }
+  *out = addr[device];
+  return hipSuccess;
+}
+
 hipError_t launch_gen_primary(const Params& P, const Batch& B, const RayQ& q, hipStream_t s) {
   hipLaunchKernelGGL(k_gen_primary, dim3(grid_for(B.n_pixels * P.SS * P.SS)), dim3(256), 0, s, P, B, q);
   return hipGetLastError();

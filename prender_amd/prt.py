@@ -53,6 +53,8 @@ def lib():
     L.prt_scene_info.argtypes = [C.c_void_p] + [C.POINTER(C.c_int64)] * 4
     L.prt_trace_closest.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(PrtStats)]
     L.prt_trace_closest_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(PrtStats)]
+    L.prt_trace_closest_device_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                                   C.POINTER(PrtStats)]
     L.prt_render.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_void_p, C.POINTER(PrtStats)]
     L.prt_render_device.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_void_p,
                                     C.POINTER(PrtStats)]
@@ -88,7 +90,8 @@ def lib():
 
 
 EXPORTED = ["prt_abi_version", "prt_last_error", "prt_device_info", "prt_scene_create", "prt_scene_destroy",
-            "prt_scene_info", "prt_trace_closest", "prt_trace_closest_device", "prt_render", "prt_render_device",
+            "prt_scene_info", "prt_trace_closest", "prt_trace_closest_device", "prt_trace_closest_device_noalloc",
+            "prt_render", "prt_render_device",
             "prt_render_mlt", "prt_render_mlt_device", "prt_render_erpt", "prt_render_erpt_device", "prt_render_batch_device", "prt_tune_batch", "prt_set_face_albedo_device", "prt_set_emission",
             "prt_hold_rr_albedo", "prt_adjoint_device", "prt_scene_file_load", "prt_scene_file_desc",
             "prt_scene_file_output", "prt_scene_file_mlt", "prt_scene_file_free", "prt_eval_expression", "prt_image_to_rgb8",
@@ -201,6 +204,14 @@ class Scene:
         check(lib().prt_trace_closest_device(self.h, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr), C.byref(st)),
               "prt_trace_closest_device")
         return st.as_dict()
+
+    def trace_device_noalloc(self, rays_ptr: int, n: int, hits_ptr: int, stats: bool = True):
+        """prt_trace_closest_device_noalloc: the same query, allocating nothing."""
+        st = PrtStats()
+        check(lib().prt_trace_closest_device_noalloc(self.h, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr),
+                                                     C.byref(st) if stats else None),
+              "prt_trace_closest_device_noalloc")
+        return st.as_dict() if stats else None
 
     # ---- render
     def render(self, seed: int = 0, samples: Optional[int] = None, variance: bool = True, **kw):
