@@ -187,7 +187,8 @@ typedef struct prt_render_opts {
     int32_t samples_override;  /* > 0 replaces prt_sampling.samples */
     int32_t batch_pixels;      /* pixels per wavefront batch (0 = auto) */
     int32_t max_depth;         /* safety cap on path vertices (0 = 128) */
-    int32_t use_graph;         /* reserved (per-scene renders launch the depth loop directly) */
+    int32_t reserved_;         /* must be 0 (per-scene renders launch their depth loop directly;
+                                  prt_batch_opts.use_graph selects graph replay for batches) */
     int32_t timing;            /* 1 = per-kernel device timing into prt_stats */
     int32_t image_f64;         /* 1 = out_rgb / out_var are double arrays (device entry points) */
     int32_t engine;            /* 0 = auto: scenes with glass stream through the batched

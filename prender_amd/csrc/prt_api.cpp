@@ -38,6 +38,10 @@
 #ifndef PRT_SHADOW_BLOCKS_PER_CU
 #define PRT_SHADOW_BLOCKS_PER_CU PRT_TRAV_BLOCKS_PER_CU
 #endif
+// the traversal-stack overflow areas hold one slot per thread of the
+// closest-hit grid; a larger shadow grid would share slots between threads
+static_assert(PRT_SHADOW_BLOCKS_PER_CU <= PRT_TRAV_BLOCKS_PER_CU,
+              "shadow traversal grid must not exceed the closest-hit grid");
 
 namespace prt {
 
@@ -472,21 +476,19 @@ static int trace_device(prt_scene* s, const prt_ray* d_rays, int64_t n, prt_hit*
   int rc;
   if ((rc = dev_alloc(s->ws.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
   HIPCHK(hipMemsetAsync(s->ws.ctr.p, 0, CTR_COUNT * sizeof(uint64_t), s->stream));
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, s->stream));
+  ScopedEvent e0, e1;
+  HIPCHK(hipEventCreate(&e0.e));
+  HIPCHK(hipEventCreate(&e1.e));
+  HIPCHK(hipEventRecord(e0.e, s->stream));
   if (n > 0)
     HIPCHK(launch_trace(s->P, (const prt_ray_dev*)d_rays, n, (prt_hit_dev*)d_hits, (uint64_t*)s->ws.ctr.p,
                         s->max_trav_blocks, s->stream));
-  HIPCHK(hipEventRecord(e1, s->stream));
+  HIPCHK(hipEventRecord(e1.e, s->stream));
   uint64_t c[CTR_COUNT];
   HIPCHK(hipMemcpyAsync(c, s->ws.ctr.p, sizeof(c), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HIPCHK(hipEventElapsedTime(&ms, e0.e, e1.e));
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     fill_stats(stats, c);
@@ -828,6 +830,7 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   // depth-synchronous batch of them drains through long, thin tails after
   // the glass splits, which the stream keeps filled with new tiles.  Same
   // image bits either way (fixed-point sums, counter-based random numbers).
+  if (o && o->reserved_ != 0) return fail(PRT_E_INVALID, "prt_render_opts.reserved_ must be 0");
   const int engine = o ? o->engine : 0;
   const bool stream_ok = !o || (o->image_f64 == 0 && o->samples_override <= 0 && o->batch_pixels <= 0);
   if (engine == 2 && !stream_ok)

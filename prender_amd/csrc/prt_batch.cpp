@@ -307,28 +307,32 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   // NEE and shadow rays on the lead scene's second stream (prt_stream.h)
   const bool overlap = PRT_BATCH_OVERLAP && nee;
   hipStream_t s2 = overlap ? lead->stream2 : s;
-  hipEvent_t e0, e1, ev[2], ev_shade, ev_sh[2];
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventCreate(&ev[0]));
-  HIPCHK(hipEventCreate(&ev[1]));
-  HIPCHK(hipEventCreateWithFlags(&ev_shade, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&ev_sh[0], hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&ev_sh[1], hipEventDisableTiming));
+  hipEvent_t e0 = nullptr, e1 = nullptr, ev[2] = {nullptr, nullptr}, ev_shade = nullptr,
+             ev_sh[2] = {nullptr, nullptr};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // every return after this point goes through cleanup (the events that were
+  // created, the graph)
   auto cleanup = [&]() {
     if (overlap) (void)hipStreamSynchronize(s2);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
-    (void)hipEventDestroy(ev_shade);
-    (void)hipEventDestroy(ev_sh[0]);
-    (void)hipEventDestroy(ev_sh[1]);
+    for (hipEvent_t e : {e0, e1, ev[0], ev[1], ev_shade, ev_sh[0], ev_sh[1]})
+      if (e) (void)hipEventDestroy(e);
   };
+  {
+    hipError_t ce = hipEventCreate(&e0);
+    if (ce == hipSuccess) ce = hipEventCreate(&e1);
+    if (ce == hipSuccess) ce = hipEventCreate(&ev[0]);
+    if (ce == hipSuccess) ce = hipEventCreate(&ev[1]);
+    if (ce == hipSuccess) ce = hipEventCreateWithFlags(&ev_shade, hipEventDisableTiming);
+    if (ce == hipSuccess) ce = hipEventCreateWithFlags(&ev_sh[0], hipEventDisableTiming);
+    if (ce == hipSuccess) ce = hipEventCreateWithFlags(&ev_sh[1], hipEventDisableTiming);
+    if (ce != hipSuccess) {
+      cleanup();
+      return fail(PRT_E_HIP, std::string("hipEventCreate: ") + hipGetErrorString(ce));
+    }
+  }
   Timer tm;
   int sh_pending = -1;  // parity of the last shadow launch not yet waited for on s
   // one iteration; with timing on, an event after each kernel (plain launches)

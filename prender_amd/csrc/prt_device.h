@@ -17,6 +17,8 @@
 #ifndef PRT_MAX_LEAF
 #define PRT_MAX_LEAF 8
 #endif
+// a leaf child stores count-1 in bits 27..30 and is negative after ~
+static_assert(PRT_MAX_LEAF >= 1 && PRT_MAX_LEAF <= 16, "leaf size must fit the 4-bit count of a child code");
 #ifndef PRT_STACK_LDS
 #define PRT_STACK_LDS 16
 #endif

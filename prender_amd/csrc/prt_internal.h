@@ -23,6 +23,17 @@ inline int fail(int code, const std::string& msg) {
   return code;
 }
 
+// A hipEvent owned by one call: destroyed on every return path.
+struct ScopedEvent {
+  hipEvent_t e = nullptr;
+  ScopedEvent() = default;
+  ScopedEvent(const ScopedEvent&) = delete;
+  ScopedEvent& operator=(const ScopedEvent&) = delete;
+  ~ScopedEvent() {
+    if (e) (void)hipEventDestroy(e);
+  }
+};
+
 #define HIPCHK(expr)                                                                     \
   do {                                                                                   \
     hipError_t e_ = (expr);                                                              \

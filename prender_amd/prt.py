@@ -111,14 +111,12 @@ def device_info():
 
 
 def render_opts(seed: int = 0, samples: Optional[int] = None, batch_pixels: int = 0, max_depth: int = 0,
-                use_graph: bool = False, timing: bool = False, image_f64: bool = False,
-                engine: int = 0) -> PrtRenderOpts:
+                timing: bool = False, image_f64: bool = False, engine: int = 0) -> PrtRenderOpts:
     o = PrtRenderOpts()
     o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
     o.samples_override = int(samples or 0)
     o.batch_pixels = int(batch_pixels)
     o.max_depth = int(max_depth)
-    o.use_graph = 1 if use_graph else 0
     o.timing = 1 if timing else 0
     o.image_f64 = 1 if image_f64 else 0
     o.engine = int(engine)

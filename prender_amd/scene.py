@@ -97,7 +97,7 @@ class PrtStats(C.Structure):
 class PrtRenderOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("samples_override", C.c_int32),
                 ("batch_pixels", C.c_int32), ("max_depth", C.c_int32),
-                ("use_graph", C.c_int32), ("timing", C.c_int32), ("image_f64", C.c_int32),
+                ("reserved_", C.c_int32), ("timing", C.c_int32), ("image_f64", C.c_int32),
                 ("engine", C.c_int32)]
 
 
