@@ -37,10 +37,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-rows", type=int, default=48, help="rows of the workload rendered by the CPU baseline")
+    ap.add_argument("--cpu-rows", type=int, default=32,
+                    help="rows of the workload rendered by the CPU baseline and the FP64 RMSE reference")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-min-s", type=float, default=2.0, help="least time the CPU baseline is timed over")
     ap.add_argument("--no-rmse", action="store_true")
+    ap.add_argument("--rmse-seeds", type=int, default=8, help="untimed GPU frames whose median RMSE is reported")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3", "cfg4", "cfg5", "mlt"],
                     help="cfg2 (default) is the metric's configuration; the others are extra measurements "
@@ -168,18 +170,74 @@ def main():
         out["roofline"]["traffic_source"] = ("profiles/k_extend_cfg2_hbm_traffic.md: rocprofv3 FETCH_SIZE x %.3f + "
                                              "WRITE_SIZE x %.3f per dispatch (scales from a 1 GiB copy / fill)"
                                              % (traffic["fetch_scale"], traffic["write_scale"]))
+    band = cpu_band(desc, args)
+    gpu_frames = None
     if not args.no_rmse:
-        # RMSE of the measured image against a 16x-spp render of the same scene
+        # untimed frames with fixed seeds (so the figure does not depend on
+        # --steps): single-frame RMSE is dominated by rare bright paths, so
+        # the median over the frames is reported
+        gpu_frames = []
+        for i in range(args.rmse_seeds):
+            scene.render_device(rgb.data_ptr(), None, seed=RMSE_SEED0 + i)
+            torch.cuda.synchronize()
+            gpu_frames.append(rgb.cpu().numpy())
         ref = torch.empty_like(rgb)
         scene.render_device(ref.data_ptr(), None, seed=999_999, samples=desc.sampling.samples * 16)
         torch.cuda.synchronize()
-        out["rmse_vs_16x_spp"] = float(torch.sqrt(torch.mean((rgb - ref) ** 2)).item())
         ref_np = ref.cpu().numpy()
-    else:
-        ref_np = None
+        whole = [float(np.sqrt(np.mean((f - ref_np) ** 2))) for f in gpu_frames]
+        out["rmse_vs_16x_spp_gpu_image"] = {"median": float(np.median(whole)), "by_seed": whole,
+                                            "reference": f"GPU render, {desc.spp * 16} spp, whole image"}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(desc, args, elapsed / args.steps, ref_np)
+        out["cpu_baseline"] = cpu_baseline(desc, args, elapsed / args.steps, band)
+    if not args.no_rmse:
+        out["rmse_vs_fp64"] = fp64_band_rmse(desc, args, band, gpu_frames, out.get("cpu_baseline"))
+        out["rmse_median"] = out["rmse_vs_fp64"]["gpu_band_median"]
+    if "cpu_baseline" in out:
+        out["cpu_baseline"].pop("_band_images", None)
     print(json.dumps(out))
+
+
+RMSE_SEED0 = 1          # GPU frames for the RMSE statistics: seeds 1..rmse_seeds
+FP64_REF_SEED = 424_242  # the FP64 reference band
+
+
+def cpu_band(desc, args):
+    """Rows [r0, r1) of the reference's y numbering that the CPU baseline
+    times and the FP64 reference covers (the band across the image centre)."""
+    H = desc.camera.height
+    rows = min(args.cpu_rows, H)
+    r0 = H // 2 - rows // 2
+    return r0, r0 + rows
+
+
+def fp64_band_rmse(desc, args, band, gpu_frames, cpu):
+    """RMSE of the GPU frames and of the CPU baseline's band renders against
+    an FP64 oracle render of the same band at 16x the spp: both sides render
+    the same estimator at the same spp, so the two figures match up to noise
+    and the throughput ratio is the speed-up at matched RMSE."""
+    from oracle.oracle import OracleScene
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    o = OracleScene(desc)
+    H = desc.camera.height
+    r0, r1 = band
+    t0 = time.perf_counter()
+    ref, _, _ = o.render(seed=FP64_REF_SEED, samples=desc.sampling.samples * 16, rows=(r0, r1), threads=threads)
+    ref_s = time.perf_counter() - t0
+    rows = slice(H - r1, H - r0)  # image rows are stored top-down (render.h:1234)
+    gpu = [float(np.sqrt(np.mean((f[rows] - ref[rows]) ** 2))) for f in gpu_frames]
+    res = {"reference": f"FP64 oracle, rows {r0}..{r1 - 1}, {desc.spp * 16} spp, seed {FP64_REF_SEED} "
+                        f"({ref_s:.1f} s on {threads} threads)",
+           "gpu_band_by_seed": gpu, "gpu_band_median": float(np.median(gpu)),
+           "gpu_seeds": f"{RMSE_SEED0}..{RMSE_SEED0 + len(gpu) - 1}"}
+    if cpu is not None and cpu.get("_band_images"):
+        c = [float(np.sqrt(np.mean((img[rows] - ref[rows]) ** 2))) for img in cpu["_band_images"]]
+        res["cpu_band_by_seed"] = c
+        res["cpu_band_median"] = float(np.median(c))
+        res["cpu_over_gpu_rmse"] = res["cpu_band_median"] / max(1e-30, res["gpu_band_median"])
+    if cpu is not None:
+        cpu.pop("_band_images", None)
+    return res
 
 
 def measured_traffic(kernel, workload_name):
@@ -431,36 +489,34 @@ def bench_inverse(args, torch, G, prt, info):
     }
 
 
-def cpu_baseline(desc, args, gpu_step_s, ref_np):
+def cpu_baseline(desc, args, gpu_step_s, band):
     """The FP64 oracle (a restatement of the reference's Render::Run) on a
     band of rows of the same workload, timed on this host's cores."""
     from oracle.oracle import OracleScene
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
     o = OracleScene(desc)
     H, W = desc.camera.height, desc.camera.width
-    r0 = H // 2 - args.cpu_rows // 2
-    r1 = r0 + args.cpu_rows
-    # the band is rendered again with further seeds until at least
-    # --cpu-min-s seconds are timed (a short band alone is too brief to time)
-    dt, reps, rays, rgb = 0.0, 0, 0, None
+    r0, r1 = band
+    # the band is rendered with the seeds of the GPU's RMSE frames (so the
+    # two RMSE medians are over the same random numbers), and again with
+    # further seeds until at least --cpu-min-s seconds are timed
+    min_reps = 1 if getattr(args, "no_rmse", False) else max(1, args.rmse_seeds)
+    dt, reps, rays, images = 0.0, 0, 0, []
     while True:
         t0 = time.perf_counter()
-        img, var, st = o.render(seed=args.seed + reps, rows=(r0, r1), threads=threads)
+        img, var, st = o.render(seed=RMSE_SEED0 + reps, rows=(r0, r1), threads=threads)
         dt += time.perf_counter() - t0
         rays += st["primary"] + st["extension"] + st["shadow"] + st["pickup"]
-        if rgb is None:
-            rgb = img
+        if len(images) < args.rmse_seeds:
+            images.append(img)
         reps += 1
-        if dt >= args.cpu_min_s or reps >= 64:
+        if (dt >= args.cpu_min_s and reps >= min_reps) or reps >= 64:
             break
     n = W * (r1 - r0) * desc.spp * reps
     res = {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-           "sample": f"rows {r0}..{r1 - 1} of the {W}x{H}x{desc.spp}spp workload, {reps} render(s) "
-                     f"({n} camera samples, {dt:.1f} s)",
-           "mrays_per_s": rays / dt / 1e6}
-    if ref_np is not None:
-        rows = slice(H - r1, H - r0)  # image rows are stored top-down (render.h:1234)
-        res["rmse_vs_16x_spp_same_rows"] = float(np.sqrt(np.mean((rgb[rows] - ref_np[rows]) ** 2)))
+           "sample": f"rows {r0}..{r1 - 1} of the {W}x{H}x{desc.spp}spp workload, {reps} render(s), seeds "
+                     f"{RMSE_SEED0}..{RMSE_SEED0 + reps - 1} ({n} camera samples, {dt:.1f} s)",
+           "mrays_per_s": rays / dt / 1e6, "_band_images": images}
     res["gpu_over_cpu"] = (desc.spp * W * H / gpu_step_s) / res["value"]
     return res
 

@@ -243,6 +243,33 @@ int prt_render(prt_scene* scene, const prt_render_opts* opts,
 int prt_render_device(prt_scene* scene, const prt_render_opts* opts,
                       float* d_rgb, float* d_var, prt_stats* stats);
 
+/* Progress images during a render: Render::Run's periodic OutputBmp(index)
+ * every imageDumpTime seconds (render.h:1390-1401; IMAGEDUMP keyword and
+ * environment variable, default 60, scene_env.h:305, 402, 1905).  Between
+ * batches of pixels, once `seconds` of wall time have passed since the render
+ * began or since the last dump (0: after every batch; < 0: never), the image
+ * so far is copied to the host and handed to `callback` (if not NULL) and
+ * written as <directory>/output_W%06d.bmp (if `directory` is not NULL: the
+ * reference's <dir of OUTPUT>/image folder; index from 0; `gamma_offset` as
+ * in prt_write_bmp).  The image so far holds the finished pixels (all their
+ * samples, sensor_response applied) and zeros elsewhere, in the reference's
+ * row order; the reference's holds every pixel with the sample passes done
+ * so far.  Dumps come only between batches, whose size
+ * prt_render_opts.batch_pixels sets (the automatic size is 2^24 camera
+ * samples: a whole 512x512x64 frame is one batch), and a render with dumps
+ * runs the depth-synchronous engine (the same image bits). */
+typedef struct prt_dump_opts {
+    double seconds;
+    const char* directory;
+    double gamma_offset;
+    void (*callback)(void* user, const float* rgb, int32_t width, int32_t height, int32_t index,
+                     double fraction_done);
+    void* user;
+} prt_dump_opts;
+/* prt_render with progress images; out_rgb / out_var as in prt_render. */
+int prt_render_dump(prt_scene* scene, const prt_render_opts* opts, const prt_dump_opts* dump,
+                    float* out_rgb, float* out_var, prt_stats* stats);
+
 /* ---- Metropolis light transport (replaces Render::RunMLT, render.h:360-780;
  *      KelemenMLT, mlt.h:70-170; generate_new_path, render.h:195-356) ------- */
 /* Primary-sample-space MLT (Kelemen et al. 2002) over the same radiance():
@@ -381,6 +408,10 @@ const char* prt_scene_file_output(const prt_scene_file* file, double* gamma_offs
  * NULL) with its mutation and pre_sample, every other field zero (library
  * defaults); an ERPT render takes opts->mutation as its chain length. */
 int prt_scene_file_mlt(const prt_scene_file* file, prt_mlt_opts* opts);
+/* The file's IMAGEDUMP seconds (default 60) after the IMAGEDUMP environment
+ * override, read with atoi as scene_env.h:402 and 1905 do (for
+ * prt_dump_opts.seconds); -1 for a NULL file. */
+int prt_scene_file_image_dump(const prt_scene_file* file);
 void prt_scene_file_free(prt_scene_file* file);
 /* One numeric field as the scene-file reader evaluates it (Expr.cpp
  * expr_eval: + - * / % ^, parentheses, pi, e, sqrt ... datan2, floor, ceil,

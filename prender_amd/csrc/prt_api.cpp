@@ -824,7 +824,49 @@ static int finish_stats(prt_scene* s, const RenderSetup& R, Timer& tm, const Cal
   return PRT_OK;
 }
 
-static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, float* d_var, prt_stats* stats) {
+// Progress images of a render with prt_dump_opts (Render::Run's periodic
+// OutputBmp, render.h:1390-1401): the device image is copied to the host
+// between batches once the interval has passed.
+struct ImageDumper {
+  const prt_dump_opts* d = nullptr;
+  std::chrono::steady_clock::time_point last;
+  int32_t index = 0;
+  std::vector<float> host;
+  std::vector<double> host64;
+
+  bool on() const { return d != nullptr && d->seconds >= 0.0; }
+  bool due() const {
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - last).count();
+    return d->seconds == 0.0 || el >= d->seconds;
+  }
+  int dump(prt_scene* s, const void* d_rgb, int f64, double fraction) {
+    const int W = s->P.W, H = s->P.H;
+    const size_t n = (size_t)W * H * 3;
+    host.resize(n);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (f64) {
+      host64.resize(n);
+      HIPCHK(hipMemcpy(host64.data(), d_rgb, n * sizeof(double), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n; ++i) host[i] = (float)host64[i];
+    } else {
+      HIPCHK(hipMemcpy(host.data(), d_rgb, n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    if (d->callback) d->callback(d->user, host.data(), W, H, index, fraction);
+    if (d->directory) {
+      char name[32];
+      std::snprintf(name, sizeof(name), "output_W%06d.bmp", (int)index);
+      const std::string path = std::string(d->directory) + "/" + name;
+      const int rc = prt_write_bmp(path.c_str(), host.data(), W, H, d->gamma_offset);
+      if (rc != PRT_OK) return rc;
+    }
+    ++index;
+    last = std::chrono::steady_clock::now();
+    return PRT_OK;
+  }
+};
+
+static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, float* d_var, prt_stats* stats,
+                         ImageDumper* dumper = nullptr) {
   HIPCHK(hipSetDevice(s->device));
   // Scenes with glass stream through the batched renderer's tile queue: a
   // depth-synchronous batch of them drains through long, thin tails after
@@ -835,7 +877,9 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   const bool stream_ok = !o || (o->image_f64 == 0 && o->samples_override <= 0 && o->batch_pixels <= 0);
   if (engine == 2 && !stream_ok)
     return fail(PRT_E_INVALID, "the streaming engine writes float images without samples_override / batch_pixels");
-  if (engine == 2 || (engine == 0 && s->has_glass && stream_ok)) {
+  const bool dumping = dumper && dumper->on();
+  if (dumping && engine == 2) return fail(PRT_E_INVALID, "progress images need the depth-synchronous engine");
+  if (engine == 2 || (engine == 0 && s->has_glass && stream_ok && !dumping)) {
     prt_batch_opts b{};
     b.seed = o ? o->seed : 0;
     b.max_depth = o ? o->max_depth : 0;
@@ -865,11 +909,25 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   if ((rc = tm.begin(o && o->timing != 0, st, &s->timer_events))) return rc;
   CallTotals tot;
   Adjoint none{};
+  const int f64 = o && o->image_f64;
+  if (dumping) {
+    // pixels not yet rendered read as 0 in the progress images
+    const int64_t n = R.n_pix * 3;
+    if (f64)
+      HIPCHK(launch_fill_u64((uint64_t*)d_rgb, 0ull, n, st));
+    else
+      HIPCHK(launch_fill_f32(d_rgb, 0.0f, n, st));
+    dumper->last = std::chrono::steady_clock::now();
+  }
   for (int64_t pb = 0; pb < R.n_pix; pb += R.bp) {
     Batch B{pb, std::min(R.bp, R.n_pix - pb)};
     if ((rc = run_batch(s, R.P, B, (uint64_t*)s->ws.accum.p, none, tm, tot))) return rc;
-    HIPCHK(launch_resolve(R.P, B, (const uint64_t*)s->ws.accum.p, d_rgb, d_var, o && o->image_f64, st));
+    HIPCHK(launch_resolve(R.P, B, (const uint64_t*)s->ws.accum.p, d_rgb, d_var, f64, st));
     if ((rc = tm.mark(T_OTHER))) return rc;
+    const int64_t done = pb + B.n_pixels;
+    if (dumping && done < R.n_pix && dumper->due() &&
+        (rc = dumper->dump(s, d_rgb, f64, (double)done / (double)R.n_pix)))
+      return rc;
   }
   uint64_t c[CTR_COUNT];
   return finish_stats(s, R, tm, tot, stats, c);
@@ -891,6 +949,27 @@ int prt_render(prt_scene* s, const prt_render_opts* opts, float* out_rgb, float*
   if (opts) o32 = *opts;
   o32.image_f64 = 0;  // host outputs are float
   rc = render_device(s, &o32, (float*)s->out_rgb.p, out_var ? (float*)s->out_var.p : nullptr, stats);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(out_rgb, s->out_rgb.p, bytes, hipMemcpyDeviceToHost, s->stream));
+  if (out_var) HIPCHK(hipMemcpyAsync(out_var, s->out_var.p, bytes, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return PRT_OK;
+}
+
+int prt_render_dump(prt_scene* s, const prt_render_opts* opts, const prt_dump_opts* dump, float* out_rgb,
+                    float* out_var, prt_stats* stats) {
+  if (!s || !out_rgb) return fail(PRT_E_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(s->device));
+  const size_t bytes = (size_t)s->P.W * s->P.H * 3 * sizeof(float);
+  int rc;
+  if ((rc = dev_alloc(s->out_rgb, bytes))) return rc;
+  if (out_var && (rc = dev_alloc(s->out_var, bytes))) return rc;
+  prt_render_opts o32{};
+  if (opts) o32 = *opts;
+  o32.image_f64 = 0;  // host outputs are float
+  ImageDumper dumper;
+  dumper.d = dump;
+  rc = render_device(s, &o32, (float*)s->out_rgb.p, out_var ? (float*)s->out_var.p : nullptr, stats, &dumper);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(out_rgb, s->out_rgb.p, bytes, hipMemcpyDeviceToHost, s->stream));
   if (out_var) HIPCHK(hipMemcpyAsync(out_var, s->out_var.p, bytes, hipMemcpyDeviceToHost, s->stream));

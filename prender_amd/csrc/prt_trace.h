@@ -167,11 +167,25 @@ __device__ __forceinline__ void push_children(Stk& S, int& sp, const float tn[4]
 // tests thus run with most lanes active instead of interleaving a node
 // step of some lanes with a leaf loop of others in every trip.
 //
+// intersect_scene_ (scene.cpp:83-214) tests the entities in index order and
+// keeps a hit only when strictly nearer, so of hits at exactly the same t the
+// one of the lowest entity index wins (a box standing on the ground mesh
+// meets it in a shared plane).  Within one mesh the QBVH keeps the first
+// triangle it finds; that order is the reference tree's, which this BVH does
+// not reproduce.
+__device__ __forceinline__ bool nearer_hit(double t, int32_t ent, double t_best, int32_t ent_best) {
+  return t < t_best || (t == t_best && ent < ent_best);
+}
+// Entity of a hit id (>= 0 triangle slot, <= -2 analytic entity, -1 none:
+// above every entity).
+__device__ __forceinline__ int32_t hit_entity(const float4* tris4, int32_t id) {
+  return id >= 0 ? __float_as_int(tris4[(size_t)id * 3].w) : (id <= -2 ? -2 - id : 0x7FFFFFFF);
+}
+
 // Every triangle a ray tests is one whose box the conservative culling
-// leaves open, and a hit is kept only when strictly closer, so the closest
-// hit is the one over all triangles; of several triangles at exactly the
-// same t, the one tested first is reported (the test order is not the
-// reference QBVH's, whose tree differs anyway).
+// leaves open (a box whose entry equals the best t stays open), so the
+// closest hit is the one over all triangles, with the tie rule of
+// nearer_hit.
 #define PRT_TRAV_DONE 0x7FFFFFFF
 template <bool ANY, class Stk>
 __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double& t_best, int32_t& id_best, double& ub,
@@ -186,10 +200,11 @@ __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double&
   float tb = t_best < 1.0e30 ? fup(t_best) : __builtin_inff();
   bool found = false;
   int sp = 0;
+  const float4* tris4 = reinterpret_cast<const float4*>(P.tris);
+  int32_t e_best = ANY ? 0 : hit_entity(tris4, id_best);
   // cur: internal node (0 <= cur < PRT_TRAV_DONE), leaf code (< 0) or
   // PRT_TRAV_DONE; lf: parked leaf (< 0) or none (>= 0)
   int32_t cur = 0, lf = 0;
-  const float4* tris4 = reinterpret_cast<const float4*>(P.tris);
   // the nearest pending entry that can still hold a closer hit
   auto pop_next = [&]() -> int32_t {
     while (sp > 0) {
@@ -231,7 +246,8 @@ __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double&
         }
         ts.tris++;
         double t, u, v;
-        if (mt_test(a, b, c, o, d, t, u, v) && t < t_best) {
+        if (mt_test(a, b, c, o, d, t, u, v) &&
+            (ANY ? t < t_best : nearer_hit(t, __float_as_int(a.w), t_best, e_best))) {
           if (ANY) {
             id_best = (int32_t)(first + k);
             return true;
@@ -240,6 +256,7 @@ __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double&
           ub = u;
           vb = v;
           id_best = (int32_t)(first + k);
+          e_best = __float_as_int(a.w);
           tb = fup(t);
           found = true;
         }
@@ -273,7 +290,7 @@ __device__ __forceinline__ void closest_hit(const Params& P, d3 o_in, d3 d, doub
   for (int k = 0; k < P.n_analytic; ++k) {
     const int32_t ei = P.analytic[k];
     double tt;
-    if (analytic_t(P.ents[ei], o, d, tt) && tt < t) {
+    if (analytic_t(P.ents[ei], o, d, tt) && nearer_hit(tt, ei, t, id <= -2 ? -2 - id : 0x7FFFFFFF)) {
       t = tt;
       id = -2 - ei;
     }
@@ -370,7 +387,7 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
   d3 o = mk(0.0, 0.0, 0.0), d = o;
   float ix = 0.0f, iy = 0.0f, iz = 0.0f, oix = 0.0f, oiy = 0.0f, oiz = 0.0f, tb = 0.0f;
   double t_best = 0.0;
-  int32_t id_best = -1, cur = PRT_TRAV_DONE, lf = 0;
+  int32_t id_best = -1, e_best = 0x7FFFFFFF, cur = PRT_TRAV_DONE, lf = 0;
   int sp = 0;
   uint32_t ri = 0;
   bool act = false, blocked = false;
@@ -425,16 +442,18 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
           o = ANY ? oo : oo + d * 0.000001;
           t_best = tmax;
           id_best = -1;
+          e_best = 0x7FFFFFFF;
           for (int k = 0; k < P.n_analytic; ++k) {
             const int32_t ei = P.analytic[k];
             double tt;
-            if (analytic_t(P.ents[ei], o, d, tt) && tt < t_best) {
+            if (analytic_t(P.ents[ei], o, d, tt) && (ANY ? tt < t_best : nearer_hit(tt, ei, t_best, e_best))) {
               if (ANY) {
                 blocked = true;
                 break;
               }
               t_best = tt;
               id_best = -2 - ei;
+              e_best = ei;
             }
           }
           lf = 0;
@@ -493,13 +512,15 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
         }
         ts.tris++;
         double t, u, v;
-        if (mt_test(a, b, c, o, d, t, u, v) && t < t_best) {
+        if (mt_test(a, b, c, o, d, t, u, v) &&
+            (ANY ? t < t_best : nearer_hit(t, __float_as_int(a.w), t_best, e_best))) {
           if (ANY) {
             blocked = true;
             break;
           }
           t_best = t;
           id_best = (int32_t)(first + k);
+          e_best = __float_as_int(a.w);
           tb = fup(t);
         }
       }

@@ -452,6 +452,8 @@ struct prt_scene_file {
   // metropolisTransport and its parameters (scene_env.h:205-208, 309-315)
   int metropolis = 0, erpt = 0;
   double mutation = 2.0, pre_sample = 0.4;
+  // IMAGEDUMP (scene_env.h:168, 305, 402, 1905)
+  int image_dump = 60;
 };
 
 namespace {
@@ -488,7 +490,8 @@ void load(const std::string& path, prt_scene_file& F) {
     if (R.is("gamma_offset\n")) { R.line(); F.gamma_offset = one(R.buf); continue; }
     if (R.is("luminance_cutoff\n")) { R.line(); cutoff = one(R.buf); continue; }
     if (R.is("tentfilter\n")) { R.line(); tent = (int)one(R.buf); continue; }
-    if (R.is("IMAGEDUMP\n") || R.is("TIMELIMIT\n") || R.is("THREAD\n")) { R.line(); (void)one(R.buf); continue; }
+    if (R.is("IMAGEDUMP\n")) { R.line(); F.image_dump = std::atoi(R.buf); continue; }
+    if (R.is("TIMELIMIT\n") || R.is("THREAD\n")) { R.line(); (void)one(R.buf); continue; }
     if (R.starts("VAR ")) {
       std::string name(R.buf + 4);
       if (!name.empty() && name.back() == '\n') name.pop_back();
@@ -746,6 +749,7 @@ void load(const std::string& path, prt_scene_file& F) {
   if (const char* s = env("ENVLIGHT_R")) envl[0] = eval1(s, vars);
   if (const char* s = env("ENVLIGHT_G")) envl[1] = eval1(s, vars);
   if (const char* s = env("ENVLIGHT_B")) envl[2] = eval1(s, vars);
+  if (const char* s = env("IMAGEDUMP")) F.image_dump = std::atoi(s);
   F.metropolis = mlt;
   F.erpt = erpt;
   if (mlt || erpt) {  // scene_env.h:1946-1968 (tent filter and the 0.25 sensor factor live in the renderers)
@@ -835,6 +839,8 @@ int prt_scene_file_mlt(const prt_scene_file* f, prt_mlt_opts* opts) {
   }
   return f->erpt ? 2 : (f->metropolis ? 1 : 0);  // main.cpp:189-196 tests ERPT first
 }
+
+int prt_scene_file_image_dump(const prt_scene_file* f) { return f ? f->image_dump : -1; }
 
 void prt_scene_file_free(prt_scene_file* f) { delete f; }
 

@@ -15,8 +15,8 @@ from typing import Optional
 
 import numpy as np
 
-from .scene import (CDesc, HIT_DTYPE, PrtBatchOpts, PrtErptOpts, PrtMltOpts, PrtRenderOpts, PrtSceneDesc, PrtStats, RAY_DTYPE,
-                    SceneDesc)
+from .scene import (CDesc, HIT_DTYPE, PrtBatchOpts, PrtDumpCallback, PrtDumpOpts, PrtErptOpts, PrtMltOpts,
+                    PrtRenderOpts, PrtSceneDesc, PrtStats, RAY_DTYPE, SceneDesc)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # PRT_LIBRARY selects another build of libprt.so (e.g. a tuning variant)
@@ -58,6 +58,9 @@ def lib():
     L.prt_render.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_void_p, C.POINTER(PrtStats)]
     L.prt_render_device.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.c_void_p, C.c_void_p,
                                     C.POINTER(PrtStats)]
+    L.prt_render_dump.argtypes = [C.c_void_p, C.POINTER(PrtRenderOpts), C.POINTER(PrtDumpOpts), C.c_void_p,
+                                  C.c_void_p, C.POINTER(PrtStats)]
+    L.prt_scene_file_image_dump.argtypes = [C.c_void_p]
     L.prt_render_erpt.argtypes = [C.c_void_p, C.POINTER(PrtErptOpts), C.c_void_p, C.POINTER(PrtStats)]
     L.prt_render_erpt_device.argtypes = [C.c_void_p, C.POINTER(PrtErptOpts), C.c_void_p, C.POINTER(PrtStats)]
     L.prt_render_mlt.argtypes = [C.c_void_p, C.POINTER(PrtMltOpts), C.c_void_p, C.POINTER(PrtStats)]
@@ -91,7 +94,7 @@ def lib():
 
 EXPORTED = ["prt_abi_version", "prt_last_error", "prt_device_info", "prt_scene_create", "prt_scene_destroy",
             "prt_scene_info", "prt_trace_closest", "prt_trace_closest_device", "prt_trace_closest_device_noalloc",
-            "prt_render", "prt_render_device",
+            "prt_render", "prt_render_dump", "prt_scene_file_image_dump", "prt_render_device",
             "prt_render_mlt", "prt_render_mlt_device", "prt_render_erpt", "prt_render_erpt_device", "prt_render_batch_device", "prt_tune_batch", "prt_set_face_albedo_device", "prt_set_emission",
             "prt_hold_rr_albedo", "prt_adjoint_device", "prt_scene_file_load", "prt_scene_file_desc",
             "prt_scene_file_output", "prt_scene_file_mlt", "prt_scene_file_free", "prt_eval_expression", "prt_image_to_rgb8",
@@ -222,6 +225,38 @@ class Scene:
         o = render_opts(seed, samples, **kw)
         check(lib().prt_render(self.h, C.byref(o), rgb.ctypes.data, var.ctypes.data if variance else None,
                                C.byref(st)), "prt_render")
+        return rgb, var, st.as_dict()
+
+    def render_dump(self, seconds: float, callback=None, directory: Optional[str] = None,
+                    gamma_offset: float = 0.0, seed: int = 0, samples: Optional[int] = None, **kw):
+        """prt_render with progress images (Render::Run's periodic OutputBmp):
+        ``callback(rgb[H,W,3] copy, index, fraction_done)`` and/or
+        <directory>/output_W%06d.bmp between batches once ``seconds`` have
+        passed.  Returns (rgb, var, stats) like render()."""
+        H, W = self.height, self.width
+        rgb = np.zeros((H, W, 3), dtype=np.float32)
+        var = np.zeros((H, W, 3), dtype=np.float32)
+        errors = []
+
+        def trampoline(user, ptr, w, h, index, fraction):
+            try:
+                img = np.ctypeslib.as_array(ptr, shape=(h * w * 3,)).reshape(h, w, 3).copy()
+                callback(img, int(index), float(fraction))
+            except Exception as e:  # an exception must not unwind through the C frames
+                errors.append(e)
+
+        d = PrtDumpOpts()
+        d.seconds = float(seconds)
+        d.directory = os.fsencode(directory) if directory else None
+        d.gamma_offset = float(gamma_offset)
+        cb = PrtDumpCallback(trampoline) if callback is not None else PrtDumpCallback()
+        d.callback = cb
+        st = PrtStats()
+        o = render_opts(seed, samples, **kw)
+        check(lib().prt_render_dump(self.h, C.byref(o), C.byref(d), rgb.ctypes.data, var.ctypes.data, C.byref(st)),
+              "prt_render_dump")
+        if errors:
+            raise errors[0]
         return rgb, var, st.as_dict()
 
     def render_device(self, rgb_ptr: int, var_ptr: Optional[int] = None, seed: int = 0,

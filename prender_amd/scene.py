@@ -101,6 +101,15 @@ class PrtRenderOpts(C.Structure):
                 ("engine", C.c_int32)]
 
 
+# void callback(void* user, const float* rgb, int32_t width, int32_t height, int32_t index, double fraction_done)
+PrtDumpCallback = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_float), C.c_int32, C.c_int32, C.c_int32, C.c_double)
+
+
+class PrtDumpOpts(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("directory", C.c_char_p), ("gamma_offset", C.c_double),
+                ("callback", PrtDumpCallback), ("user", C.c_void_p)]
+
+
 class PrtMltOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("mutation", C.c_double), ("pre_sample", C.c_double),
                 ("p_large", C.c_double), ("mutations_per_chain", C.c_int64), ("samples_override", C.c_int32),

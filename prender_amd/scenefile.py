@@ -63,6 +63,7 @@ def load(path: str) -> SceneDesc:
                          name=os.path.splitext(os.path.basename(path))[0])
         desc.output = output
         desc.gamma_offset = g.value
+        desc.image_dump = int(L.prt_scene_file_image_dump(h))  # IMAGEDUMP seconds
         return desc
     finally:
         L.prt_scene_file_free(h)

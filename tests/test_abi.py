@@ -19,6 +19,7 @@ STRUCTS = {
     "prt_material": S.PrtMaterial, "prt_entity": S.PrtEntity, "prt_camera": S.PrtCamera,
     "prt_sampling": S.PrtSampling, "prt_scene_desc": S.PrtSceneDesc, "prt_ray": S.PrtRay, "prt_hit": S.PrtHit,
     "prt_stats": S.PrtStats, "prt_render_opts": S.PrtRenderOpts, "prt_batch_opts": S.PrtBatchOpts,
+    "prt_dump_opts": S.PrtDumpOpts,
 }
 
 

@@ -127,3 +127,39 @@ def test_engines_give_identical_images(prt, name):
     assert sa["camera_samples"] == sb["camera_samples"] and sa["shadow_rays"] == sb["shadow_rays"]
     with pytest.raises(prt.PrtError):
         s.render(seed=13, engine=2, samples=1)
+
+
+@pytest.mark.parametrize("make", [lambda: G.config2(width=64, height=48, samples=2),
+                                  lambda: G.config1(width=64, height=48, samples=2)], ids=["cfg2", "cfg1_glass"])
+def test_progress_dumps_between_batches(prt, make, tmp_path):
+    # Render::Run's periodic OutputBmp (render.h:1390-1401) as prt_render_dump:
+    # with a 0-second interval a dump follows every batch but the last; each
+    # holds the finished pixels (those of the batches done, in the order the
+    # batches cover the reference's rows y = 0, 1, ...) and zeros elsewhere,
+    # and the final image equals a render without dumps bit for bit
+    d = make()
+    s = prt.Scene(d)
+    W, H = d.camera.width, d.camera.height
+    bp = 512
+    n_batches = -(-W * H // bp)
+    seen = []
+    rgb, var, st = s.render_dump(0.0, callback=lambda img, i, f: seen.append((img, i, f)),
+                                 directory=str(tmp_path), seed=8, batch_pixels=bp)
+    ref, ref_var, _ = s.render(seed=8)
+    assert np.array_equal(rgb, ref) and np.array_equal(var, ref_var)
+    assert [i for _, i, _ in seen] == list(range(n_batches - 1))
+    flat_ref = ref[::-1].reshape(-1, 3)          # pixel y*W + x of the reference's numbering
+    for img, i, f in seen:
+        done = (i + 1) * bp
+        assert f == pytest.approx(done / (W * H))
+        flat = img[::-1].reshape(-1, 3)
+        assert np.array_equal(flat[:done], flat_ref[:done])
+        assert not flat[done:].any()
+        name = tmp_path / f"output_W{i:06d}.bmp"
+        again = tmp_path / f"again_{i}.bmp"
+        prt.write_bmp(str(again), img)
+        assert name.read_bytes() == again.read_bytes()
+    # a negative interval never dumps
+    seen.clear()
+    rgb2, _, _ = s.render_dump(-1.0, callback=lambda img, i, f: seen.append(i), seed=8, batch_pixels=bp)
+    assert not seen and np.array_equal(rgb2, ref)

@@ -2,21 +2,34 @@
 
 Closest hits must match the oracle's bit for bit (entity, triangle and t;
 a different triangle is allowed only where both report the same t, i.e. a
-tie on a shared edge).  Images are compared two ways: on common random numbers
-(same seed: both sides draw the same Philox stream per path vertex, so most
-paths are identical and the per-pixel difference is far below the noise) and
-with independent seeds, where every pixel must agree within the combined
-Monte Carlo error.
+tie on a shared edge), on the seeded 2^20-ray batch (seed 0x5EED) over the
+full cfg2 scene and 2^18 rays over the full 2.1M-triangle cfg3 geometry.
+
+Images are compared two ways:
+* common random numbers (same seed: both sides draw the same Philox numbers
+  at every path vertex, so they follow the same paths): at least 99.9 % of
+  pixel channels must agree to float32 rounding (1e-5 relative), and the few
+  that do not (a path that took another branch after a last-bit difference
+  or an exact-t triangle tie) must stay within 6 standard errors of the
+  pixel mean;
+* independent seeds: at most 0.6 % of pixel channels beyond 3 combined
+  standard errors (a correct renderer gives about 0.3 %; the oracle against
+  itself measured 0-0.3 % on these scenes), at most 6 of them in any 8x8
+  tile (no cluster), every 8x8 tile mean within 4.5 and the image mean
+  within 4 standard errors.
 """
-import math
+import os
 
 import numpy as np
 import pytest
 
 from oracle.oracle import OracleScene
+from parity_util import assert_crn, assert_hits_match, assert_independent
 from prender_amd import generators as G
 
 pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes")
 
 
 @pytest.fixture(scope="module")
@@ -28,20 +41,20 @@ def prt():
     return P
 
 
-def assert_hits_match(hg, ho):
-    assert (hg["entity"] == ho["entity"]).all(), np.flatnonzero(hg["entity"] != ho["entity"])[:10]
-    hit = ho["entity"] >= 0
-    assert (hg["t"][hit] == ho["t"][hit]).all()
-    diff_prim = hg["prim"] != ho["prim"]
-    # a different triangle only on an exact tie of t
-    assert (hg["t"][diff_prim] == ho["t"][diff_prim]).all()
-    return int(diff_prim.sum())
+@pytest.fixture(scope="module")
+def cfg3_full(prt):
+    """Full cfg3 geometry (320 displaced icospheres L3-L6 + 80 boxes + ground,
+    9 lights, 2.13M triangles) at a reduced image size."""
+    d = G.config3(width=48, height=36, samples=2)
+    print("cfg3 full geometry:", len(d.entities), "entities,", d.n_triangles, "triangles", flush=True)
+    return d, prt.Scene(d), OracleScene(d)
 
 
-def test_hit_parity_cfg2(prt):
-    d = G.config2(width=64, height=64, samples=1)
-    rays = G.hit_parity_rays(d, n=1 << 16)
-    print("hit parity cfg2: 65536 rays", flush=True)
+# ------------------------------------------------------------------ closest hits
+def test_hit_parity_cfg2_full_batch(prt):
+    d = G.config2()                                   # 512x512 camera, L6 mesh (81,944 triangles)
+    rays = G.hit_parity_rays(d)                       # 2^20 rays, seed 0x5EED
+    print("hit parity cfg2:", len(rays), "rays,", d.n_triangles, "triangles", flush=True)
     hg, st = prt.Scene(d).trace(rays)
     ho, _, _ = OracleScene(d).trace(rays, "qbvh")
     ties = assert_hits_match(hg, ho)
@@ -64,88 +77,125 @@ def test_hit_parity_cfg1_spheres(prt):
     assert_hits_match(hg, ho)
 
 
-def sigma_test(a, va, b, vb, k=4.0):
-    """Fraction of pixel channels outside k combined sigma.  Pixels whose
-    samples all agree have zero variance; the 1e-6 relative term keeps the
-    float32 rounding of the device image from counting there."""
-    s = np.sqrt(va + vb) + 1e-6 * np.abs(b) + 1e-12
-    return float((np.abs(a - b) > k * s).mean())
+def test_hit_ties_across_entities_take_the_lowest_index(prt):
+    # three copies of one ground mesh in one device BVH: every ray meets all
+    # three at the same t, and intersect_scene_'s entity order (scene.cpp:83-214)
+    # reports entity 0; a sphere cut by the plane adds analytic-entity ties
+    rng = np.random.default_rng(5)
+    g = np.linspace(-10.0, 10.0, 9)
+    quads = [np.array([[g[i], 0, g[j]], [g[i], 0, g[j + 1]], [g[i + 1], 0, g[j + 1]], [g[i + 1], 0, g[j]]])
+             for i in range(8) for j in range(8)]
+    v, f = G.quad_mesh(quads)
+    from prender_amd.scene import Camera, Material, Polygon, Sampling, SceneDesc
+    d = SceneDesc(camera=Camera(width=8, height=8), sampling=Sampling(samples=1),
+                  materials=[Material(color=(0.5,) * 3)], entities=[Polygon(v, f, 0) for _ in range(3)])
+    org = np.column_stack([rng.uniform(-9, 9, 4096), rng.uniform(0.5, 5, 4096), rng.uniform(-9, 9, 4096)])
+    dirs = np.column_stack([rng.normal(size=4096) * 0.3, -np.ones(4096), rng.normal(size=4096) * 0.3])
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    rays = G.rays_to_array(org, dirs)
+    hg, _ = prt.Scene(d).trace(rays)
+    ho, _, _ = OracleScene(d).trace(rays, "qbvh")
+    hit = ho["entity"] >= 0                          # a few tilted rays leave the 20 x 20 ground
+    assert hit.mean() > 0.9 and (ho["entity"][hit] == 0).all()
+    assert_hits_match(hg, ho)
 
 
+def test_hit_parity_cfg3_full_geometry(prt, cfg3_full):
+    d, sc, o = cfg3_full
+    rays = G.hit_parity_rays(d, n=1 << 18)
+    print("hit parity cfg3:", len(rays), "rays", flush=True)
+    hg, _ = sc.trace(rays)
+    ho, _, _ = o.trace(rays, "qbvh")
+    ties = assert_hits_match(hg, ho)
+    assert (ho["entity"] >= 0).mean() > 0.5
+    print("ties", ties, "entities hit", len(np.unique(ho["entity"])))
+
+
+# ------------------------------------------------------------------ images, common random numbers
 def test_render_parity_cfg1_common_random_numbers(prt):
     d = G.config1(width=64, height=48, samples=4)
     print("render cfg1 64x48x16spp seed 9", flush=True)
     rg, vg, st = prt.Scene(d).render(seed=9)
     ro, vo, _ = OracleScene(d).render(seed=9)
     assert np.isfinite(rg).all()
-    err = np.abs(rg - ro)
-    # identical random numbers: the per-pixel difference stays far below the
-    # Monte Carlo error of either image
-    assert np.median(err) < 1e-3 * max(1e-6, float(np.median(ro)))
-    assert sigma_test(rg, vg, ro, vo, 1.0) < 0.01
-    print("cfg1 CRN: max err", err.max(), "mean", rg.mean(), ro.mean(), st)
-
-
-def test_render_parity_cfg1_independent(prt):
-    d = G.config1(width=32, height=24, samples=64)
-    rg, vg, _ = prt.Scene(d).render(seed=1)
-    ro, vo, _ = OracleScene(d).render(seed=2)
-    frac = sigma_test(rg, vg, ro, vo, 3.0)
-    # about 0.3% of normal deviates exceed 3 sigma; allow for the heavy
-    # tails of path-traced pixels
-    assert frac < 0.02, frac
-    assert abs(rg.mean() - ro.mean()) < 4 * math.sqrt((vg.mean() + vo.mean()) / rg[..., 0].size)
+    assert_crn(rg, vg, ro, vo, "cfg1")
 
 
 def test_render_parity_cfg2_common_random_numbers(prt):
-    d = G.config2(width=48, height=48, samples=2, level=4)
-    print("render cfg2 48x48x8spp (L4 mesh) seed 4", flush=True)
+    d = G.config2(width=48, height=48, samples=2)
+    print("render cfg2 48x48x8spp (L6 mesh) seed 4", flush=True)
     rg, vg, st = prt.Scene(d).render(seed=4)
     ro, vo, so = OracleScene(d).render(seed=4)
-    err = np.abs(rg - ro)
-    assert np.median(err) < 1e-3 * max(1e-6, float(np.median(ro)))
-    assert sigma_test(rg, vg, ro, vo, 1.0) < 0.01
     assert st["shadow_rays"] > 0
-    print("cfg2 CRN: max err", err.max(), "gpu", st, "cpu", so)
+    assert_crn(rg, vg, ro, vo, "cfg2")
 
 
-def test_hit_parity_cfg3_instances(prt):
-    # cfg3 generator with a reduced instance set: many POLYGON entities in one
-    # device BVH against the oracle's per-entity QBVH loop
-    d = G.config3(width=64, height=48, samples=1, scale=0.08)
-    rays = G.hit_parity_rays(d, n=1 << 15)
-    print("hit parity cfg3 (scaled):", len(d.entities), "entities,", d.n_triangles, "tris", flush=True)
-    hg, _ = prt.Scene(d).trace(rays)
-    ho, _, _ = OracleScene(d).trace(rays, "qbvh")
-    assert_hits_match(hg, ho)
+def test_render_parity_cfg3_common_random_numbers(prt, cfg3_full):
+    d, sc, o = cfg3_full
+    print("render cfg3 (full geometry) 48x36x8spp seed 12", flush=True)
+    rg, vg, st = sc.render(seed=12)
+    ro, vo, so = o.render(seed=12)
+    assert st["shadow_rays"] > 0
+    assert_crn(rg, vg, ro, vo, "cfg3")
 
 
-def test_render_parity_cfg3_common_random_numbers(prt):
-    d = G.config3(width=48, height=36, samples=2, scale=0.08)
-    print("render cfg3 (scaled) 48x36x8spp seed 12", flush=True)
-    rg, vg, st = prt.Scene(d).render(seed=12)
-    ro, vo, so = OracleScene(d).render(seed=12)
-    err = np.abs(rg - ro)
-    assert np.median(err) < 1e-3 * max(1e-6, float(np.median(ro)))
-    assert sigma_test(rg, vg, ro, vo, 1.0) < 0.02
-    print("cfg3 CRN: max err", err.max(), "means", rg.mean(), ro.mean(), "gpu", st, "cpu", so)
+def _scene_file(name, width, height, samples):
+    from prender_amd import scenefile
+    d = scenefile.load(os.path.join(GOLDEN, name))
+    d.camera.width, d.camera.height = width, height
+    d.sampling.samples = samples
+    return d
 
 
 def test_reference_scene_file_parity(prt):
     # the reference's own proj/test/CornellBox.txt through the scene-file
     # reader, at reduced size; it keeps the file's defaults, among them the
     # luminance clamp (0.025) and DEPTH 8 / DEPTH_LIMIT 64
-    import os
-    from prender_amd import scenefile
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scenes", "CornellBox.txt")
-    d = scenefile.load(path)
-    d.camera.width, d.camera.height = 64, 48
-    d.sampling.samples = 2
+    d = _scene_file("CornellBox.txt", 64, 48, 2)
     assert d.sampling.luminance_cutoff == 0.025
     print("render CornellBox.txt 64x48x8spp seed 17", flush=True)
     rg, vg, st = prt.Scene(d).render(seed=17)
     ro, vo, _ = OracleScene(d).render(seed=17)
-    err = np.abs(rg - ro)
-    assert np.median(err) < 1e-3 * max(1e-6, float(np.median(ro)))
-    assert sigma_test(rg, vg, ro, vo, 1.0) < 0.01
-    print("CornellBox.txt CRN: max err", err.max(), "mean", rg.mean(), ro.mean(), st)
+    assert_crn(rg, vg, ro, vo, "CornellBox.txt")
+
+
+def test_reference_area_light_scene_file_parity(prt):
+    # a byte copy of the reference's proj/test/CornellBox面光源.txt: NEE on, a
+    # UVPlane light with |U| = |V| = 20, so camera and bounce rays see a
+    # 400 x 400 emitter (plane.h:232-238) while NEE samples 20 x 20
+    # (directLight_radiance.h:356-365); mirror and glass spheres; the
+    # luminance clamp 0.025
+    d = _scene_file("CornellBoxAreaLight.txt", 64, 48, 2)
+    assert d.sampling.next_event_estimation == 1 and d.sampling.luminance_cutoff == 0.025
+    light = [e for e in d.entities if d.materials[e.material].is_light()]
+    assert len(light) == 1 and np.linalg.norm(light[0].U) == 20.0
+    print("render CornellBoxAreaLight.txt 64x48x8spp seed 23", flush=True)
+    rg, vg, st = prt.Scene(d).render(seed=23)
+    ro, vo, so = OracleScene(d).render(seed=23)
+    assert st["shadow_rays"] > 0 and st["pickup_hits"] > 0
+    assert_crn(rg, vg, ro, vo, "CornellBoxAreaLight.txt")
+
+
+# ------------------------------------------------------------------ images, independent seeds
+def test_render_parity_cfg1_independent(prt):
+    d = G.config1(width=32, height=24, samples=64)
+    print("render cfg1 32x24x256spp, seeds 1 (GPU) / 2 (oracle)", flush=True)
+    rg, vg, _ = prt.Scene(d).render(seed=1)
+    ro, vo, _ = OracleScene(d).render(seed=2)
+    assert_independent(rg, vg, ro, vo, "cfg1")
+
+
+def test_render_parity_cfg2_independent(prt):
+    d = G.config2(width=32, height=32, samples=64)
+    print("render cfg2 (L6 mesh) 32x32x256spp, seeds 3 (GPU) / 4 (oracle)", flush=True)
+    rg, vg, _ = prt.Scene(d).render(seed=3)
+    ro, vo, _ = OracleScene(d).render(seed=4)
+    assert_independent(rg, vg, ro, vo, "cfg2")
+
+
+def test_render_parity_cfg3_independent(prt, cfg3_full):
+    d, sc, o = cfg3_full
+    print("render cfg3 (full geometry) 48x36x256spp, seeds 5 (GPU) / 6 (oracle)", flush=True)
+    rg, vg, _ = sc.render(seed=5, samples=64)
+    ro, vo, _ = o.render(seed=6, samples=64)
+    assert_independent(rg, vg, ro, vo, "cfg3")

@@ -132,6 +132,20 @@ def test_nearest_entity_wins():
     assert h["entity"] == 1
 
 
+def test_equal_t_lowest_entity_wins():
+    # intersect_scene_ keeps a hit only when strictly nearer, entity by entity
+    # in index order (scene.cpp:83-214): two meshes with the same triangle
+    # meet every ray at the same t and the first entity is reported, in
+    # either order of creation
+    tri = np.array([[0, 0, 0], [4, 0, 0], [0, 4, 0]], np.float32)
+    for mats in ([0, 1], [1, 0]):
+        d = SceneDesc(camera=Camera(width=4, height=4), sampling=Sampling(samples=1),
+                      materials=[Material(color=(0.5,) * 3), Material(color=(0.2,) * 3)],
+                      entities=[Polygon(tri, [[0, 1, 2]], mats[0]), Polygon(tri, [[0, 1, 2]], mats[1])])
+        h = trace_one(OracleScene(d), [1.0, 1.0, 2.0], [0, 0, -1.0])
+        assert h["entity"] == 0 and h["t"] == pytest.approx(2.0 - 1e-6, abs=1e-12)
+
+
 # ---------------------------------------------------------------- QBVH
 @pytest.fixture(scope="module")
 def cfg2_small():

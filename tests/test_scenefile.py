@@ -234,6 +234,19 @@ def test_metropolis_keywords(tmp_path, monkeypatch):
     assert scenefile.mlt_settings(off) == (1, {"mutation": 8.0, "pre_sample": 0.4})
 
 
+def test_image_dump_keyword(tmp_path, monkeypatch):
+    # IMAGEDUMP (scene_env.h:305 default 60, 402 atoi of the line, 1905 atoi
+    # of the environment variable, which wins)
+    monkeypatch.delenv("IMAGEDUMP", raising=False)
+    assert scenefile.load(CORNELL).image_dump == 60
+    path = write(tmp_path, "d.txt", "IMAGEDUMP\n15\nOBJECT\nsphere 1 2 3 4\ncolor 1 1 1\n\n")
+    assert scenefile.load(path).image_dump == 15
+    frac = write(tmp_path, "f.txt", "IMAGEDUMP\n2.9\nOBJECT\nsphere 1 2 3 4\ncolor 1 1 1\n\n")
+    assert scenefile.load(frac).image_dump == 2        # atoi, not the expression reader
+    monkeypatch.setenv("IMAGEDUMP", "7")
+    assert scenefile.load(path).image_dump == 7
+
+
 def test_commented_object_blocks_are_skipped(tmp_path):
     # '#OBJECT' hides the block: its lines are top-level lines that match no keyword
     path = write(tmp_path, "c.txt", "\n".join([
