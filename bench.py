@@ -29,6 +29,11 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def progress(msg):
+    """A phase line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 METRIC = "Mrays/s + samples/s at matched RMSE vs host CPU; adjoint grad evals/s, 1 GPU"
 
 
@@ -84,6 +89,7 @@ def main():
         print(json.dumps(bench_mlt(args, torch, G, prt, info)))
         return
     desc, wl_text = workload(G, args.workload)
+    progress(f"{args.workload}: scene build")
     W, H, spp = desc.camera.width, desc.camera.height, desc.spp
     t0 = time.time()
     scene = prt.Scene(desc)
@@ -94,6 +100,7 @@ def main():
     for i in range(args.warmup):
         scene.render_device(rgb.data_ptr(), var.data_ptr(), seed=1000 + i, timing=True)
     torch.cuda.synchronize()
+    progress(f"timed renders: {args.steps}")
     stats = []
     t0 = time.perf_counter()
     for i in range(args.steps):
@@ -164,6 +171,13 @@ def main():
     }
     tu = out["traversal_union"]
     tu["frac"] = tu["achieved"] / HBM_PEAK_GBS
+    if streamed:
+        # the streaming engine traces camera rays in the same launches as
+        # bounce rays; their rate is measured on their own
+        out["mrays_per_s_by_class"]["primary"] = primary_trace_rate(torch, G, desc, scene)
+        out["mrays_per_s_by_class"]["primary_note"] = (
+            "the frame's W*H*SS^2 camera rays traced on their own through prt_trace_closest_device "
+            "(device buffers, HIP-event time of the trace kernel, median of 5)")
     traffic = measured_traffic(dom, args.workload)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic["traffic_bytes_per_launch"]
@@ -176,6 +190,7 @@ def main():
         # untimed frames with fixed seeds (so the figure does not depend on
         # --steps): single-frame RMSE is dominated by rare bright paths, so
         # the median over the frames is reported
+        progress(f"RMSE frames: {args.rmse_seeds}")
         gpu_frames = []
         for i in range(args.rmse_seeds):
             scene.render_device(rgb.data_ptr(), None, seed=RMSE_SEED0 + i)
@@ -189,13 +204,28 @@ def main():
         out["rmse_vs_16x_spp_gpu_image"] = {"median": float(np.median(whole)), "by_seed": whole,
                                             "reference": f"GPU render, {desc.spp * 16} spp, whole image"}
     if not args.no_cpu:
+        progress(f"CPU baseline: rows {band[0]}..{band[1] - 1}")
         out["cpu_baseline"] = cpu_baseline(desc, args, elapsed / args.steps, band)
     if not args.no_rmse:
+        progress("FP64 reference band at 16x spp")
         out["rmse_vs_fp64"] = fp64_band_rmse(desc, args, band, gpu_frames, out.get("cpu_baseline"))
         out["rmse_median"] = out["rmse_vs_fp64"]["gpu_band_median"]
     if "cpu_baseline" in out:
         out["cpu_baseline"].pop("_band_images", None)
     print(json.dumps(out))
+
+
+def primary_trace_rate(torch, G, desc, scene, reps=5):
+    """Mrays/s of the frame's unique camera rays through the closest-hit
+    entry point (rays and hits resident on the device)."""
+    from prender_amd.scene import HIT_DTYPE
+    cam = G.camera_rays(desc)
+    n = len(cam)
+    rays = torch.from_numpy(cam.view(np.uint8).reshape(-1)).to("cuda")
+    hits = torch.empty(n * HIT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    scene.trace_device(rays.data_ptr(), n, hits.data_ptr())
+    ms = sorted(scene.trace_device(rays.data_ptr(), n, hits.data_ptr())["ms_total"] for _ in range(reps))
+    return n / ms[reps // 2] / 1e3
 
 
 RMSE_SEED0 = 1          # GPU frames for the RMSE statistics: seeds 1..rmse_seeds
@@ -343,6 +373,7 @@ def bench_batch(args, torch, G, prt, info):
         opts = prt.batch_opts()
         tune_ns = None
     else:
+        progress("batch tuner")
         opts, tune_ns = prt.tune_batch(scenes, ptrs)
     tune_s = time.time() - t0
     opts.use_graph = 0
@@ -362,6 +393,7 @@ def bench_batch(args, torch, G, prt, info):
     topts = prt.PrtBatchOpts.from_buffer_copy(opts)
     topts.use_graph, topts.timing, topts.seed = -1, 1, args.seed + args.steps - 1
     tst = prt.render_batch_device(scenes, ptrs, opts=topts)
+    progress("per-scene renders of the same scenes")
     # sequential per-scene renders of the same work
     for s, o in zip(scenes, outs):  # warm-up: workspaces
         s.render_device(o.data_ptr(), seed=1)
@@ -377,7 +409,7 @@ def bench_batch(args, torch, G, prt, info):
     rays = sum_("primary_rays") + sum_("extension_rays") + sum_("shadow_rays")
     ext_b, ext_ms, n_it = tst["bytes_extend"], tst["ms_extend"], tst["launches_extend"]
     achieved = ext_b / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
-    return {
+    out = {
         "metric": METRIC, "value": samples / batched_s, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * batched_s, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (64 seeded procedural scenes)",
@@ -401,6 +433,37 @@ def bench_batch(args, torch, G, prt, info):
                      "bytes_per_launch": ext_b / max(1, n_it), "avg_launch_ms": ext_ms / max(1, n_it)},
         "device": info["arch"], "scene_build_s": build_s,
     }
+    if not args.no_cpu:
+        progress("CPU baseline: oracle on a sample of the scenes")
+        out["cpu_baseline"] = cpu_batch_baseline(descs, samples / batched_s)
+    return out
+
+
+def cpu_batch_baseline(descs, gpu_samples_per_s, every=8, rows=8):
+    """The FP64 oracle on a bounded sample of the batch: every `every`-th
+    scene, a band of `rows` rows through its middle at the scene's spp (the
+    reference renders one scene per process; the sample's samples/s is the
+    host rate for this mix of scenes)."""
+    from oracle.oracle import OracleScene
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    n = rays = 0
+    dt = 0.0
+    picked = descs[::every]
+    for d in picked:
+        o = OracleScene(d)
+        H, W = d.camera.height, d.camera.width
+        r0 = H // 2 - rows // 2
+        t0 = time.perf_counter()
+        _, _, st = o.render(seed=1, rows=(r0, r0 + rows), threads=threads)
+        dt += time.perf_counter() - t0
+        n += W * rows * d.spp
+        rays += st["primary"] + st["extension"] + st["shadow"] + st["pickup"]
+        o.close()
+    v = n / dt
+    return {"value": v, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": f"{len(picked)} of the {len(descs)} scenes (every {every}th), {rows} rows through the middle "
+                      f"of each at its spp ({n} camera samples, {dt:.1f} s; BVH builds excluded)",
+            "mrays_per_s": rays / dt / 1e6, "gpu_over_cpu": gpu_samples_per_s / v}
 
 
 def bench_inverse(args, torch, G, prt, info):
@@ -450,6 +513,7 @@ def bench_inverse(args, torch, G, prt, info):
     losses = []
     alb_err0 = float((alb.detach() - alb_true).abs().mean())
     t0 = time.perf_counter()
+    progress(f"Adam steps: {args.adam_steps}")
     for it in range(args.adam_steps):
         opt.zero_grad()
         out = render(alb, em, sc, 1, 2, seed=2 * it + 100, adjoint_seed=2 * it + 101)
@@ -461,13 +525,14 @@ def bench_inverse(args, torch, G, prt, info):
         losses.append(float(loss.detach()))
     torch.cuda.synchronize()
     opt_s = time.perf_counter() - t0
+    recovery = albedo_recovery(G, d, sc, alb.detach(), alb_true)
     # faces seen by the camera move towards the truth; report the error over
     # faces whose albedo changed
     moved = (alb.detach() - 0.5).abs().sum(dim=1) > 1e-3
     alb_err = float((alb.detach() - alb_true).abs().mean())
     alb_err_moved0 = float((0.5 - alb_true[moved]).abs().mean()) if bool(moved.any()) else None
     alb_err_moved = float((alb.detach()[moved] - alb_true[moved]).abs().mean()) if bool(moved.any()) else None
-    return {
+    out = {
         "metric": METRIC, "value": args.adam_steps / opt_s, "unit": "grad evals/s", "n_gpus": 1,
         "steps": args.adam_steps, "warmup": 1, "ms_per_step": 1e3 * opt_s / args.adam_steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
@@ -485,8 +550,60 @@ def bench_inverse(args, torch, G, prt, info):
                                    for k, m in top.items()},
                                 "note": "faces ranked by |d image sum / d albedo|; most faces cover less than a "
                                         "pixel, so only pixel averages of neighbouring faces are constrained"},
+        "albedo_recovery": recovery,
         "device": info["arch"],
     }
+    if not args.no_cpu:
+        progress("CPU baseline: FP64 forward + adjoint")
+        out["cpu_baseline"] = cpu_adjoint_baseline(d, fwd_s + adj_s)
+    return out
+
+
+def albedo_recovery(G, desc, scene, alb, alb_true, mesh=1):
+    """What the camera can identify: the albedo seen through each pixel
+    (the mean over its SS^2 camera rays of the albedo of the mesh face each
+    ray hits first), and the albedo of faces hit by at least SS^2 camera rays
+    (a footprint of one pixel or more); mean absolute error of the initial
+    guess (0.5) and of the recovered albedo against the truth."""
+    cam = G.camera_rays(desc)
+    hits, _ = scene.trace(cam)
+    ss2 = desc.sampling.supersamples ** 2
+    ent = hits["entity"].reshape(-1, ss2)
+    face = hits["prim"].reshape(-1, ss2)
+    a = alb.cpu().numpy().astype(np.float64)
+    t = alb_true.cpu().numpy().astype(np.float64)
+    on = (ent == mesh).all(axis=1)                    # every ray of the pixel meets the mesh first
+    pf = face[on]
+    seen_t, seen_a = t[pf].mean(axis=1), a[pf].mean(axis=1)
+    counts = np.bincount(face[ent == mesh], minlength=a.shape[0])
+    big = counts >= ss2
+    return {"pixels_on_mesh": int(on.sum()),
+            "pixel_albedo_mae": {"start": float(np.abs(0.5 - seen_t).mean()),
+                                 "end": float(np.abs(seen_a - seen_t).mean())},
+            "faces_with_pixel_footprint": int(big.sum()),
+            "footprint_faces_mae": {"start": float(np.abs(0.5 - t[big]).mean()) if big.any() else None,
+                                    "end": float(np.abs(a[big] - t[big]).mean()) if big.any() else None},
+            "note": "camera rays of the fixed sub-pixel grid traced through prt_trace_closest; a pixel counts "
+                    "when all its rays hit the mesh first"}
+
+
+def cpu_adjoint_baseline(desc, gpu_eval_s):
+    """One gradient evaluation of the inverse-rendering workload on the host:
+    the FP64 oracle's forward render and its adjoint (the CPU restatement of
+    the path-replay adjoint) of the full image at the workload's spp."""
+    from oracle.oracle import OracleScene
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    o = OracleScene(desc)
+    H, W = desc.camera.height, desc.camera.width
+    t0 = time.perf_counter()
+    o.render(seed=1, threads=threads)
+    t1 = time.perf_counter()
+    o.adjoint(seed=2, dimage=np.ones((H, W, 3)), entity=1, light=2, threads=threads)
+    t2 = time.perf_counter()
+    return {"value": 1.0 / (t2 - t0), "unit": "grad evals/s", "cores": threads, "kind": "port",
+            "sample": f"one forward ({t1 - t0:.1f} s) + one adjoint ({t2 - t1:.1f} s) of the full "
+                      f"{W}x{H}x{desc.spp}spp image",
+            "forward_s": t1 - t0, "adjoint_s": t2 - t1, "gpu_over_cpu": (t2 - t0) / gpu_eval_s}
 
 
 def cpu_baseline(desc, args, gpu_step_s, band):

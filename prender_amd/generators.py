@@ -388,6 +388,21 @@ def camera_basis(desc: SceneDesc):
     return np.asarray(cam.position, dtype=np.float64), center, sx, sy
 
 
+def camera_rays(desc: SceneDesc) -> np.ndarray:
+    """The unique camera rays of a frame: one per sub-pixel of the fixed
+    SSxSS grid, r = (s + 0.5) / SS (render.h:1248-1269), in Run's order
+    (y, x, sy, sx) with y = 0 the bottom image row."""
+    pos, center, sx, sy = camera_basis(desc)
+    W, H, ss = desc.camera.width, desc.camera.height, desc.sampling.supersamples
+    y, x, syi, sxi = np.meshgrid(np.arange(H), np.arange(W), np.arange(ss), np.arange(ss), indexing="ij")
+    r1 = ((sxi + 0.5) / ss + x) / W - 0.5
+    r2 = ((syi + 0.5) / ss + y) / H - 0.5
+    sp = center + sx * r1.reshape(-1, 1) + sy * r2.reshape(-1, 1)
+    d = sp - pos
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return rays_to_array(np.broadcast_to(pos, d.shape), d)
+
+
 def hit_parity_rays(desc: SceneDesc, n: int = 1 << 20, seed: int = 0x5EED) -> np.ndarray:
     """Seeded closest-hit batch: half camera rays through uniform sub-pixel
     positions, half with origins uniform in the mesh AABB and directions

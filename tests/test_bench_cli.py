@@ -52,3 +52,15 @@ def test_fp64_band_rmse_matches_equal_spp_renders():
     assert len(res["gpu_band_by_seed"]) == 3 and res["gpu_band_median"] > 0
     assert res["cpu_band_median"] > 0
     assert 0.5 < res["cpu_over_gpu_rmse"] < 2.0
+
+
+def test_cpu_batch_and_adjoint_baselines():
+    sys.path.insert(0, ROOT)
+    import bench
+    from prender_amd import generators as G
+    descs = G.config4(n_scenes=4, tri_range=(500, 2000), resolutions=(16,), spps=(8,))
+    r = bench.cpu_batch_baseline(descs, 1e6, every=2, rows=4)
+    assert r["kind"] == "port" and r["value"] > 0 and "2 of the 4 scenes" in r["sample"]
+    d = G.config5(width=16, height=16, samples=1)
+    a = bench.cpu_adjoint_baseline(d, 1e-3)
+    assert a["unit"] == "grad evals/s" and a["value"] > 0 and a["forward_s"] > 0 and a["adjoint_s"] > 0

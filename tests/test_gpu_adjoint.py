@@ -100,6 +100,57 @@ def test_gpu_adjoint_matches_finite_differences(torch_prt, russian_roulette):
     assert abs(ge.sum() - fd_e) < 0.02 * abs(fd_e)
 
 
+def test_gpu_adjoint_finite_differences_uniform_parameters(torch_prt):
+    # 256 parameters drawn uniformly from all 3 * 81,920 face albedos (no
+    # selection by gradient).  Faces reached only by deep bounces have
+    # gradients many orders below the largest; a 1e-3 step moves the
+    # fixed-point image of such a face by less than its rounding, so the
+    # difference quotient cannot resolve them.  Resolvable parameters
+    # (|g| >= 1e-4 max |g|) must agree within 2 %; for the others adjoint and
+    # difference quotient must both be below that floor.
+    torch, prt = torch_prt
+    d = G.config5(width=64, height=64, samples=8)
+    d.sampling.depth = d.sampling.depth_limit = 250      # no Russian roulette to flip
+    sc = prt.Scene(d)
+    dimg = np.random.default_rng(4).uniform(-1, 1, (64, 64, 3)).astype(np.float32)
+    dimg_t = torch.from_numpy(dimg).cuda()
+    nf = d.entities[1].indices.shape[0]
+    g_t = torch.zeros((nf, 3), dtype=torch.float32, device="cuda")
+    seed = 13
+    sc.adjoint_device(dimg_t.data_ptr(), 1, g_t.data_ptr(), 2, seed=seed)
+    g = g_t.cpu().numpy().astype(np.float64)
+    rng = np.random.default_rng(6)
+    flat = rng.choice(nf * 3, 256, replace=False)
+    faces, chans = flat // 3, flat % 3
+    alb0 = torch.from_numpy(d.entities[1].face_albedo.copy()).cuda()
+    eps = 1e-3
+    floor = 1e-4 * np.abs(g).max()
+    res_rel, low = [], []
+    for f, c in zip(faces, chans):
+        a = alb0.clone(); a[f, c] += eps
+        sc.set_face_albedo_device(1, a.data_ptr()); gp = _gpu_loss(torch, sc, dimg_t, seed)
+        a = alb0.clone(); a[f, c] -= eps
+        sc.set_face_albedo_device(1, a.data_ptr()); gm = _gpu_loss(torch, sc, dimg_t, seed)
+        step = float(np.float32(alb0[f, c].item() + eps)) - float(np.float32(alb0[f, c].item() - eps))
+        fd = (gp - gm) / step
+        if abs(g[f, c]) >= floor:
+            res_rel.append(abs(g[f, c] - fd) / max(abs(fd), 1e-12))
+        else:
+            low.append((g[f, c], fd))
+    sc.set_face_albedo_device(1, alb0.data_ptr())
+    res_rel = np.array(res_rel)
+    low = np.array(low).reshape(-1, 2)
+    print(f"uniform FD check: {len(res_rel)} of 256 resolvable, within 2%: "
+          f"{np.mean(res_rel < 0.02) if len(res_rel) else float('nan'):.3f}, median rel err "
+          f"{np.median(res_rel) if len(res_rel) else float('nan'):.2e}; {len(low)} below the floor {floor:.2e}: "
+          f"max |adjoint| {np.abs(low[:, 0]).max() if len(low) else 0:.2e}, "
+          f"max |fd| {np.abs(low[:, 1]).max() if len(low) else 0:.2e}", flush=True)
+    assert len(res_rel) >= 16
+    assert np.mean(res_rel < 0.02) >= 0.99
+    if len(low):
+        assert np.abs(low[:, 1]).max() < 2 * floor
+
+
 def test_autograd_function_and_adam_decrease_loss(torch_prt):
     torch, prt = torch_prt
     from prender_amd.autograd import render

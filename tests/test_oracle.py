@@ -197,6 +197,15 @@ def test_camera_rays_closed_form():
     assert math.degrees(math.acos(top @ c / np.linalg.norm(top))) == pytest.approx(15.0, abs=1e-9)
 
 
+def test_generator_camera_rays_match_run():
+    # the numpy camera rays used by the measurements equal Run's
+    d = G.config1(width=16, height=12)
+    r = G.camera_rays(d)
+    o = OracleScene(d).camera_rays()
+    np.testing.assert_allclose(r["dir"], o["dir"], atol=1e-14)
+    np.testing.assert_array_equal(r["org"], o["org"])
+
+
 def test_light_order_by_power():
     d = G.config3(width=8, height=8, samples=1, scale=0.02)
     o = OracleScene(d)
