@@ -393,6 +393,28 @@ def bench_batch(args, torch, G, prt, info):
     topts = prt.PrtBatchOpts.from_buffer_copy(opts)
     topts.use_graph, topts.timing, topts.seed = -1, 1, args.seed + args.steps - 1
     tst = prt.render_batch_device(scenes, ptrs, opts=topts)
+    # the persistent engine: one launch over every sample of the batch; one
+    # trial per register budget (2, 3 or 4 waves per SIMD), the fastest timed
+    progress("persistent engine")
+    pers = {}
+    trials = {}
+    for wv in (2, 3, 4):
+        po = prt.batch_opts(seed=args.seed + args.steps - 1, persistent=True, trav_blocks_per_cu=wv)
+        trials[wv] = prt.render_batch_device(scenes, ptrs, opts=po)["ms_total"]
+    best_w = min(trials, key=trials.get)
+    torch.cuda.synchronize()
+    pers_same = all(torch.equal(a, b) for a, b in zip(last, outs))
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        po = prt.batch_opts(seed=args.seed + i, persistent=True, trav_blocks_per_cu=best_w)
+        pst = prt.render_batch_device(scenes, ptrs, opts=po)
+    torch.cuda.synchronize()
+    pers_s = (time.perf_counter() - t0) / args.steps
+    pers = {"samples_per_s": samples / pers_s, "ms_per_step": 1e3 * pers_s,
+            "waves_per_simd": best_w, "trial_ms_by_waves": trials,
+            "images_identical_to_graph_form": pers_same,
+            "over_graph_form": batched_s / pers_s, "over_per_scene": None,
+            "kernel_ms": pst["ms_extend"], "resolve_ms": pst["ms_other"]}
     progress("per-scene renders of the same scenes")
     # sequential per-scene renders of the same work
     for s, o in zip(scenes, outs):  # warm-up: workspaces
@@ -409,6 +431,7 @@ def bench_batch(args, torch, G, prt, info):
     rays = sum_("primary_rays") + sum_("extension_rays") + sum_("shadow_rays")
     ext_b, ext_ms, n_it = tst["bytes_extend"], tst["ms_extend"], tst["launches_extend"]
     achieved = ext_b / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    pers["over_per_scene"] = seq_s / pers_s
     out = {
         "metric": METRIC, "value": samples / batched_s, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * batched_s, "higher_is_better": True, "scaling": "weak",
@@ -419,6 +442,7 @@ def bench_batch(args, torch, G, prt, info):
         "mrays_per_s": rays / (batched_s * args.steps) / 1e6,
         "per_scene_launches": {"samples_per_s": samples / seq_s, "ms_per_step": 1e3 * seq_s},
         "batched_over_per_scene": seq_s / batched_s,
+        "persistent": pers,
         "images_identical": max_diff == 0.0, "max_abs_diff": max_diff,
         "batch_options": opts.as_dict(), "tuner_ns_per_sample": tune_ns, "tune_s": tune_s,
         "iterations_per_step": stats[-1]["launches_extend"],

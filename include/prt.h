@@ -347,6 +347,14 @@ typedef struct prt_batch_opts {
     int32_t tile_stride;        /* render only every k-th tile (tuning trials; 0 = all) */
     int32_t max_depth;          /* safety cap on path vertices (0 = 128) */
     int32_t timing;             /* 1 = per-kernel device timing into prt_stats (plain launches only) */
+    int32_t persistent;         /* 1 = one persistent launch renders every camera sample of the batch:
+                                   waves claim 64-sample chunks from one shared counter and follow each
+                                   sample's whole path (shading, NEE, shadow rays, glass branches) with
+                                   no ray queues and no waits between waves; then one launch folds the
+                                   pixels.  Same images bit for bit.  tile_samples, inflight_tiles,
+                                   use_graph and tile_stride do not apply; the batch may hold at most
+                                   2^28 camera samples (a fixed-point accumulator each). */
+    int32_t pad_;
 } prt_batch_opts;
 /* d_rgb[i] / d_var[i]: W_i*H_i*3 float device buffers of scene i (d_var or
  * its entries may be NULL).  All scenes must live on one device; the call

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "_lib")
 LIB_PATH = os.path.join(LIB_DIR, "libprt.so")
-SOURCES = ["prt_kernels.hip", "prt_stream.hip", "prt_mlt.hip", "prt_api.cpp", "prt_batch.cpp", "prt_mlt.cpp", "bvh.cpp",
+SOURCES = ["prt_kernels.hip", "prt_stream.hip", "prt_persist.hip", "prt_mlt.hip", "prt_api.cpp", "prt_batch.cpp", "prt_mlt.cpp", "bvh.cpp",
            "scene_file.cpp", "image_io.cpp"]
 HEADERS = ["prt_device.h", "prt_kernels.h", "prt_math.h", "prt_trace.h", "prt_shade.h", "prt_stream.h",
            "prt_internal.h", "prt_mlt.h", "bvh.h"]

@@ -31,6 +31,9 @@ struct StreamWorkspace {
   DevBuf q[2][RAYQ_COLS], hit_t, hit_id, sq[SHADOWQ_COLS];
   DevBuf accum, ctr, ctl, slot_info, live, free_list, new_tiles;
   DevBuf scenes, tiles, outs, ovf, runs, nruns, desc[2], sh_desc, fetch;
+  // persistent engine: per-sample accumulators, pending glass segments per
+  // thread, the chunk / sample / pixel bases of the scenes and the claim counter
+  DevBuf p_accum, p_pend, p_bases;
   int64_t ovf_threads = 0;
   int ovf_words = 0;
   StreamCtl* h_ctl = nullptr;  // pinned, two copies (double-buffered status reads)
@@ -40,7 +43,7 @@ struct StreamWorkspace {
     for (auto& b : sq) b.release();
     DevBuf* all[] = {&hit_t,  &hit_id, &accum, &ctr,     &ctl,     &slot_info, &live,    &free_list, &new_tiles,
                      &scenes, &tiles,  &outs,  &ovf,     &runs,    &nruns,     &desc[0], &desc[1],   &sh_desc,
-                     &fetch};
+                     &fetch,  &p_accum, &p_pend, &p_bases};
     for (DevBuf* b : all) b->release();
     if (h_ctl) (void)hipHostFree(h_ctl);
     h_ctl = nullptr;
@@ -87,6 +90,8 @@ struct BatchConfig {
   int tile_stride = 1;
   int max_vertices = 128;
   bool timing = false;
+  bool persistent = false;
+  int persist_blocks_per_cu = 2;
   uint64_t seed = 0;
 };
 
@@ -119,6 +124,9 @@ int resolve_config(prt_scene* const* scenes, int n, const prt_batch_opts* o, Bat
   c.tile_stride = std::max(1, opts.tile_stride);
   c.max_vertices = opts.max_depth > 0 ? std::min(opts.max_depth, 255) : 128;
   c.seed = opts.seed;
+  c.persistent = opts.persistent != 0;
+  c.persist_blocks_per_cu = opts.trav_blocks_per_cu > 0 ? opts.trav_blocks_per_cu : 2;
+  if (c.persistent && opts.tile_stride > 1) return fail(PRT_E_INVALID, "the persistent engine renders every sample");
   return PRT_OK;
 }
 
@@ -494,6 +502,174 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   return PRT_OK;
 }
 
+// Chunks of 64 samples a wave claims at a time in the persistent engine
+// (256 samples: one claim atomic per 256 camera samples)
+#ifndef PRT_PERSIST_CLAIM
+#define PRT_PERSIST_CLAIM 4
+#endif
+constexpr uint64_t kMaxPersistSamples = (uint64_t)1 << 28;
+
+// One batched render as a single persistent launch (prt_persist.hip), then
+// one fold of every pixel.  The scenes' parameters, the images' pointers and
+// the traversal overflow area live in the lead scene's stream workspace, as
+// for the tile-queue engine.
+int run_persistent(prt_scene* const* scenes, int n, const BatchConfig& c, float* const* d_rgb, float* const* d_var,
+                   prt_stats* stats, BatchResult* res) {
+  prt_scene* lead = scenes[0];
+  HIPCHK(hipSetDevice(lead->device));
+  // chunk / sample / pixel bases of the scenes (host layout of Persist)
+  std::vector<uint32_t> chunk_base(n + 1);
+  std::vector<uint64_t> sample_base(n + 1);
+  std::vector<int64_t> pixel_base(n + 1);
+  uint64_t samples = 0, chunks = 0;
+  int64_t pixels = 0;
+  int ovf_words = 2;
+  for (int i = 0; i < n; ++i) {
+    const Params& P = scenes[i]->P;
+    const uint64_t ns = (uint64_t)P.W * P.H * P.spp;
+    chunk_base[i] = (uint32_t)chunks;
+    sample_base[i] = samples;
+    pixel_base[i] = pixels;
+    chunks += (ns + 63) / 64;
+    samples += ns;
+    pixels += (int64_t)P.W * P.H;
+    ovf_words = std::max(ovf_words, (int)P.ovf_words);
+    if (chunks >= ((uint64_t)1 << 31)) return fail(PRT_E_INVALID, "too many camera samples in the batch");
+  }
+  chunk_base[n] = (uint32_t)chunks;
+  sample_base[n] = samples;
+  pixel_base[n] = pixels;
+  if (samples > kMaxPersistSamples)
+    return fail(PRT_E_INVALID, "the persistent engine takes at most 2^28 camera samples per batch (" +
+                                   std::to_string(samples) + " asked)");
+  // one 256-thread block per CU puts one wave on each SIMD: blocks per CU
+  // picks the register budget (2 = the compiler's own, about 230 VGPRs;
+  // 3 and 4 spill part of the state), capped by what fits
+  const int waves = std::min(4, std::max(2, c.persist_blocks_per_cu));
+  const int stack = c.stack_lds <= 8 ? 8 : 16;
+  int occ = 0;
+  HIPCHK(persist_blocks_per_cu(stack, waves, &occ));
+  const int cu = std::max(1, lead->cu_count);
+  const int per_cu = std::max(1, std::min(occ, waves));
+  const int blocks = cu * per_cu;
+  const int64_t threads = (int64_t)blocks * 256;
+
+  if (!lead->sws) lead->sws = new (std::nothrow) StreamWorkspace();
+  if (!lead->sws) return fail(PRT_E_NOMEM, "out of host memory");
+  StreamWorkspace& w = *lead->sws;
+  int rc;
+  const int64_t ovf_threads = (int64_t)cu * kMaxTravBlocksPerCu * 256;
+  if ((int64_t)w.ovf.bytes < (int64_t)PRT_OVF_REGIONS * ovf_threads * ovf_words * (int64_t)sizeof(uint32_t) ||
+      w.ovf_words < ovf_words) {
+    if ((rc = dev_alloc(w.ovf, (size_t)PRT_OVF_REGIONS * ovf_threads * ovf_words * sizeof(uint32_t)))) return rc;
+    w.ovf_threads = ovf_threads;
+    w.ovf_words = ovf_words;
+  }
+  if (threads > w.ovf_threads) return fail(PRT_E_INTERNAL, "persistent grid larger than the overflow area");
+  if ((rc = dev_alloc(w.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
+  if ((rc = dev_alloc(w.scenes, (size_t)n * sizeof(Params)))) return rc;
+  if ((rc = dev_alloc(w.outs, (size_t)n * 2 * sizeof(float*)))) return rc;
+  if ((rc = dev_alloc(w.p_accum, (size_t)samples * 3 * sizeof(uint64_t)))) return rc;
+  if ((rc = dev_alloc(w.p_pend, (size_t)threads * PRT_PEND_DEPTH * sizeof(PendRec)))) return rc;
+  const size_t bases_bytes = (size_t)(n + 1) * (sizeof(uint32_t) + sizeof(uint64_t) + sizeof(int64_t)) + 64;
+  if ((rc = dev_alloc(w.p_bases, bases_bytes))) return rc;
+  hipStream_t s = lead->stream;
+
+  std::vector<Params> params(n);
+  std::vector<float*> outs(2 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    Params P = scenes[i]->P;
+    P.key0 = (uint32_t)c.seed;
+    P.key1 = (uint32_t)(c.seed >> 32);
+    P.max_vertices = c.max_vertices;
+    P.stack_ovf = (uint32_t*)w.ovf.p;
+    P.ovf_threads = (int32_t)w.ovf_threads;
+    P.ovf_words = w.ovf_words;
+    params[i] = P;
+    outs[i] = d_rgb[i];
+    outs[n + i] = d_var ? d_var[i] : nullptr;
+  }
+  // bases: sample_base (u64, 8-aligned), pixel_base (i64), chunk_base (u32), then the claim counter
+  char* b = (char*)w.p_bases.p;
+  uint64_t* d_sample = (uint64_t*)b;
+  int64_t* d_pixel = (int64_t*)(b + (n + 1) * sizeof(uint64_t));
+  uint32_t* d_chunk = (uint32_t*)(b + (n + 1) * (sizeof(uint64_t) + sizeof(int64_t)));
+  uint32_t* d_next = d_chunk + (n + 1);
+  for (int i = 1; i < n; ++i)
+    if (scenes[i]->stream != s) HIPCHK(hipStreamSynchronize(scenes[i]->stream));
+  HIPCHK(hipMemcpyAsync(w.scenes.p, params.data(), params.size() * sizeof(Params), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(w.outs.p, outs.data(), outs.size() * sizeof(float*), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_sample, sample_base.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_pixel, pixel_base.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_chunk, chunk_base.data(), (n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+
+  Persist P{};
+  P.scenes = (const Params*)w.scenes.p;
+  P.chunk_base = d_chunk;
+  P.sample_base = d_sample;
+  P.next = d_next;
+  P.accum = (uint64_t*)w.p_accum.p;
+  P.pend = (PendRec*)w.p_pend.p;
+  P.ctr = (uint64_t*)w.ctr.p;
+  P.out_rgb = (float* const*)w.outs.p;
+  P.out_var = (float* const*)w.outs.p + n;
+  P.n_chunks = (uint32_t)chunks;
+  P.claim = PRT_PERSIST_CLAIM;
+  P.n_scenes = n;
+
+  ScopedEvent e0, e1, e2;
+  HIPCHK(hipEventCreate(&e0.e));
+  HIPCHK(hipEventCreate(&e1.e));
+  HIPCHK(hipEventCreate(&e2.e));
+  // counters and the claim counter start at zero (fill kernels, not memsets)
+  HIPCHK(launch_fill_u64(P.ctr, 0ull, CTR_COUNT, s));
+  HIPCHK(launch_fill_u32(d_next, 0u, 1, s));
+  HIPCHK(hipEventRecord(e0.e, s));
+  HIPCHK(launch_persist(P, stack, waves, blocks, s));
+  HIPCHK(hipEventRecord(e1.e, s));
+  HIPCHK(launch_persist_resolve(P, pixels, d_pixel, s));
+  HIPCHK(hipEventRecord(e2.e, s));
+  uint64_t ctrh[CTR_COUNT];
+  HIPCHK(hipMemcpyAsync(ctrh, P.ctr, sizeof(ctrh), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  float ms_render = 0, ms_resolve = 0;
+  HIPCHK(hipEventElapsedTime(&ms_render, e0.e, e1.e));
+  HIPCHK(hipEventElapsedTime(&ms_resolve, e1.e, e2.e));
+  if (ctrh[CTR_DROP] != 0)
+    return fail(PRT_E_INTERNAL, "persistent render: " + std::to_string(ctrh[CTR_DROP]) + " glass segments dropped");
+  // every sample's camera ray and every emitted child was traced exactly once
+  if (ctrh[CTR_QUEUED] != samples || ctrh[CTR_TRACED] != ctrh[CTR_QUEUED] + ctrh[CTR_EXT])
+    return fail(PRT_E_INTERNAL, "persistent render traced " + std::to_string(ctrh[CTR_TRACED]) + " segments for " +
+                                    std::to_string(ctrh[CTR_QUEUED]) + " samples (of " + std::to_string(samples) +
+                                    ") and " + std::to_string(ctrh[CTR_EXT]) + " children");
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->node_visits = ctrh[CTR_NODES] + ctrh[CTR_SNODES];
+    stats->tri_tests = ctrh[CTR_TRIS] + ctrh[CTR_STRIS];
+    stats->shadow_node_visits = ctrh[CTR_SNODES];
+    stats->shadow_tri_tests = ctrh[CTR_STRIS];
+    stats->pickup_hits = ctrh[CTR_PICKUP];
+    stats->truncated_paths = ctrh[CTR_TRUNC];
+    stats->camera_samples = samples;
+    stats->primary_rays = ctrh[CTR_QUEUED];  // one camera ray per sample in this engine
+    stats->extension_rays = ctrh[CTR_EXT];
+    stats->shadow_rays = ctrh[CTR_SHADOW];
+    stats->ms_total = ms_render + ms_resolve;
+    stats->ms_extend = ms_render;  // the one launch does every stage
+    stats->ms_traversal = ms_render;
+    stats->ms_other = ms_resolve;
+    stats->launches_extend = 1;
+    stats->bytes_extend = 128 * ctrh[CTR_NODES] + 48 * ctrh[CTR_TRIS] + 48 * ctrh[CTR_TRACED];
+    stats->bytes_shadow = 128 * ctrh[CTR_SNODES] + 48 * ctrh[CTR_STRIS] + 48 * ctrh[CTR_SHADOW];
+    stats->bytes_algorithmic = stats->bytes_extend + stats->bytes_shadow;
+  }
+  if (res) {
+    res->ms = ms_render + ms_resolve;
+    res->samples = samples;
+  }
+  return PRT_OK;
+}
+
 int check_batch_args(prt_scene* const* scenes, int n, float* const* d_rgb) {
   if (!scenes || n <= 0 || !d_rgb) return fail(PRT_E_INVALID, "bad arguments");
   for (int i = 0; i < n; ++i) {
@@ -513,6 +689,7 @@ int prt_render_batch_device(prt_scene* const* scenes, int n_scenes, const prt_ba
   if ((rc = check_batch_args(scenes, n_scenes, d_rgb))) return rc;
   BatchConfig c;
   if ((rc = resolve_config(scenes, n_scenes, opts, c))) return rc;
+  if (c.persistent) return run_persistent(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
   return run_stream(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
 }
 

@@ -25,6 +25,19 @@ enum {
   CTR_COUNT = 15
 };
 
+// A path segment a lane of the persistent renderer keeps for later: a glass
+// vertex at depth <= 2 emits two children; the lane follows the first and
+// keeps the second here.  A lane's path is a binary tree split at depths 0,
+// 1 and 2 walked depth first, so at most one kept segment per split depth.
+struct PendRec {
+  double o[3], d[3];
+  float t[3], p[3], n[3];
+  uint32_t meta;
+  uint32_t pad_[2];
+};
+static_assert(sizeof(PendRec) == 96, "pending segment record must be 96 bytes");
+#define PRT_PEND_DEPTH 3
+
 struct prt_ray_dev {
   double org[3];
   double dir[3];

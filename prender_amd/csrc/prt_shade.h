@@ -18,6 +18,16 @@
 #include "prt_kernels.h"
 #include "prt_math.h"
 
+// Parameters of scene s (s wave-uniform) through a constant-address-space
+// pointer, so that their loads are scalar loads as kernel arguments' are.
+__device__ __forceinline__ const Params& scene_params(const Params* scenes, int s) {
+  const uint64_t a = (uint64_t)(scenes + s);
+  const uint64_t au = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
+                      (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+  const __attribute__((address_space(4))) Params* Pc = (const __attribute__((address_space(4))) Params*)au;
+  return *(const Params*)Pc;
+}
+
 // Camera ray of sub-pixel (sx, sy) of pixel (x, y) (render.h:1248-1269:
 // r = (s + 0.5)/SS, no jitter).
 __device__ __forceinline__ void camera_ray(const Params& P, int x, int y, int sx, int sy, d3& o, d3& d) {
@@ -88,12 +98,60 @@ __device__ __forceinline__ void emit_child(ShadeCtx& C, d3 o, d3 d, d3 ct, d3 pw
   C.n_ext++;
 }
 
+// The children of one vertex in the persistent renderer: the first becomes
+// the lane's next segment (registers), a second goes to the lane's pending
+// stack.  Throughput, pickup weight and NEE normal are rounded to float as a
+// queue entry stores them (emit_child above), so a segment continues
+// bit-identically to the queued renderers.
+struct LaneCtx {
+  d3 o0, d0;
+  float t0[3], p0[3], n0[3];
+  uint32_t meta0;
+  bool has0;
+  PendRec* pend;
+  int npend;
+  uint32_t truncated, n_ext, pickups, dropped;
+  uint32_t direct;
+};
+
+__device__ __forceinline__ void emit_child(LaneCtx& C, d3 o, d3 d, d3 ct, d3 pw, uint32_t meta, d3 nrm) {
+  const float t[3] = {(float)ct.x, (float)ct.y, (float)ct.z};
+  const float p[3] = {(float)pw.x, (float)pw.y, (float)pw.z};
+  const float n[3] = {(float)nrm.x, (float)nrm.y, (float)nrm.z};
+  if (!C.has0) {
+    C.has0 = true;
+    C.o0 = o;
+    C.d0 = d;
+    for (int k = 0; k < 3; ++k) {
+      C.t0[k] = t[k];
+      C.p0[k] = p[k];
+      C.n0[k] = n[k];
+    }
+    C.meta0 = meta;
+  } else if (C.npend < PRT_PEND_DEPTH) {
+    PendRec& r = C.pend[C.npend++];
+    r.o[0] = o.x; r.o[1] = o.y; r.o[2] = o.z;
+    r.d[0] = d.x; r.d[1] = d.y; r.d[2] = d.z;
+    for (int k = 0; k < 3; ++k) {
+      r.t[k] = t[k];
+      r.p[k] = p[k];
+      r.n[k] = n[k];
+    }
+    r.meta = meta;
+  } else {
+    C.dropped++;
+    return;
+  }
+  C.n_ext++;
+}
+
 // One path vertex: the radiance() body for the hit (or miss) of ray (o, d).
 // Returns the radiance added to the camera sample; child rays go to the
-// output queue, and a diffuse vertex marks its child for next-event
-// estimation.
+// context's sink (an output queue, or a lane's own segments), and a diffuse
+// vertex marks its child for next-event estimation.
 // mats: the material table (P.mats, or a block's LDS copy of it).
-__device__ __forceinline__ d3 shade_vertex(const Params& P, const DevMaterial* mats, ShadeCtx& C, d3 o, d3 d,
+template <class Ctx>
+__device__ __forceinline__ d3 shade_vertex(const Params& P, const DevMaterial* mats, Ctx& C, d3 o, d3 d,
                                            double t, int32_t id, d3 T, d3 Pw, uint32_t pixel, uint32_t sample,
                                            int depth, uint32_t branch, VertexAdj& va) {
   d3 add = mk(0.0, 0.0, 0.0);

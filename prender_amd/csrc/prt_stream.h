@@ -148,6 +148,35 @@ struct Stream {
   int32_t nee_max_lights;   // most lights of any scene (picks how k_stream_nee reads light records)
 };
 
+// The persistent form of the batched renderer (prt_persist.hip): one launch
+// renders every camera sample of the batch.  Camera samples are numbered
+// scene by scene (scene s owns samples sample_base[s] ..., pixel-major, spp
+// per pixel) and cut into 64-sample chunks (scene s owns chunks
+// chunk_base[s] .. chunk_base[s+1]-1; a scene's last chunk may be short).
+struct Persist {
+  const Params* scenes;         // [n_scenes]
+  const uint32_t* chunk_base;   // [n_scenes + 1]
+  const uint64_t* sample_base;  // [n_scenes + 1]; sample_base[n_scenes] = all samples
+  uint32_t* next;               // claim counter: first chunk not yet handed out
+  uint64_t* accum;              // [all samples][3] fixed point, written once per sample
+  PendRec* pend;                // [grid threads][PRT_PEND_DEPTH] pending glass segments
+  uint64_t* ctr;                // [CTR_COUNT]
+  float* const* out_rgb;        // [n_scenes] W*H*3 device images
+  float* const* out_var;        // [n_scenes] (entries may be null)
+  uint32_t n_chunks;
+  uint32_t claim;               // chunks per claim
+  int32_t n_scenes;
+  int32_t pad_;
+};
+
+// stack_lds: 8 or 16 LDS stack entries (32 does not fit beside the kernel's
+// registers); waves: waves per SIMD its registers are budgeted for (2..4).
+hipError_t launch_persist(const Persist& p, int stack_lds, int waves, int blocks, hipStream_t s);
+// Resident 256-thread blocks per CU of that kernel instance.
+hipError_t persist_blocks_per_cu(int stack_lds, int waves, int* blocks);
+hipError_t launch_persist_resolve(const Persist& p, int64_t n_pixels_total, const int64_t* d_pixel_base,
+                                  hipStream_t s);
+
 hipError_t launch_stream_init(const Stream& st, hipStream_t s);
 hipError_t launch_stream_refill(const Stream& st, int cur, hipStream_t s);
 hipError_t launch_stream_gen(const Stream& st, const RayQ& q, int cur, int blocks, hipStream_t s);

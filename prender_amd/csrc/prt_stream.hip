@@ -24,15 +24,6 @@ __device__ __forceinline__ uint32_t round64(uint32_t n) { return (n + 63u) & ~63
 // chunks per claim of the traversal kernels
 #define FETCH_CHUNKS 2u
 
-// Parameters of scene s (s wave-uniform) through a constant-address-space
-// pointer, so that their loads are scalar loads as kernel arguments' are.
-__device__ __forceinline__ const Params& scene_params(const Params* scenes, int s) {
-  const uint64_t a = (uint64_t)(scenes + s);
-  const uint64_t au = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32)) << 32) |
-                      (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
-  const __attribute__((address_space(4))) Params* Pc = (const __attribute__((address_space(4))) Params*)au;
-  return *(const Params*)Pc;
-}
 
 // Chunk c of a chunk list (c wave-uniform), read with scalar loads.
 __device__ __forceinline__ ChunkDesc chunk_at(const ChunkDesc* list, uint32_t c) {

@@ -2,7 +2,8 @@
 image (and per-pixel variance) must be bit-identical to a per-scene render
 with the same seed, whatever the tile size, in-flight tile count, LDS stack
 depth, traversal grid or launch mode (the per-sample sums are exact integers
-and a sample's random numbers depend only on its pixel and index)."""
+and a sample's random numbers depend only on its pixel and index) -- also
+in the persistent engine, where one launch follows every sample's whole path."""
 import numpy as np
 import pytest
 
@@ -42,6 +43,9 @@ def _run_batch(torch, prt, scenes, **kw):
     {"tile_samples": 256, "inflight_tiles": 7},
     {"tile_samples": 1024, "inflight_tiles": 3, "stack_lds": 8, "trav_blocks_per_cu": 4},
     {"tile_samples": 16384, "stack_lds": 32, "trav_blocks_per_cu": 16},
+    {"persistent": True},
+    {"persistent": True, "trav_blocks_per_cu": 3, "stack_lds": 8},
+    {"persistent": True, "trav_blocks_per_cu": 4},
 ])
 def test_batch_matches_per_scene_renders(batch_env, kw):
     torch, prt, scenes, ref = batch_env

@@ -6,7 +6,9 @@ prt_render_batch_device call.
 * Every scene's image and variance must equal its own prt_render_device
   render with the same seed bit for bit (a sample's random numbers depend
   only on its pixel and index, and the per-sample sums are exact integers),
-  and the batch must report exactly the sum of W*H*spp camera samples.
+  for the replayed tile-queue engine and for the persistent one-launch
+  engine, and the batch must report exactly the sum of W*H*spp camera
+  samples.
 * Four of the scenes (two per generator, the largest mesh of each and a
   small one) are rendered at reduced size on common random numbers against
   the FP64 oracle, with the criterion of the other parity tests.
@@ -48,6 +50,20 @@ def test_cfg4_full_batch_equals_per_scene_renders(cfg4_descs):
     assert st["truncated_paths"] == 0
     print("batch stats", {k: st[k] for k in ("camera_samples", "extension_rays", "shadow_rays", "ms_total")},
           flush=True)
+    # the persistent engine: one launch over every sample of the 64 scenes
+    po = [torch.full_like(o, -1.0) for o in outs]
+    pv = [torch.full_like(o, -1.0) for o in outs]
+    print("persistent render of the 64 scenes", flush=True)
+    sp = prt.render_batch_device(scenes, [o.data_ptr() for o in po], [v.data_ptr() for v in pv], seed=SEED,
+                                 persistent=True)
+    torch.cuda.synchronize()
+    assert sp["camera_samples"] == total and sp["launches_extend"] == 1
+    for i in range(len(scenes)):
+        assert torch.equal(po[i], outs[i]), f"persistent scene {i}: max |diff| {float((po[i] - outs[i]).abs().max())}"
+        assert torch.equal(pv[i], vars_[i]), f"persistent scene {i}: variance differs"
+    print("persistent stats", {k: sp[k] for k in ("camera_samples", "extension_rays", "shadow_rays", "ms_total")},
+          flush=True)
+    del po, pv
     # per-scene renders one scene at a time; each handle (and the render
     # workspace it allocated) is released after its comparison
     for i, sc in enumerate(scenes):
