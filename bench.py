@@ -403,7 +403,8 @@ def bench_batch(args, torch, G, prt, info):
         trials[wv] = prt.render_batch_device(scenes, ptrs, opts=po)["ms_total"]
     best_w = min(trials, key=trials.get)
     torch.cuda.synchronize()
-    pers_same = all(torch.equal(a, b) for a, b in zip(last, outs))
+    pers_diff = [i for i, (a, b) in enumerate(zip(last, outs)) if not torch.equal(a, b)]
+    pers_same = not pers_diff
     t0 = time.perf_counter()
     for i in range(args.steps):
         po = prt.batch_opts(seed=args.seed + i, persistent=True, trav_blocks_per_cu=best_w)
@@ -412,7 +413,7 @@ def bench_batch(args, torch, G, prt, info):
     pers_s = (time.perf_counter() - t0) / args.steps
     pers = {"samples_per_s": samples / pers_s, "ms_per_step": 1e3 * pers_s,
             "waves_per_simd": best_w, "trial_ms_by_waves": trials,
-            "images_identical_to_graph_form": pers_same,
+            "images_identical_to_graph_form": pers_same, "differing_scenes": pers_diff[:16],
             "over_graph_form": batched_s / pers_s, "over_per_scene": None,
             "kernel_ms": pst["ms_extend"], "resolve_ms": pst["ms_other"]}
     progress("per-scene renders of the same scenes")
@@ -427,6 +428,7 @@ def bench_batch(args, torch, G, prt, info):
     torch.cuda.synchronize()
     seq_s = (time.perf_counter() - t0) / args.steps
     max_diff = max(float((a - b).abs().max()) for a, b in zip(last, outs))
+    seq_diff = [i for i, (a, b) in enumerate(zip(last, outs)) if not torch.equal(a, b)]
     sum_ = lambda k: sum(s[k] for s in stats)
     rays = sum_("primary_rays") + sum_("extension_rays") + sum_("shadow_rays")
     ext_b, ext_ms, n_it = tst["bytes_extend"], tst["ms_extend"], tst["launches_extend"]
@@ -444,6 +446,7 @@ def bench_batch(args, torch, G, prt, info):
         "batched_over_per_scene": seq_s / batched_s,
         "persistent": pers,
         "images_identical": max_diff == 0.0, "max_abs_diff": max_diff,
+        "differing_scenes": [(i, bool(descs[i].sampling.next_event_estimation)) for i in seq_diff[:16]],
         "batch_options": opts.as_dict(), "tuner_ns_per_sample": tune_ns, "tune_s": tune_s,
         "iterations_per_step": stats[-1]["launches_extend"],
         "rays_per_step": {"primary": stats[-1]["primary_rays"], "extension": stats[-1]["extension_rays"],

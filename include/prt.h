@@ -191,11 +191,14 @@ typedef struct prt_render_opts {
                                   prt_batch_opts.use_graph selects graph replay for batches) */
     int32_t timing;            /* 1 = per-kernel device timing into prt_stats */
     int32_t image_f64;         /* 1 = out_rgb / out_var are double arrays (device entry points) */
-    int32_t engine;            /* 0 = auto: scenes with glass stream through the batched
-                                  renderer's tile queue, others run depth-synchronous
-                                  batches; 1 = depth-synchronous batches; 2 = streaming
-                                  (float images, no samples_override / batch_pixels).
-                                  The image is bit-identical either way. */
+    int32_t engine;            /* 0 = auto: scenes without next-event estimation render in
+                                  one persistent launch, scenes with glass stream through
+                                  the batched renderer's tile queue, others run
+                                  depth-synchronous batches; 1 = depth-synchronous
+                                  batches; 2 = streaming; 3 = persistent (2 and 3: float
+                                  images, no samples_override / batch_pixels; 3: at most
+                                  2^28 camera samples).  The image is bit-identical
+                                  whichever runs. */
 } prt_render_opts;
 
 /* ---- version / device ---------------------------------------------------- */
@@ -347,13 +350,18 @@ typedef struct prt_batch_opts {
     int32_t tile_stride;        /* render only every k-th tile (tuning trials; 0 = all) */
     int32_t max_depth;          /* safety cap on path vertices (0 = 128) */
     int32_t timing;             /* 1 = per-kernel device timing into prt_stats (plain launches only) */
-    int32_t persistent;         /* 1 = one persistent launch renders every camera sample of the batch:
+    int32_t persistent;         /* 0 = the replayed tile queue above.
+                                   1 = one persistent launch renders every camera sample of the batch:
                                    waves claim 64-sample chunks from one shared counter and follow each
                                    sample's whole path (shading, NEE, shadow rays, glass branches) with
                                    no ray queues and no waits between waves; then one launch folds the
-                                   pixels.  Same images bit for bit.  tile_samples, inflight_tiles,
-                                   use_graph and tile_stride do not apply; the batch may hold at most
-                                   2^28 camera samples (a fixed-point accumulator each). */
+                                   pixels.  trav_blocks_per_cu (2..4) sets its waves per SIMD;
+                                   tile_samples, inflight_tiles, use_graph and tile_stride do not
+                                   apply; at most 2^28 camera samples (a fixed-point accumulator each).
+                                   2 = split: the scenes without next-event estimation in one
+                                   persistent launch, the others through the tile queue (with these
+                                   options), one after the other.
+                                   The images are the same bit for bit in every mode. */
     int32_t pad_;
 } prt_batch_opts;
 /* d_rgb[i] / d_var[i]: W_i*H_i*3 float device buffers of scene i (d_var or
@@ -365,9 +373,11 @@ int prt_render_batch_device(prt_scene* const* scenes, int n_scenes,
 /* Measures candidate in-flight sizes (2^20..2^22 samples), tile sizes
  * (1024..16384 samples), LDS stack depths and traversal grids by coordinate
  * descent on a subsample of the batch's tiles (every k-th tile, k from
- * base->tile_stride or chosen for ~16M samples per trial), and writes the
- * fastest options to *best (tile_stride copied from base).  The trial renders
- * write the output buffers. */
+ * base->tile_stride or chosen for ~16M samples per trial), then the
+ * persistent engine at 2, 3 and 4 waves per SIMD on the whole batch (when it
+ * fits and base->tile_stride asks for no subsample), and writes the fastest
+ * options to *best (tile_stride copied from base).  The trial renders write
+ * the output buffers. */
 int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts* base,
                    float* const* d_rgb, float* const* d_var, prt_batch_opts* best,
                    double* best_ns_per_sample);

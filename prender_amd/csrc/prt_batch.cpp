@@ -90,7 +90,7 @@ struct BatchConfig {
   int tile_stride = 1;
   int max_vertices = 128;
   bool timing = false;
-  bool persistent = false;
+  int persistent = 0;  // 0 tile queue, 1 one persistent launch, 2 split by NEE
   int persist_blocks_per_cu = 2;
   uint64_t seed = 0;
 };
@@ -124,8 +124,12 @@ int resolve_config(prt_scene* const* scenes, int n, const prt_batch_opts* o, Bat
   c.tile_stride = std::max(1, opts.tile_stride);
   c.max_vertices = opts.max_depth > 0 ? std::min(opts.max_depth, 255) : 128;
   c.seed = opts.seed;
-  c.persistent = opts.persistent != 0;
-  c.persist_blocks_per_cu = opts.trav_blocks_per_cu > 0 ? opts.trav_blocks_per_cu : 2;
+  if (opts.persistent < 0 || opts.persistent > 2) return fail(PRT_E_INVALID, "persistent must be 0, 1 or 2");
+  c.persistent = opts.persistent;
+  // mode 1: trav_blocks_per_cu picks the persistent kernel's register budget;
+  // mode 2: it sizes the tile-queue part's grid, and the persistent part
+  // (scenes without NEE) keeps the compiler's own budget
+  c.persist_blocks_per_cu = (opts.persistent == 1 && opts.trav_blocks_per_cu > 0) ? opts.trav_blocks_per_cu : 2;
   if (c.persistent && opts.tile_stride > 1) return fail(PRT_E_INVALID, "the persistent engine renders every sample");
   return PRT_OK;
 }
@@ -670,6 +674,71 @@ int run_persistent(prt_scene* const* scenes, int n, const BatchConfig& c, float*
   return PRT_OK;
 }
 
+// Split mode: the scenes without next-event estimation in one persistent
+// launch (waves follow whole paths: no queue round trips), the others through
+// the replayed tile queue (whose traversal kernels keep their lanes busy
+// with shadow rays from many vertices); one after the other, each group on
+// its first scene's stream.  Statistics are summed.
+int run_split(prt_scene* const* scenes, int n, const BatchConfig& c, float* const* d_rgb, float* const* d_var,
+              prt_stats* stats, BatchResult* res) {
+  std::vector<prt_scene*> grp[2];
+  std::vector<float*> rgb[2], var[2];
+  for (int i = 0; i < n; ++i) {
+    const int g = (scenes[i]->P.nee && scenes[i]->P.n_lights > 0) ? 1 : 0;
+    grp[g].push_back(scenes[i]);
+    rgb[g].push_back(d_rgb[i]);
+    var[g].push_back(d_var ? d_var[i] : nullptr);
+  }
+  prt_stats part[2];
+  BatchResult r[2];
+  int rc;
+  BatchConfig cp = c;
+  cp.persistent = 1;
+  BatchConfig cs = c;
+  cs.persistent = 0;
+  cs.trav_blocks_per_cu = c.trav_blocks_per_cu;
+  if (!grp[0].empty() &&
+      (rc = run_persistent(grp[0].data(), (int)grp[0].size(), cp, rgb[0].data(), var[0].data(), &part[0], &r[0])))
+    return rc;
+  if (!grp[1].empty() &&
+      (rc = run_stream(grp[1].data(), (int)grp[1].size(), cs, rgb[1].data(), var[1].data(), &part[1], &r[1])))
+    return rc;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    for (int g = 0; g < 2; ++g) {
+      if (grp[g].empty()) continue;
+      const prt_stats& p = part[g];
+      stats->camera_samples += p.camera_samples;
+      stats->primary_rays += p.primary_rays;
+      stats->extension_rays += p.extension_rays;
+      stats->shadow_rays += p.shadow_rays;
+      stats->pickup_hits += p.pickup_hits;
+      stats->node_visits += p.node_visits;
+      stats->tri_tests += p.tri_tests;
+      stats->truncated_paths += p.truncated_paths;
+      stats->bytes_algorithmic += p.bytes_algorithmic;
+      stats->ms_total += p.ms_total;
+      stats->ms_extend += p.ms_extend;
+      stats->ms_shade += p.ms_shade;
+      stats->ms_shadow += p.ms_shadow;
+      stats->ms_nee += p.ms_nee;
+      stats->ms_other += p.ms_other;
+      stats->ms_traversal += p.ms_traversal;
+      stats->launches_extend += p.launches_extend;
+      stats->shadow_node_visits += p.shadow_node_visits;
+      stats->shadow_tri_tests += p.shadow_tri_tests;
+      stats->bytes_extend += p.bytes_extend;
+      stats->bytes_shadow += p.bytes_shadow;
+      stats->mixed_scene_passes += p.mixed_scene_passes;
+    }
+  }
+  if (res) {
+    res->ms = (grp[0].empty() ? 0.0 : r[0].ms) + (grp[1].empty() ? 0.0 : r[1].ms);
+    res->samples = (grp[0].empty() ? 0 : r[0].samples) + (grp[1].empty() ? 0 : r[1].samples);
+  }
+  return PRT_OK;
+}
+
 int check_batch_args(prt_scene* const* scenes, int n, float* const* d_rgb) {
   if (!scenes || n <= 0 || !d_rgb) return fail(PRT_E_INVALID, "bad arguments");
   for (int i = 0; i < n; ++i) {
@@ -689,7 +758,8 @@ int prt_render_batch_device(prt_scene* const* scenes, int n_scenes, const prt_ba
   if ((rc = check_batch_args(scenes, n_scenes, d_rgb))) return rc;
   BatchConfig c;
   if ((rc = resolve_config(scenes, n_scenes, opts, c))) return rc;
-  if (c.persistent) return run_persistent(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
+  if (c.persistent == 1) return run_persistent(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
+  if (c.persistent == 2) return run_split(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
   return run_stream(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
 }
 
@@ -767,6 +837,34 @@ int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts*
   }
   cur.inflight_tiles = (int32_t)std::max<int64_t>(1, inflight / cur.tile_samples);
   cur.tile_stride = base ? base->tile_stride : 0;
+  // the persistent engine (every sample, one launch) at each register
+  // budget, when the batch fits it and no subsample was asked for
+  if (total <= kMaxPersistSamples && (!base || base->tile_stride <= 1)) {
+    const prt_batch_opts queued = cur;
+    for (int wv = 2; wv <= 4; ++wv) {
+      prt_batch_opts o = queued;
+      o.persistent = 1;
+      o.trav_blocks_per_cu = wv;
+      BatchConfig c;
+      if ((rc = resolve_config(scenes, n_scenes, &o, c))) return rc;
+      BatchResult res;
+      if ((rc = run_persistent(scenes, n_scenes, c, d_rgb, d_var, nullptr, &res))) return rc;
+      const double ns = res.samples ? res.ms * 1e6 / (double)res.samples : 1e30;
+      if (ns < best_ns) { best_ns = ns; cur = o; }
+    }
+    // split: scenes without NEE persistent (the compiler's register budget,
+    // fastest for them), the others through the tuned tile queue
+    {
+      prt_batch_opts o = queued;
+      o.persistent = 2;
+      BatchConfig c;
+      if ((rc = resolve_config(scenes, n_scenes, &o, c))) return rc;
+      BatchResult res;
+      if ((rc = run_split(scenes, n_scenes, c, d_rgb, d_var, nullptr, &res))) return rc;
+      const double ns = res.samples ? res.ms * 1e6 / (double)res.samples : 1e30;
+      if (ns < best_ns) { best_ns = ns; cur = o; }
+    }
+  }
   *best = cur;
   if (best_ns_per_sample) *best_ns_per_sample = best_ns;
   return PRT_OK;

@@ -6,8 +6,8 @@ prt_render_batch_device call.
 * Every scene's image and variance must equal its own prt_render_device
   render with the same seed bit for bit (a sample's random numbers depend
   only on its pixel and index, and the per-sample sums are exact integers),
-  for the replayed tile-queue engine and for the persistent one-launch
-  engine, and the batch must report exactly the sum of W*H*spp camera
+  for the replayed tile-queue engine, for the persistent one-launch
+  engine and for the split form (each engine on its half), and the batch must report exactly the sum of W*H*spp camera
   samples.
 * Four of the scenes (two per generator, the largest mesh of each and a
   small one) are rendered at reduced size on common random numbers against
@@ -63,6 +63,17 @@ def test_cfg4_full_batch_equals_per_scene_renders(cfg4_descs):
         assert torch.equal(pv[i], vars_[i]), f"persistent scene {i}: variance differs"
     print("persistent stats", {k: sp[k] for k in ("camera_samples", "extension_rays", "shadow_rays", "ms_total")},
           flush=True)
+    # split form: the no-NEE scenes through the persistent engine, the NEE
+    # scenes through the tile queue
+    print("split render of the 64 scenes", flush=True)
+    po = [torch.full_like(o, -1.0) for o in outs]
+    pv = [torch.full_like(o, -1.0) for o in outs]
+    ss = prt.render_batch_device(scenes, [o.data_ptr() for o in po], [v.data_ptr() for v in pv], seed=SEED,
+                                 persistent=2)
+    torch.cuda.synchronize()
+    assert ss["camera_samples"] == total
+    bad = [i for i in range(len(scenes)) if not (torch.equal(po[i], outs[i]) and torch.equal(pv[i], vars_[i]))]
+    assert not bad, f"split form differs on scenes {bad}"
     del po, pv
     # per-scene renders one scene at a time; each handle (and the render
     # workspace it allocated) is released after its comparison

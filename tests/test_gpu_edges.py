@@ -111,22 +111,27 @@ def test_bad_descriptions_fail_loudly(prt):
 
 @pytest.mark.parametrize("name", ["cfg1", "cfg2", "cfg3"])
 def test_engines_give_identical_images(prt, name):
-    # depth-synchronous batches (engine 1) and the streaming tile queue
-    # (engine 2; the default for scenes with glass) are two schedules of the
-    # same per-sample computation
+    # depth-synchronous batches (engine 1), the streaming tile queue (engine
+    # 2; the default for scenes with glass and NEE) and one persistent launch
+    # (engine 3; the default for scenes without NEE) are three schedules of
+    # the same per-sample computation
     d = {"cfg1": lambda: G.config1(width=64, height=48, samples=4),
          "cfg2": lambda: G.config2(width=48, height=48, samples=4, level=4),
          "cfg3": lambda: G.config3(width=64, height=48, samples=2)}[name]()
     s = prt.Scene(d)
-    print(f"{name} engines 1 / 2 / auto", flush=True)
+    print(f"{name} engines 1 / 2 / 3 / auto", flush=True)
     a, va, sa = s.render(seed=13, engine=1)
     b, vb, sb = s.render(seed=13, engine=2)
+    p, vp, sp = s.render(seed=13, engine=3)
     c, vc, _ = s.render(seed=13)
-    assert np.array_equal(a, b) and np.array_equal(a, c)
-    assert np.array_equal(va, vb) and np.array_equal(va, vc)
-    assert sa["camera_samples"] == sb["camera_samples"] and sa["shadow_rays"] == sb["shadow_rays"]
-    with pytest.raises(prt.PrtError):
-        s.render(seed=13, engine=2, samples=1)
+    assert np.array_equal(a, b) and np.array_equal(a, c) and np.array_equal(a, p)
+    assert np.array_equal(va, vb) and np.array_equal(va, vc) and np.array_equal(va, vp)
+    assert sa["camera_samples"] == sb["camera_samples"] == sp["camera_samples"]
+    assert sa["shadow_rays"] == sb["shadow_rays"] == sp["shadow_rays"]
+    assert sp["launches_extend"] == 1
+    for engine in (2, 3):
+        with pytest.raises(prt.PrtError):
+            s.render(seed=13, engine=engine, samples=1)
 
 
 @pytest.mark.parametrize("make", [lambda: G.config2(width=64, height=48, samples=2),
