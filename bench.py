@@ -112,16 +112,22 @@ def main():
     rays = sum_("primary_rays") + sum_("extension_rays") + sum_("shadow_rays")
     ms_ext, ms_sh, ms_shade = sum_("ms_extend"), sum_("ms_shadow"), sum_("ms_shade")
     launches = sum_("launches_extend")
-    # scenes with glass render through the streaming tile queue (engine 0
-    # routes them there), whose traversal kernels are k_stream_extend / _shadow
-    streamed = any(m.reflection == "reflection" for m in desc.materials)
-    kx, ks = ("k_stream_extend", "k_stream_shadow") if streamed else ("k_extend", "k_shadow")
-    kernels = {
-        kx: (sum_("bytes_extend"), ms_ext, launches),
-        ks: (sum_("bytes_shadow"), ms_sh, launches),
-    }
-    # the traversal kernel moving the most algorithmic bytes (k_shadow overlaps
-    # k_extend on a second stream, so their launch times share the chip)
+    # engine 0's choice: scenes without light sampling render in one
+    # persistent launch (k_persist traces every ray of the frame), scenes
+    # with glass through the streaming tile queue (k_stream_extend /
+    # _shadow), the others depth by depth (k_extend / k_shadow)
+    persistent = not (desc.sampling.next_event_estimation and sinfo["lights"] > 0)
+    streamed = not persistent and any(m.reflection == "reflection" for m in desc.materials)
+    if persistent:
+        kx = ks = "k_persist"
+        kernels = {kx: (sum_("bytes_extend") + sum_("bytes_shadow"), ms_ext, launches)}
+    else:
+        kx, ks = ("k_stream_extend", "k_stream_shadow") if streamed else ("k_extend", "k_shadow")
+        kernels = {
+            kx: (sum_("bytes_extend"), ms_ext, launches),
+            ks: (sum_("bytes_shadow"), ms_sh, launches),
+        }
+    # the traversal kernel moving the most algorithmic bytes
     dom = max(kernels, key=lambda k: kernels[k][0])
     b, ms, nl = kernels[dom]
     achieved = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
@@ -159,7 +165,8 @@ def main():
                      "note": "algorithmic bytes = 128 B per BVH node + 48 B per triangle + 48 B per ray; "
                              "node and triangle reuse in L2 / MALL lets this exceed the HBM figure"},
         "traversal_union": {
-            "kernels": kx + " + " + ks, "engine": "streaming tile queue" if streamed else "depth-synchronous",
+            "kernels": kx if persistent else kx + " + " + ks,
+            "engine": "persistent" if persistent else ("streaming tile queue" if streamed else "depth-synchronous"),
             "ms_per_step": sum_("ms_traversal") / args.steps,
             "achieved": (sum_("bytes_extend") + sum_("bytes_shadow")) / max(1e-12, sum_("ms_traversal") * 1e-3) / 1e9,
             "unit": "GB/s", "peak": HBM_PEAK_GBS,
@@ -171,9 +178,16 @@ def main():
     }
     tu = out["traversal_union"]
     tu["frac"] = tu["achieved"] / HBM_PEAK_GBS
-    if streamed:
-        # the streaming engine traces camera rays in the same launches as
-        # bounce rays; their rate is measured on their own
+    if persistent:
+        out["roofline"]["note"] += ("; k_persist also shades and samples lights, so its launch time is the "
+                                    "whole frame's and its bound is latency rather than HBM")
+        out["traversal_union"]["note"] = "one launch traces, shades and resolves every path of the frame"
+        out["mrays_per_s_by_class"].update(
+            secondary=None, shadow=None,
+            note="one launch traces every class of ray; the camera-ray rate is measured on its own below")
+    if streamed or persistent:
+        # the streaming and persistent engines trace camera rays in the same
+        # launches as bounce rays; their rate is measured on their own
         out["mrays_per_s_by_class"]["primary"] = primary_trace_rate(torch, G, desc, scene)
         out["mrays_per_s_by_class"]["primary_note"] = (
             "the frame's W*H*SS^2 camera rays traced on their own through prt_trace_closest_device "
@@ -393,6 +407,36 @@ def bench_batch(args, torch, G, prt, info):
     topts = prt.PrtBatchOpts.from_buffer_copy(opts)
     topts.use_graph, topts.timing, topts.seed = -1, 1, args.seed + args.steps - 1
     tst = prt.render_batch_device(scenes, ptrs, opts=topts)
+    # the traversal kernels' algorithmic bytes and device time: (bytes, ms,
+    # launches); the split form's halves are timed on their own so that the
+    # persistent kernel and the tile queue's closest-hit kernel each report
+    # their own figure
+    roof = {}
+    if opts.persistent == 2:
+        nee = [i for i, s in enumerate(scenes) if s.desc.sampling.next_event_estimation and s.info()["lights"] > 0]
+        rest = [i for i in range(len(scenes)) if i not in nee]
+        if rest:
+            o1 = prt.PrtBatchOpts.from_buffer_copy(topts)
+            o1.persistent, o1.persist_waves = 1, opts.persist_waves or 2
+            s1 = prt.render_batch_device([scenes[i] for i in rest], [ptrs[i] for i in rest], opts=o1)
+            roof["k_persist"] = (s1["bytes_extend"] + s1["bytes_shadow"], s1["ms_extend"], 1)
+        if nee:
+            o0 = prt.PrtBatchOpts.from_buffer_copy(topts)
+            o0.persistent = 0
+            s0 = prt.render_batch_device([scenes[i] for i in nee], [ptrs[i] for i in nee], opts=o0)
+            roof["k_stream_extend"] = (s0["bytes_extend"], s0["ms_extend"], s0["launches_extend"])
+    elif opts.persistent == 1:
+        roof["k_persist"] = (tst["bytes_extend"] + tst["bytes_shadow"], tst["ms_extend"], 1)
+    else:
+        roof["k_stream_extend"] = (tst["bytes_extend"], tst["ms_extend"], tst["launches_extend"])
+
+    def roofline_of(k):
+        b, ms, nl = roof[k]
+        a = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        return {"bound": "hbm", "kernel": k, "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": a / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": b / max(1, nl),
+                "avg_launch_ms": ms / max(1, nl)}
+    dom = max(roof, key=lambda k: roof[k][1])  # the kernel with the most device time
     # the persistent engine: one launch over every sample of the batch; one
     # trial per register budget (2, 3 or 4 waves per SIMD), the fastest timed
     progress("persistent engine")
@@ -431,8 +475,7 @@ def bench_batch(args, torch, G, prt, info):
     seq_diff = [i for i, (a, b) in enumerate(zip(last, outs)) if not torch.equal(a, b)]
     sum_ = lambda k: sum(s[k] for s in stats)
     rays = sum_("primary_rays") + sum_("extension_rays") + sum_("shadow_rays")
-    ext_b, ext_ms, n_it = tst["bytes_extend"], tst["ms_extend"], tst["launches_extend"]
-    achieved = ext_b / (ext_ms * 1e-3) / 1e9 if ext_ms > 0 else 0.0
+    ext_ms = tst["ms_extend"]
     pers["over_per_scene"] = seq_s / pers_s
     out = {
         "metric": METRIC, "value": samples / batched_s, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
@@ -455,9 +498,8 @@ def bench_batch(args, torch, G, prt, info):
         "kernel_ms_per_step_plain_launches": {"extend": ext_ms, "shade": tst["ms_shade"], "nee": tst["ms_nee"],
                                               "shadow": tst["ms_shadow"], "other": tst["ms_other"],
                                               "total": tst["ms_total"]},
-        "roofline": {"bound": "hbm", "kernel": "k_stream_extend", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": ext_b / max(1, n_it), "avg_launch_ms": ext_ms / max(1, n_it)},
+        "roofline": roofline_of(dom),
+        "roofline_other_kernels": [roofline_of(k) for k in roof if k != dom],
         "device": info["arch"], "scene_build_s": build_s,
     }
     if not args.no_cpu:

@@ -362,7 +362,8 @@ typedef struct prt_batch_opts {
                                    persistent launch, the others through the tile queue (with these
                                    options), one after the other.
                                    The images are the same bit for bit in every mode. */
-    int32_t pad_;
+    int32_t persist_waves;      /* waves per SIMD the persistent kernel's registers are budgeted for
+                                   (2..4) in modes 1 and 2; 0 = mode 1: trav_blocks_per_cu, mode 2: 2 */
 } prt_batch_opts;
 /* d_rgb[i] / d_var[i]: W_i*H_i*3 float device buffers of scene i (d_var or
  * its entries may be NULL).  All scenes must live on one device; the call
@@ -374,8 +375,9 @@ int prt_render_batch_device(prt_scene* const* scenes, int n_scenes,
  * (1024..16384 samples), LDS stack depths and traversal grids by coordinate
  * descent on a subsample of the batch's tiles (every k-th tile, k from
  * base->tile_stride or chosen for ~16M samples per trial), then the
- * persistent engine at 2, 3 and 4 waves per SIMD on the whole batch (when it
- * fits and base->tile_stride asks for no subsample), and writes the fastest
+ * persistent engine and the split form, each at 2, 3 and 4 waves per SIMD, on
+ * the whole batch (when it fits and base->tile_stride asks for no
+ * subsample), and writes the fastest
  * options to *best (tile_stride copied from base).  The trial renders write
  * the output buffers. */
 int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts* base,

@@ -35,8 +35,24 @@
 #ifndef PRT_BATCH_LOG2
 #define PRT_BATCH_LOG2 24
 #endif
+// Per-scene renders of NEE scenes: where a depth's shadow rays are traced.
+// 0: on the main stream after the depth's NEE; 1: on a second stream, while
+// the next depth's extension rays are traced; 2: NEE and shadow rays both on
+// the second stream, after the shade that wrote the NEE input; 3: the NEE on
+// the main stream into one of two shadow queues, and its shadow rays on the
+// second stream once the next depth's extension rays are done, so that they
+// share the chip with that depth's shade and NEE (queue-streaming kernels)
+// rather than with another traversal.
+#ifndef PRT_OVERLAP_SHADOW
+#define PRT_OVERLAP_SHADOW 3
+#endif
+
+// Shadow-ray blocks per CU.  The traversal kernels' LDS stacks allow 4
+// resident blocks per CU; 3 leave room for the shade and NEE blocks that run
+// beside the shadow rays in the alternating schedule (cfg2 348 -> 353 M
+// samples/s; 2 let the shadow rays run into the next closest-hit launch).
 #ifndef PRT_SHADOW_BLOCKS_PER_CU
-#define PRT_SHADOW_BLOCKS_PER_CU PRT_TRAV_BLOCKS_PER_CU
+#define PRT_SHADOW_BLOCKS_PER_CU 3
 #endif
 // the traversal-stack overflow areas hold one slot per thread of the
 // closest-hit grid; a larger shadow grid would share slots between threads
@@ -66,6 +82,7 @@ void Workspace::release() {
   hit_t.release();
   hit_id.release();
   for (auto& b : sq) b.release();
+  for (auto& b : sq2) b.release();
   for (auto& b : desc) b.release();
   slabs.release();
   accum.release();
@@ -183,6 +200,8 @@ void prt_scene_destroy(prt_scene* s) {
   s->timer_events.release();
   for (hipEvent_t e : s->ev_nee)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : s->ev_sh)
+    if (e) (void)hipEventDestroy(e);
   if (s->ev_shadow) (void)hipEventDestroy(s->ev_shadow);
   if (s->ev_shade) (void)hipEventDestroy(s->ev_shade);
   if (s->stream2) (void)hipStreamDestroy(s->stream2);
@@ -217,6 +236,8 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
   HIPCHK(hipEventCreateWithFlags(&s->ev_nee[1], hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&s->ev_shadow, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&s->ev_shade, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_sh[0], hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_sh[1], hipEventDisableTiming));
 
   // materials (Material::isLight, material.h:195-210)
   s->h_mats.resize(d->n_materials);
@@ -572,8 +593,10 @@ extern "C++" int prt::ensure_workspace(prt_scene* s, int64_t batch_samples, bool
   const int64_t qcap = grow * batch_samples + 2 * 64 * W + 64;
   const int64_t scap = 3 * (grow * batch_samples + 64 * W);
   if (qcap >= ((int64_t)1 << 32) || scap >= ((int64_t)1 << 32)) return fail(PRT_E_INVALID, "batch too large");
+  // the second shadow queue of the alternating schedule (NEE scenes only)
+  const bool two_sq = PRT_OVERLAP_SHADOW == 3 && s->P.nee && s->P.n_lights > 0;
   if (w.batch_samples >= batch_samples && w.queue_cap >= qcap && w.shadow_cap >= scap && w.h_counts &&
-      (!two_accum || w.accum2.p))
+      (!two_accum || w.accum2.p) && (!two_sq || w.sq2[0].p))
     return PRT_OK;
   w.release();
   int rc;
@@ -582,8 +605,9 @@ extern "C++" int prt::ensure_workspace(prt_scene* s, int64_t batch_samples, bool
   if ((rc = dev_alloc(w.hit_t, (size_t)qcap * 8))) return rc;
   if ((rc = dev_alloc(w.hit_id, (size_t)qcap * 4))) return rc;
   if ((rc = alloc_shadow_queue(w.sq, scap))) return rc;
+  if (two_sq && (rc = alloc_shadow_queue(w.sq2, scap))) return rc;
   // chunks: a slab's last chunk may be partly filled
-  for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < (two_sq ? 4 : 3); ++k)
     if ((rc = dev_alloc(w.desc[k], (size_t)((k < 2 ? qcap : scap) / 64 + W + 1) * sizeof(uint2)))) return rc;
   if ((rc = dev_alloc(w.slabs, (size_t)3 * W * sizeof(uint2)))) return rc;
   if ((rc = dev_alloc(w.accum, (size_t)batch_samples * 3 * sizeof(uint64_t)))) return rc;
@@ -637,12 +661,6 @@ static int setup_render(prt_scene* s, const prt_render_opts* o, RenderSetup& R) 
 #ifndef PRT_DEPTHS_PER_SYNC
 #define PRT_DEPTHS_PER_SYNC 4
 #endif
-// 1: the shadow rays of a depth are traced on a second stream while the next
-// depth's extension rays are; 2: the NEE of a depth runs there as well
-// (forward renders only)
-#ifndef PRT_OVERLAP_SHADOW
-#define PRT_OVERLAP_SHADOW 2
-#endif
 
 // One batch through the wavefront: generate, then extend / shade / NEE /
 // shadow per path depth until no ray is left.  Sample radiance accumulates
@@ -651,9 +669,9 @@ static int setup_render(prt_scene* s, const prt_render_opts* o, RenderSetup& R) 
 // of the queue it wrote for the kernels that read it.  The host reads the
 // ray count once per PRT_DEPTHS_PER_SYNC depths.  Without the adjoint (which
 // reads the partial sums of a sample in path order) a depth's shadow rays are
-// traced on the scene's second stream, overlapping the next depth's
-// extension rays; the next NEE, which rewrites the shadow queue, waits for
-// them, and so does the batch's end.
+// traced on the scene's second stream as PRT_OVERLAP_SHADOW selects; a NEE
+// that rewrites a shadow queue waits for the shadow launch that read it, and
+// the batch's end waits for every shadow launch.
 static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t* accum, const Adjoint& A, Timer& tm,
                         CallTotals& tot) {
   Workspace& w = s->ws;
@@ -667,8 +685,9 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
   RayQ q[2] = {ray_queue(w.q[0], w.queue_cap), ray_queue(w.q[1], w.queue_cap)};
   ShadowQ sq = shadow_queue(w.sq, w.shadow_cap);
   const int tb = s->max_trav_blocks;
-  // the shadow grid (on the second stream) may leave room on each CU
-  const int tb_sh = std::max(1, s->cu_count) * PRT_SHADOW_BLOCKS_PER_CU;
+  // the shadow grid on the second stream leaves room on each CU; on the main
+  // stream (no overlap) it is the closest-hit grid
+  const int tb_sh = overlap ? std::max(1, s->cu_count) * PRT_SHADOW_BLOCKS_PER_CU : tb;
   const bool nee = P.nee && P.n_lights > 0;
   const uint32_t W = PRT_SLAB_WAVES;
   const size_t fetch_launch = (size_t)PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE;
@@ -727,6 +746,46 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
     }
     return PRT_OK;
   };
+  // Alternating schedule (overlap 3): the NEE of q[k] on the main stream
+  // into shadow queue sh_next, whose shadow rays are traced on the second
+  // stream after the next depth's closest-hit launch (shadow_alt).
+  const bool alt = overlap == 3 && nee;
+  const ShadowQ sqs[2] = {sq, alt ? shadow_queue(w.sq2, w.shadow_cap) : sq};
+  int sh_next = 0, sh_pending = -1;
+  bool sh_used[2] = {false, false};
+  auto nee_alt = [&](int k) -> int {
+    const int b = sh_next;
+    sh_next ^= 1;
+    if (sh_used[b]) HIPCHK(hipStreamWaitEvent(st, s->ev_sh[b], 0));  // its last shadow launch has read it
+    HIPCHK(launch_nee(P, B, q[k], chunks(k), sqs[b], slabs + 2 * W, ctr, st));
+    HIPCHK(index(2 + b, W, slabs + 2 * W, st));
+    if ((rc = tm.mark(T_NEE))) return rc;
+    sh_pending = b;
+    return PRT_OK;
+  };
+  // the pending shadow queue's rays, after everything enqueued so far on the
+  // main stream (on the main stream itself when `side` is false)
+  auto shadow_alt = [&](bool side) -> int {
+    if (sh_pending < 0) return PRT_OK;
+    const int b = sh_pending;
+    sh_pending = -1;
+    if ((rc = next_fetch(&f))) return rc;  // a fill clearing its counters goes on st first
+    hipStream_t on = side ? st_sh : st;
+    if (side) {
+      HIPCHK(hipEventRecord(s->ev_shade, st));
+      HIPCHK(hipStreamWaitEvent(on, s->ev_shade, 0));
+      if ((rc = tm.begin_side(on))) return rc;
+    }
+    HIPCHK(launch_shadow(P, B, sqs[b], chunks(2 + b), f, accum, A, ctr, side ? tb_sh : tb, on));
+    if (side) {
+      if ((rc = tm.end_side(on, T_SHADOW))) return rc;
+      HIPCHK(hipEventRecord(s->ev_sh[b], on));
+      sh_used[b] = true;
+    } else if ((rc = tm.mark(T_SHADOW))) {
+      return rc;
+    }
+    return PRT_OK;
+  };
   const int64_t ns = B.n_pixels * P.spp;
   const int64_t np = B.n_pixels * P.SS * P.SS;
   HIPCHK(launch_fill_u64(accum, 0ull, 3 * ns, st));
@@ -743,12 +802,16 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
   if ((rc = tm.mark(T_EXTEND0))) return rc;
   tot.primary_rays += (uint64_t)np;
   tot.launches++;
-  // depth 0: every camera sample against its sub-pixel's primary hit
-  HIPCHK(launch_shade(P, B, 1, q[0], chunks(0), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[1], fan(0),
-                      slabs + W, accum, A, ctr, st));
-  HIPCHK(index(1, W, slabs + W, st));
-  if ((rc = tm.mark(T_SHADE))) return rc;
-  if (nee && (rc = nee_and_shadow(1))) return rc;
+  // Shade q[kin] (primary: every camera sample against its sub-pixel's
+  // primary hit) into q[kout], then the NEE of the children.
+  auto shade_nee = [&](int primary, int kin, int kout, uint32_t fn) -> int {
+    HIPCHK(launch_shade(P, B, primary, q[kin], chunks(kin), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p,
+                        q[kout], fn, slabs + kout * W, accum, A, ctr, st));
+    HIPCHK(index(kout, W, slabs + kout * W, st));
+    if ((rc = tm.mark(T_SHADE))) return rc;
+    return nee ? (alt ? nee_alt(kout) : nee_and_shadow(kout)) : PRT_OK;
+  };
+  if ((rc = shade_nee(1, 0, 1, fan(0)))) return rc;
   int cur = 1;
   for (int depth = 1;; ++depth) {  // depth of the rays in q[cur]
     if ((depth - 1) % PRT_DEPTHS_PER_SYNC == 0) {
@@ -763,16 +826,20 @@ static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t*
                          tb, st));
     if ((rc = tm.mark(T_EXTEND))) return rc;
     tot.launches++;
+    // the shadow rays of q[cur]'s NEE, alongside this depth's shade and NEE
+    if (alt && (rc = shadow_alt(true))) return rc;
     // the NEE two depths back read q[nxt] and its chunk list
     if (overlap == 2 && nee) HIPCHK(hipStreamWaitEvent(st, s->ev_nee[nxt], 0));
-    HIPCHK(launch_shade(P, B, 0, q[cur], chunks(cur), (const double*)w.hit_t.p, (const int32_t*)w.hit_id.p, q[nxt],
-                        fan(depth), slabs + nxt * W, accum, A, ctr, st));
-    HIPCHK(index(nxt, W, slabs + nxt * W, st));
-    if ((rc = tm.mark(T_SHADE))) return rc;
-    if (nee && (rc = nee_and_shadow(nxt))) return rc;
+    if ((rc = shade_nee(0, cur, nxt, fan(depth)))) return rc;
     cur = nxt;
   }
-  if (overlap) HIPCHK(hipStreamWaitEvent(st, s->ev_shadow, 0));
+  if (alt) {
+    if ((rc = shadow_alt(false))) return rc;
+    for (int b = 0; b < 2; ++b)
+      if (sh_used[b]) HIPCHK(hipStreamWaitEvent(st, s->ev_sh[b], 0));
+  } else if (overlap) {
+    HIPCHK(hipStreamWaitEvent(st, s->ev_shadow, 0));
+  }
   return PRT_OK;
 }
 

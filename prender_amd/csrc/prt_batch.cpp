@@ -126,10 +126,14 @@ int resolve_config(prt_scene* const* scenes, int n, const prt_batch_opts* o, Bat
   c.seed = opts.seed;
   if (opts.persistent < 0 || opts.persistent > 2) return fail(PRT_E_INVALID, "persistent must be 0, 1 or 2");
   c.persistent = opts.persistent;
-  // mode 1: trav_blocks_per_cu picks the persistent kernel's register budget;
-  // mode 2: it sizes the tile-queue part's grid, and the persistent part
-  // (scenes without NEE) keeps the compiler's own budget
-  c.persist_blocks_per_cu = (opts.persistent == 1 && opts.trav_blocks_per_cu > 0) ? opts.trav_blocks_per_cu : 2;
+  // the persistent kernel's register budget (waves per SIMD): persist_waves,
+  // else in mode 1 trav_blocks_per_cu (mode 2 sizes the tile-queue part's
+  // grid with it), else the compiler's own budget (2)
+  if (opts.persist_waves != 0 && (opts.persist_waves < 2 || opts.persist_waves > 4))
+    return fail(PRT_E_INVALID, "persist_waves must be 0, 2, 3 or 4");
+  c.persist_blocks_per_cu = opts.persist_waves > 0                               ? opts.persist_waves
+                            : (opts.persistent == 1 && opts.trav_blocks_per_cu > 0) ? opts.trav_blocks_per_cu
+                                                                                   : 2;
   if (c.persistent && opts.tile_stride > 1) return fail(PRT_E_INVALID, "the persistent engine renders every sample");
   return PRT_OK;
 }
@@ -852,11 +856,12 @@ int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts*
       const double ns = res.samples ? res.ms * 1e6 / (double)res.samples : 1e30;
       if (ns < best_ns) { best_ns = ns; cur = o; }
     }
-    // split: scenes without NEE persistent (the compiler's register budget,
-    // fastest for them), the others through the tuned tile queue
-    {
+    // split: scenes without NEE persistent (at each register budget), the
+    // others through the tuned tile queue
+    for (int wv = 2; wv <= 4; ++wv) {
       prt_batch_opts o = queued;
       o.persistent = 2;
+      o.persist_waves = wv;
       BatchConfig c;
       if ((rc = resolve_config(scenes, n_scenes, &o, c))) return rc;
       BatchResult res;

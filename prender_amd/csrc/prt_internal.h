@@ -79,10 +79,12 @@ struct Workspace {
   DevBuf q[2][RAYQ_COLS];     // two ray queues (SoA)
   DevBuf hit_t, hit_id;
   DevBuf sq[SHADOWQ_COLS];
-  DevBuf desc[3];             // chunk descriptors (uint2) of q[0], q[1], the shadow queue
-  DevBuf slabs;               // uint2 [3][PRT_SLAB_WAVES] slab (start, count) of the same three
+  DevBuf sq2[SHADOWQ_COLS];   // second shadow queue (NEE scenes, when a depth's shadow rays are traced
+                              // while the next depth's NEE fills the other queue)
+  DevBuf desc[4];             // chunk descriptors (uint2) of q[0], q[1], the shadow queue, the second one
+  DevBuf slabs;               // uint2 [3][PRT_SLAB_WAVES] slab (start, count) of q[0], q[1], the NEE output
   DevBuf accum, accum2;       // fixed-point sample radiance (accum2: adjoint primal)
-  DevBuf counts;              // uint32 [16]: chunks of q0, q1, sq at 0..2, entries at 4..6
+  DevBuf counts;              // uint32 [16]: chunks of q0, q1, sq, sq2 at 0..3, entries at 4..7
   DevBuf fetch;               // uint32 claim counters, PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE per launch
   DevBuf ctr;                 // uint64 [CTR_COUNT]
   uint32_t* h_counts = nullptr;
@@ -245,6 +247,7 @@ struct prt_scene {
   hipStream_t stream2 = nullptr;              // shadow traversal, overlapping the next depth
   hipEvent_t ev_nee[2] = {nullptr, nullptr};  // last NEE that read ray queue 0 / 1
   hipEvent_t ev_shadow = nullptr, ev_shade = nullptr;
+  hipEvent_t ev_sh[2] = {nullptr, nullptr};    // last shadow launch that read shadow queue 0 / 1
   Params P{};  // scene + sampling part; per-call fields are patched in
   prt::DevBuf nodes, tris, tri_nrm, tri_alb, ents, mats, analytic, lights, ovf;
   std::vector<DevEntity> h_ents;

@@ -111,6 +111,10 @@ struct Batch {
 #define PRT_SLAB_WAVES 4096  // a multiple of 4, at most PRT_SLAB_INDEX_MAX
 #endif
 #define PRT_SLAB_INDEX_MAX 8192
+// slabs (or slab writers) listed by one block of the chunk-list kernels; one
+// wave scans them, so at most 64
+#define PRT_INDEX_SLABS 64
+static_assert(PRT_INDEX_SLABS <= 64, "one wave scans a block's slabs");
 #define PRT_FETCH_COUNTERS 64
 #define PRT_FETCH_STRIDE 32
 struct ChunkList {

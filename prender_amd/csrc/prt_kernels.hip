@@ -31,11 +31,6 @@
 #define PRT_WAVES_NEE 0
 #endif
 #define PRT_WAVES_ATTR(n) __attribute__((amdgpu_waves_per_eu((n) > 0 ? (n) : 1)))
-// threads of the one-block chunk-list kernel (256 fits beside the other
-// stream's traversal blocks sooner than 1024)
-#ifndef PRT_INDEX_THREADS
-#define PRT_INDEX_THREADS 1024
-#endif
 
 // ------------------------------------------------------------ trace API kernel
 __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_TRAVERSE) void k_trace(
@@ -392,78 +387,69 @@ __global__ __launch_bounds__(256) PRT_WAVES_ATTR(PRT_WAVES_NEE) void k_nee(
 }
 
 // ------------------------------------------------------------ slab index
-// Chunk descriptors of a slab queue (one block of 1024 threads): slab s holds
-// slabs[s].y entries from queue index slabs[s].x on, ceil(that / 64) chunks;
-// chunks are numbered slab by slab.  Also stores the number of chunks and of
-// entries (and adds the entries to ctr[ctr_entries] unless that is < 0).
-// More chunks than max_chunks (never, with the host's sizing) are
-// dropped and counted.
-template <int NT>
-__global__ __launch_bounds__(NT) void k_slab_index(const uint2* __restrict__ slabs, uint32_t n_slabs,
-                                                   uint2* __restrict__ desc, uint32_t max_chunks,
-                                                   uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr,
-                                                   int ctr_entries) {
-  constexpr uint32_t NW = NT / 64;
-  __shared__ uint32_t s_pref[PRT_SLAB_INDEX_MAX + 1];
-  __shared__ uint32_t s_wch[NW], s_went[NW];
+// Chunk descriptors of a slab queue: slab s holds slabs[s].y entries from
+// queue index slabs[s].x on, ceil(that / 64) chunks, numbered slab by slab.
+// Block g lists the chunks of the PRT_INDEX_SLABS slabs from
+// g * PRT_INDEX_SLABS on: it sums the chunks of all slabs before them (every
+// block reads those slab counts, 8 bytes each, mostly from L2), scans its own
+// slabs' chunk counts in one wave, and writes their descriptors, a wave per
+// slab and a lane per chunk.  Block 0, whose range starts at slab 0, sums
+// every slab instead and stores the number of chunks and of entries (and adds
+// the entries to ctr[ctr_entries] unless that is < 0).  Chunks beyond
+// max_chunks (never, with the host's sizing) are dropped and counted.
+__global__ __launch_bounds__(256) void k_slab_index(const uint2* __restrict__ slabs, uint32_t n_slabs,
+                                                    uint2* __restrict__ desc, uint32_t max_chunks,
+                                                    uint32_t* n_chunks, uint32_t* n_entries, uint64_t* ctr,
+                                                    int ctr_entries) {
+  __shared__ uint32_t s_part[2][4];
+  __shared__ uint32_t s_first[PRT_INDEX_SLABS];
   const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
-  const uint32_t per = (n_slabs + NT - 1u) / NT;
-  const uint32_t b = min(tid * per, n_slabs), e = min(b + per, n_slabs);
+  const uint32_t s0 = blockIdx.x * (uint32_t)PRT_INDEX_SLABS;  // < n_slabs by the grid
+  const uint32_t ns = min(n_slabs - s0, (uint32_t)PRT_INDEX_SLABS);
+  const bool totals = blockIdx.x == 0;
+  const uint32_t lim = totals ? n_slabs : s0;
   uint32_t ch = 0, ent = 0;
-  for (uint32_t s = b; s < e; ++s) {
+  for (uint32_t s = tid; s < lim; s += 256u) {
     const uint32_t c = slabs[s].y;
     ch += (c + 63u) >> 6;
     ent += c;
   }
-  // block exclusive scan of (ch, ent): within waves, then over the NW waves
-  uint32_t ich = ch, ient = ent;
+  ch = (uint32_t)wave_sum_u32(ch);
+  ent = (uint32_t)wave_sum_u32(ent);
+  if (lane == 0) {
+    s_part[0][wv] = ch;
+    s_part[1][wv] = ent;
+  }
+  if (wv == 0) {
+    // exclusive scan of this block's slabs' chunk counts
+    const uint32_t c = lane < ns ? slabs[s0 + lane].y : 0u;
+    const uint32_t n = (c + 63u) >> 6;
+    uint32_t incl = n;
 #pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t a1 = (uint32_t)__shfl_up((int)ich, o, 64), a2 = (uint32_t)__shfl_up((int)ient, o, 64);
-    if (lane >= (uint32_t)o) {
-      ich += a1;
-      ient += a2;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+      if (lane >= (uint32_t)o) incl += y;
     }
-  }
-  if (lane == 63) {
-    s_wch[wv] = ich;
-    s_went[wv] = ient;
+    s_first[lane] = incl - n;
   }
   __syncthreads();
-  uint32_t wb_ch = 0, wb_ent = 0, tot_ch = 0, tot_ent = 0;
-  for (uint32_t k = 0; k < NW; ++k) {
-    if (k < wv) {
-      wb_ch += s_wch[k];
-      wb_ent += s_went[k];
+  const uint32_t sum_ch = s_part[0][0] + s_part[0][1] + s_part[0][2] + s_part[0][3];
+  const uint32_t before = totals ? 0u : sum_ch;
+  for (uint32_t i = wv; i < ns; i += 4u) {
+    const uint2 sl = slabs[s0 + i];
+    const uint32_t nch = (sl.y + 63u) >> 6, base = before + s_first[i];
+    for (uint32_t j = lane; j < nch; j += 64u) {
+      const uint32_t c = base + j;
+      if (c < max_chunks) desc[c] = make_uint2(sl.x + (j << 6), min(64u, sl.y - (j << 6)));
     }
-    tot_ch += s_wch[k];
-    tot_ent += s_went[k];
   }
-  uint32_t run = wb_ch + ich - ch;
-  for (uint32_t s = b; s < e; ++s) {
-    s_pref[s] = run;
-    run += (slabs[s].y + 63u) >> 6;
-  }
-  if (tid == 0) s_pref[n_slabs] = tot_ch;
-  __syncthreads();
-  const uint32_t total = min(tot_ch, max_chunks);
-  for (uint32_t c = tid; c < total; c += (uint32_t)NT) {
-    uint32_t lo = 0, hi = n_slabs;  // the slab s with s_pref[s] <= c < s_pref[s + 1]
-    while (hi - lo > 1u) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (s_pref[mid] <= c) lo = mid;
-      else hi = mid;
-    }
-    const uint32_t k = c - s_pref[lo];
-    const uint2 sl = slabs[lo];
-    desc[c] = make_uint2(sl.x + (k << 6), min(64u, sl.y - (k << 6)));
-  }
-  if (tid == 0) {
-    *n_chunks = total;
+  if (totals && tid == 0) {
+    const uint32_t tot_ent = s_part[1][0] + s_part[1][1] + s_part[1][2] + s_part[1][3];
+    *n_chunks = min(sum_ch, max_chunks);
     *n_entries = tot_ent;
     if (ctr_entries >= 0 && tot_ent) atomicAdd((unsigned long long*)&ctr[ctr_entries], (unsigned long long)tot_ent);
-    if (tot_ch > max_chunks)
-      atomicAdd((unsigned long long*)&ctr[CTR_DROP], (unsigned long long)(tot_ch - max_chunks) * 64ull);
+    if (sum_ch > max_chunks)
+      atomicAdd((unsigned long long*)&ctr[CTR_DROP], (unsigned long long)(sum_ch - max_chunks) * 64ull);
   }
 }
 
@@ -604,8 +590,8 @@ hipError_t launch_nee(const Params& P, const Batch& B, const RayQ& q, const Chun
 hipError_t launch_slab_index(const uint2* slabs, uint32_t n_slabs, uint2* desc, uint32_t max_chunks, uint32_t* n_chunks,
                              uint32_t* n_entries, uint64_t* ctr, int ctr_entries, hipStream_t s) {
   if (n_slabs == 0 || n_slabs > PRT_SLAB_INDEX_MAX) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_slab_index<PRT_INDEX_THREADS>, dim3(1), dim3(PRT_INDEX_THREADS), 0, s, slabs, n_slabs, desc,
-                     max_chunks, n_chunks, n_entries, ctr, ctr_entries);
+  hipLaunchKernelGGL(k_slab_index, dim3((n_slabs + PRT_INDEX_SLABS - 1) / PRT_INDEX_SLABS), dim3(256), 0, s, slabs,
+                     n_slabs, desc, max_chunks, n_chunks, n_entries, ctr, ctr_entries);
   return hipGetLastError();
 }
 hipError_t launch_resolve(const Params& P, const Batch& B, const uint64_t* accum, void* rgb, void* var, int f64,

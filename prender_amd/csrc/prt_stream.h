@@ -25,7 +25,7 @@
 //   extend             closest hit of every chunk of the current queue
 //   shade              one vertex per queued ray and per new camera sample;
 //                      children go to the wave's slab of the next queue
-//   index   (1 block)  chunk list of the next queue (NEE input, and the next
+//   index              chunk list of the next queue (NEE input, and the next
 //                      iteration's extend / shade input)
 //   nee / index / shadow  light samples of the diffuse vertices into shadow
 //                      slabs, their chunk list, any-hit tests

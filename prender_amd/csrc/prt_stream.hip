@@ -380,22 +380,30 @@ __global__ __launch_bounds__(256) void k_stream_shade(Stream st, RayQ qin, const
   }
 }
 
-// One block of 1024 threads: the chunk list of a queue from its slab
-// writers' runs (which 0 / 1: ray queue p; 2: the shadow queue), in writer
-// order, and its chunk and entry totals.  Chunks beyond the list's capacity
-// (never, with the host's sizing) are dropped and counted.
-__global__ __launch_bounds__(1024) void k_stream_index(Stream st, int which) {
-  __shared__ uint32_t s_wch[16], s_went[16];
+// The chunk list of a queue from its slab writers' runs (which 0 / 1: ray
+// queue p; 2: the shadow queue), in writer order, and its chunk and entry
+// totals.  Block g lists the chunks of the PRT_INDEX_SLABS writers from
+// g * PRT_INDEX_SLABS on: it sums the chunks of every writer before them over
+// their run records, scans its own writers' chunk counts in one wave, and
+// writes their descriptors, a wave per writer and a lane per chunk.  Block 0
+// sums every writer instead and stores the totals.  Chunks beyond the list's
+// capacity (never, with the host's sizing) are dropped and counted.
+__global__ __launch_bounds__(256) void k_stream_index(Stream st, int which) {
+  __shared__ uint32_t s_part[2][4];
+  __shared__ uint32_t s_first[PRT_INDEX_SLABS];
   const bool sh = which == 2;
   const RunRec* runs = sh ? st.sh_runs : st.ray_runs;
   const uint32_t* nruns = sh ? st.sh_nruns : st.ray_nruns;
   ChunkDesc* desc = sh ? st.sh_desc : st.desc[which];
   const uint32_t max_desc = sh ? st.max_sh_desc : st.max_desc;
   const uint32_t tid = threadIdx.x, lane = lane_id(), wv = tid >> 6;
-  const uint32_t W = PRT_SLAB_WAVES, per = (W + 1023u) / 1024u;
-  const uint32_t b = min(tid * per, W), e = min(b + per, W);
+  const uint32_t W = PRT_SLAB_WAVES;
+  const uint32_t w0 = blockIdx.x * (uint32_t)PRT_INDEX_SLABS;  // < W by the grid
+  const uint32_t nw = min(W - w0, (uint32_t)PRT_INDEX_SLABS);
+  const bool totals = blockIdx.x == 0;
+  const uint32_t lim = totals ? W : w0;
   uint32_t ch = 0, ent = 0;
-  for (uint32_t w = b; w < e; ++w) {
+  for (uint32_t w = tid; w < lim; w += 256u) {
     const uint32_t n = min(nruns[w], st.max_runs);
     for (uint32_t r = 0; r < n; ++r) {
       const uint32_t cnt = runs[(size_t)w * st.max_runs + r].cnt;
@@ -403,51 +411,58 @@ __global__ __launch_bounds__(1024) void k_stream_index(Stream st, int which) {
       ent += cnt;
     }
   }
-  uint32_t ich = ch, ient = ent;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t a1 = (uint32_t)__shfl_up((int)ich, o, 64), a2 = (uint32_t)__shfl_up((int)ient, o, 64);
-    if (lane >= (uint32_t)o) {
-      ich += a1;
-      ient += a2;
-    }
+  ch = (uint32_t)wave_sum_u32(ch);
+  ent = (uint32_t)wave_sum_u32(ent);
+  if (lane == 0) {
+    s_part[0][wv] = ch;
+    s_part[1][wv] = ent;
   }
-  if (lane == 63) {
-    s_wch[wv] = ich;
-    s_went[wv] = ient;
+  if (wv == 0) {
+    // exclusive scan of this block's writers' chunk counts
+    uint32_t n = 0;
+    if (lane < nw) {
+      const uint32_t w = w0 + lane, nr = min(nruns[w], st.max_runs);
+      for (uint32_t r = 0; r < nr; ++r) n += (runs[(size_t)w * st.max_runs + r].cnt + 63u) >> 6;
+    }
+    uint32_t incl = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+      if (lane >= (uint32_t)o) incl += y;
+    }
+    s_first[lane] = incl - n;
   }
   __syncthreads();
-  uint32_t base = 0, tot_ch = 0, tot_ent = 0;
-  for (uint32_t k = 0; k < 16; ++k) {
-    if (k < wv) base += s_wch[k];
-    tot_ch += s_wch[k];
-    tot_ent += s_went[k];
-  }
-  base += ich - ch;
+  const uint32_t sum_ch = s_part[0][0] + s_part[0][1] + s_part[0][2] + s_part[0][3];
   uint32_t lost = 0;
-  for (uint32_t w = b; w < e; ++w) {
-    const uint32_t n = min(nruns[w], st.max_runs);
-    for (uint32_t r = 0; r < n; ++r) {
+  for (uint32_t i = wv; i < nw; i += 4u) {
+    const uint32_t w = w0 + i, nr = min(nruns[w], st.max_runs);
+    uint32_t base = (totals ? 0u : sum_ch) + s_first[i];
+    for (uint32_t r = 0; r < nr; ++r) {
       const RunRec rr = runs[(size_t)w * st.max_runs + r];
-      for (uint32_t j = 0; j < rr.cnt; j += 64) {
-        if (base < max_desc) {
+      const uint32_t nch = (rr.cnt + 63u) >> 6;
+      for (uint32_t j = lane; j < nch; j += 64u) {
+        const uint32_t c = base + j, m = min(64u, rr.cnt - (j << 6));
+        if (c < max_desc) {
           ChunkDesc d;
-          d.base = rr.off + j;
-          d.cnt = min(64u, rr.cnt - j);
+          d.base = rr.off + (j << 6);
+          d.cnt = m;
           d.scene = rr.scene;
           d.pad_ = 0;
-          desc[base] = d;
+          desc[c] = d;
         } else {
-          lost += min(64u, rr.cnt - j);
+          lost += m;
         }
-        ++base;
       }
+      base += nch;
     }
   }
-  if (lost) atomicAdd((unsigned long long*)&st.ctr[CTR_DROP], (unsigned long long)lost);
-  if (tid == 0) {
+  const uint64_t s_lost = wave_sum_u32(lost);
+  if (lane == 0 && s_lost) atomicAdd((unsigned long long*)&st.ctr[CTR_DROP], (unsigned long long)s_lost);
+  if (totals && tid == 0) {
     StreamCtl* c = st.ctl;
-    const uint32_t n_ch = min(tot_ch, max_desc);
+    const uint32_t n_ch = min(sum_ch, max_desc);
+    const uint32_t tot_ent = s_part[1][0] + s_part[1][1] + s_part[1][2] + s_part[1][3];
     if (sh) {
       c->sh_chunks = n_ch;
       c->sh_entries = tot_ent;
@@ -635,7 +650,8 @@ hipError_t launch_stream_shade(const Stream& st, const RayQ& qin, const double* 
   return hipGetLastError();
 }
 hipError_t launch_stream_index(const Stream& st, int which, hipStream_t s) {
-  hipLaunchKernelGGL(k_stream_index, dim3(1), dim3(1024), 0, s, st, which);
+  hipLaunchKernelGGL(k_stream_index, dim3((PRT_SLAB_WAVES + PRT_INDEX_SLABS - 1) / PRT_INDEX_SLABS), dim3(256), 0, s, st,
+                     which);
   return hipGetLastError();
 }
 hipError_t launch_stream_nee(const Stream& st, const RayQ& q, int nxt, const ShadowQ& sq, hipStream_t s) {

@@ -333,7 +333,8 @@ class Scene:
 # ---- batched scenes
 def batch_opts(seed: int = 0, tile_samples: int = 0, inflight_tiles: int = 0, stack_lds: int = 0,
                trav_blocks_per_cu: int = 0, use_graph: bool = True, tile_stride: int = 0,
-               max_depth: int = 0, timing: bool = False, persistent: int = 0) -> PrtBatchOpts:
+               max_depth: int = 0, timing: bool = False, persistent: int = 0,
+               persist_waves: int = 0) -> PrtBatchOpts:
     o = PrtBatchOpts()
     o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
     o.tile_samples = int(tile_samples)
@@ -345,6 +346,7 @@ def batch_opts(seed: int = 0, tile_samples: int = 0, inflight_tiles: int = 0, st
     o.max_depth = int(max_depth)
     o.timing = 1 if timing else 0
     o.persistent = int(persistent)  # 0 / False tile queue, 1 / True one persistent launch, 2 split by NEE
+    o.persist_waves = int(persist_waves)  # persistent kernel's waves per SIMD (0 = the mode's default)
     return o
 
 

@@ -130,7 +130,7 @@ class PrtBatchOpts(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("tile_samples", C.c_int32), ("inflight_tiles", C.c_int32),
                 ("stack_lds", C.c_int32), ("trav_blocks_per_cu", C.c_int32), ("use_graph", C.c_int32),
                 ("tile_stride", C.c_int32), ("max_depth", C.c_int32), ("timing", C.c_int32),
-                ("persistent", C.c_int32), ("pad_", C.c_int32)]
+                ("persistent", C.c_int32), ("persist_waves", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -46,8 +46,10 @@ def _run_batch(torch, prt, scenes, **kw):
     {"persistent": True},
     {"persistent": True, "trav_blocks_per_cu": 3, "stack_lds": 8},
     {"persistent": True, "trav_blocks_per_cu": 4},
+    {"persistent": True, "persist_waves": 3},
     {"persistent": 2},
     {"persistent": 2, "tile_samples": 1024, "stack_lds": 8, "use_graph": False},
+    {"persistent": 2, "persist_waves": 4},
 ])
 def test_batch_matches_per_scene_renders(batch_env, kw):
     torch, prt, scenes, ref = batch_env
@@ -83,9 +85,10 @@ def test_tuner_returns_valid_options(batch_env):
     assert ns > 0
     assert d["stack_lds"] in (8, 16, 32) and d["trav_blocks_per_cu"] in (4, 8, 16)
     assert d["tile_samples"] in (1024, 4096, 16384) and d["inflight_tiles"] >= 1
-    assert d["persistent"] in (0, 1, 2)
+    assert d["persistent"] in (0, 1, 2) and d["persist_waves"] in (0, 2, 3, 4)
     # the tuned options still render identically
     o2, v2, _ = _run_batch(torch, prt, scenes, **{k: d[k] for k in ("tile_samples", "inflight_tiles", "stack_lds",
-                                                                      "trav_blocks_per_cu", "persistent")})
+                                                                      "trav_blocks_per_cu", "persistent",
+                                                                      "persist_waves")})
     for i, o in enumerate(o2):
         assert torch.equal(o, ref[i][0])
