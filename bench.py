@@ -53,6 +53,10 @@ def parse():
                     help="cfg2 (default) is the metric's configuration; the others are extra measurements "
                          "(cfg4: 64-scene batch vs per-scene renders, cfg5: inverse rendering)")
     ap.add_argument("--no-tune", action="store_true", help="cfg4: skip the batch tuner (library defaults)")
+    ap.add_argument("--batch-opts", default=None,
+                    help='cfg4: fixed batch options as JSON (e.g. \'{"persistent": 2, "stack_lds": 8}\'); no tuner')
+    ap.add_argument("--no-trials", action="store_true",
+                    help="cfg4: time only the batched renders (no persistent-engine trials, no per-scene renders)")
     ap.add_argument("--adam-steps", type=int, default=200, help="cfg5: optimisation steps")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference dispatch for PMC byte counters)")
@@ -195,9 +199,9 @@ def main():
     traffic = measured_traffic(dom, args.workload)
     if traffic is not None:
         out["roofline"]["traffic"] = traffic["traffic_bytes_per_launch"]
-        out["roofline"]["traffic_source"] = ("profiles/k_extend_cfg2_hbm_traffic.md: rocprofv3 FETCH_SIZE x %.3f + "
+        out["roofline"]["traffic_source"] = ("profiles/%s_%s_hbm_traffic.md: rocprofv3 FETCH_SIZE x %.3f + "
                                              "WRITE_SIZE x %.3f per dispatch (scales from a 1 GiB copy / fill)"
-                                             % (traffic["fetch_scale"], traffic["write_scale"]))
+                                             % (dom, args.workload, traffic["fetch_scale"], traffic["write_scale"]))
     band = cpu_band(desc, args)
     gpu_frames = None
     if not args.no_rmse:
@@ -383,7 +387,10 @@ def bench_batch(args, torch, G, prt, info):
     samples = sum(s.width * s.height * s.desc.spp for s in scenes)
     tris = sum(s.info()["tris"] for s in scenes)
     t0 = time.time()
-    if args.no_tune:
+    if args.batch_opts:
+        opts = prt.batch_opts(**json.loads(args.batch_opts))
+        tune_ns = None
+    elif args.no_tune:
         opts = prt.batch_opts()
         tune_ns = None
     else:
@@ -433,50 +440,60 @@ def bench_batch(args, torch, G, prt, info):
     def roofline_of(k):
         b, ms, nl = roof[k]
         a = b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-        return {"bound": "hbm", "kernel": k, "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": a / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": b / max(1, nl),
-                "avg_launch_ms": ms / max(1, nl)}
+        r = {"bound": "hbm", "kernel": k, "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": a / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": b / max(1, nl),
+             "avg_launch_ms": ms / max(1, nl)}
+        tr = measured_traffic(k, "cfg4")
+        cur = opts.as_dict()
+        if tr is not None and all(cur.get(key) == v for key, v in (tr.get("batch_opts") or {}).items()):
+            r["traffic"] = tr["traffic_bytes_per_launch"]
+            r["traffic_source"] = ("profiles/%s_cfg4_hbm_traffic.md: rocprofv3 FETCH_SIZE x %.3f + WRITE_SIZE x %.3f "
+                                   "per dispatch, batch options %s" % (k, tr["fetch_scale"], tr["write_scale"],
+                                                                      tr.get("batch_opts")))
+        return r
     dom = max(roof, key=lambda k: roof[k][1])  # the kernel with the most device time
-    # the persistent engine: one launch over every sample of the batch; one
-    # trial per register budget (2, 3 or 4 waves per SIMD), the fastest timed
-    progress("persistent engine")
-    pers = {}
-    trials = {}
-    for wv in (2, 3, 4):
-        po = prt.batch_opts(seed=args.seed + args.steps - 1, persistent=True, trav_blocks_per_cu=wv)
-        trials[wv] = prt.render_batch_device(scenes, ptrs, opts=po)["ms_total"]
-    best_w = min(trials, key=trials.get)
-    torch.cuda.synchronize()
-    pers_diff = [i for i, (a, b) in enumerate(zip(last, outs)) if not torch.equal(a, b)]
-    pers_same = not pers_diff
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        po = prt.batch_opts(seed=args.seed + i, persistent=True, trav_blocks_per_cu=best_w)
-        pst = prt.render_batch_device(scenes, ptrs, opts=po)
-    torch.cuda.synchronize()
-    pers_s = (time.perf_counter() - t0) / args.steps
-    pers = {"samples_per_s": samples / pers_s, "ms_per_step": 1e3 * pers_s,
-            "waves_per_simd": best_w, "trial_ms_by_waves": trials,
-            "images_identical_to_graph_form": pers_same, "differing_scenes": pers_diff[:16],
-            "over_graph_form": batched_s / pers_s, "over_per_scene": None,
-            "kernel_ms": pst["ms_extend"], "resolve_ms": pst["ms_other"]}
-    progress("per-scene renders of the same scenes")
-    # sequential per-scene renders of the same work
-    for s, o in zip(scenes, outs):  # warm-up: workspaces
-        s.render_device(o.data_ptr(), seed=1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        for s, o in zip(scenes, outs):
-            s.render_device(o.data_ptr(), seed=args.seed + i)
-    torch.cuda.synchronize()
-    seq_s = (time.perf_counter() - t0) / args.steps
-    max_diff = max(float((a - b).abs().max()) for a, b in zip(last, outs))
-    seq_diff = [i for i, (a, b) in enumerate(zip(last, outs)) if not torch.equal(a, b)]
+    pers = seq_s = max_diff = seq_diff = None
+    if not args.no_trials:
+        # the persistent engine: one launch over every sample of the batch; one
+        # trial per register budget (2, 3 or 4 waves per SIMD), the fastest timed
+        progress("persistent engine")
+        pers = {}
+        trials = {}
+        for wv in (2, 3, 4):
+            po = prt.batch_opts(seed=args.seed + args.steps - 1, persistent=True, trav_blocks_per_cu=wv)
+            trials[wv] = prt.render_batch_device(scenes, ptrs, opts=po)["ms_total"]
+        best_w = min(trials, key=trials.get)
+        torch.cuda.synchronize()
+        pers_diff = [i for i, (a, b) in enumerate(zip(last, outs)) if not torch.equal(a, b)]
+        pers_same = not pers_diff
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            po = prt.batch_opts(seed=args.seed + i, persistent=True, trav_blocks_per_cu=best_w)
+            pst = prt.render_batch_device(scenes, ptrs, opts=po)
+        torch.cuda.synchronize()
+        pers_s = (time.perf_counter() - t0) / args.steps
+        pers = {"samples_per_s": samples / pers_s, "ms_per_step": 1e3 * pers_s,
+                "waves_per_simd": best_w, "trial_ms_by_waves": trials,
+                "images_identical_to_batched": pers_same, "differing_scenes": pers_diff[:16],
+                "over_batched": batched_s / pers_s, "over_per_scene": None,
+                "kernel_ms": pst["ms_extend"], "resolve_ms": pst["ms_other"]}
+        progress("per-scene renders of the same scenes")
+        # sequential per-scene renders of the same work
+        for s, o in zip(scenes, outs):  # warm-up: workspaces
+            s.render_device(o.data_ptr(), seed=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            for s, o in zip(scenes, outs):
+                s.render_device(o.data_ptr(), seed=args.seed + i)
+        torch.cuda.synchronize()
+        seq_s = (time.perf_counter() - t0) / args.steps
+        max_diff = max(float((a - b).abs().max()) for a, b in zip(last, outs))
+        seq_diff = [i for i, (a, b) in enumerate(zip(last, outs)) if not torch.equal(a, b)]
+        pers["over_per_scene"] = seq_s / pers_s
     sum_ = lambda k: sum(s[k] for s in stats)
     rays = sum_("primary_rays") + sum_("extension_rays") + sum_("shadow_rays")
     ext_ms = tst["ms_extend"]
-    pers["over_per_scene"] = seq_s / pers_s
     out = {
         "metric": METRIC, "value": samples / batched_s, "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * batched_s, "higher_is_better": True, "scaling": "weak",
@@ -485,11 +502,12 @@ def bench_batch(args, torch, G, prt, info):
                                "log-uniform), 128^2..512^2, 8..64 spp (SUPERSAMPLING 2), DEPTH 5 / DEPTH_LIMIT 8",
                    "scenes": len(scenes), "camera_samples": samples, "triangles": tris, "parallelism": "1 GPU"},
         "mrays_per_s": rays / (batched_s * args.steps) / 1e6,
-        "per_scene_launches": {"samples_per_s": samples / seq_s, "ms_per_step": 1e3 * seq_s},
-        "batched_over_per_scene": seq_s / batched_s,
+        "per_scene_launches": {"samples_per_s": samples / seq_s, "ms_per_step": 1e3 * seq_s} if seq_s else None,
+        "batched_over_per_scene": seq_s / batched_s if seq_s else None,
         "persistent": pers,
-        "images_identical": max_diff == 0.0, "max_abs_diff": max_diff,
-        "differing_scenes": [(i, bool(descs[i].sampling.next_event_estimation)) for i in seq_diff[:16]],
+        "images_identical": max_diff == 0.0 if seq_s else None, "max_abs_diff": max_diff,
+        "differing_scenes": ([(i, bool(descs[i].sampling.next_event_estimation)) for i in seq_diff[:16]]
+                             if seq_s else None),
         "batch_options": opts.as_dict(), "tuner_ns_per_sample": tune_ns, "tune_s": tune_s,
         "iterations_per_step": stats[-1]["launches_extend"],
         "rays_per_step": {"primary": stats[-1]["primary_rays"], "extension": stats[-1]["extension_rays"],
