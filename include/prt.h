@@ -18,9 +18,15 @@
  *   - the caller keeps ownership of every array it passes; arrays are copied
  *     to device memory inside the call;
  *   - a scene handle is not thread-safe and owns two HIP streams (render work,
- *     and shadow rays traced alongside it); different
- *     handles may be used from different threads (no library globals besides
- *     the thread-local error string).
+ *     and shadow rays traced alongside it); different handles may be used
+ *     from different threads;
+ *   - the render memory (ray and shadow queues, sample accumulators, the
+ *     batched renderer's workspace) is one pool per device, shared by every
+ *     handle on it and sized for the largest call so far: render, trace,
+ *     batch, adjoint and Metropolis calls on one device take turns on it
+ *     (a call holds the device's pool for its duration; calls return after
+ *     the device is done), and the pool is freed with the device's last
+ *     handle.
  */
 #ifndef PRT_H
 #define PRT_H

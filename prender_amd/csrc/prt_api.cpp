@@ -76,6 +76,33 @@ int dev_alloc(DevBuf& b, size_t bytes) {
   return PRT_OK;
 }
 
+namespace {
+constexpr int kMaxPoolDevices = 64;
+std::mutex g_pool_mu;
+DevicePool* g_pools[kMaxPoolDevices] = {};
+}  // namespace
+
+DevicePool* acquire_device_pool(int device) {
+  if (device < 0 || device >= kMaxPoolDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  if (!g_pools[device]) g_pools[device] = new (std::nothrow) DevicePool();
+  if (g_pools[device]) g_pools[device]->scenes++;
+  return g_pools[device];
+}
+
+void release_device_pool(int device) {
+  if (device < 0 || device >= kMaxPoolDevices) return;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  DevicePool* p = g_pools[device];
+  if (!p || --p->scenes > 0) return;
+  // the last handle on the device is gone: free the memory (the pool object,
+  // whose lock another thread could still name, stays)
+  std::lock_guard<std::recursive_mutex> l2(p->m);
+  p->ws.release();
+  release_stream_workspace(p->sws);
+  p->sws = nullptr;
+}
+
 void Workspace::release() {
   for (auto& a : q)
     for (auto& b : a) b.release();
@@ -188,9 +215,8 @@ void prt_scene_destroy(prt_scene* s) {
   if (s->stream2) (void)hipStreamSynchronize(s->stream2);
   s->nodes.release(); s->tris.release(); s->tri_nrm.release(); s->tri_alb.release();
   s->ents.release(); s->mats.release(); s->analytic.release(); s->lights.release(); s->ovf.release();
-  s->ws.release();
-  release_stream_workspace(s->sws);
-  s->sws = nullptr;
+  if (s->pool) release_device_pool(s->device);
+  s->pool = nullptr;
   s->out_rgb.release();
   s->out_var.release();
   s->d_slot_of_tri.release();
@@ -230,6 +256,8 @@ static int scene_build(prt_scene* s, const prt_scene_desc* d) {
     return fail(PRT_E_NODEVICE, std::string("libprt is built for gfx950, device is ") + prop.gcnArchName);
   s->device = dev;
   s->cu_count = prop.multiProcessorCount;
+  s->pool = acquire_device_pool(dev);
+  if (!s->pool) return fail(PRT_E_NOMEM, "no render memory pool for this device");
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   HIPCHK(hipStreamCreateWithFlags(&s->stream2, hipStreamNonBlocking));
   HIPCHK(hipEventCreateWithFlags(&s->ev_nee[0], hipEventDisableTiming));
@@ -494,19 +522,20 @@ extern "C++" void prt::fill_stats(prt_stats* st, const uint64_t* c) {
 
 static int trace_device(prt_scene* s, const prt_ray* d_rays, int64_t n, prt_hit* d_hits, prt_stats* stats) {
   HIPCHK(hipSetDevice(s->device));
+  std::lock_guard<std::recursive_mutex> pool_lock(s->pool->m);
   int rc;
-  if ((rc = dev_alloc(s->ws.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
-  HIPCHK(hipMemsetAsync(s->ws.ctr.p, 0, CTR_COUNT * sizeof(uint64_t), s->stream));
+  if ((rc = dev_alloc(s->pool->ws.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
+  HIPCHK(launch_fill_u64((uint64_t*)s->pool->ws.ctr.p, 0ull, CTR_COUNT, s->stream));
   ScopedEvent e0, e1;
   HIPCHK(hipEventCreate(&e0.e));
   HIPCHK(hipEventCreate(&e1.e));
   HIPCHK(hipEventRecord(e0.e, s->stream));
   if (n > 0)
-    HIPCHK(launch_trace(s->P, (const prt_ray_dev*)d_rays, n, (prt_hit_dev*)d_hits, (uint64_t*)s->ws.ctr.p,
+    HIPCHK(launch_trace(s->P, (const prt_ray_dev*)d_rays, n, (prt_hit_dev*)d_hits, (uint64_t*)s->pool->ws.ctr.p,
                         s->max_trav_blocks, s->stream));
   HIPCHK(hipEventRecord(e1.e, s->stream));
   uint64_t c[CTR_COUNT];
-  HIPCHK(hipMemcpyAsync(c, s->ws.ctr.p, sizeof(c), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipMemcpyAsync(c, s->pool->ws.ctr.p, sizeof(c), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0.e, e1.e));
@@ -581,7 +610,7 @@ int prt_trace_closest(prt_scene* s, const prt_ray* rays, int64_t n, prt_hit* hit
 
 // ---------------------------------------------------------------- render
 extern "C++" int prt::ensure_workspace(prt_scene* s, int64_t batch_samples, bool two_accum) {
-  Workspace& w = s->ws;
+  Workspace& w = s->pool->ws;
   // A slab writer over input chunks [c0, c1) owns fan * 64 * (c1 - c0)
   // entries from fan * 64 * c0 on, so a launch's outputs span fan * 64 *
   // (input chunks) <= fan * (input rays + 64 * PRT_SLAB_WAVES).  With glass a
@@ -590,15 +619,22 @@ extern "C++" int prt::ensure_workspace(prt_scene* s, int64_t batch_samples, bool
   // Shadow rays: three per ray of the NEE input.
   const int64_t grow = s->has_glass ? 8 : 1;
   const int64_t W = PRT_SLAB_WAVES;
-  const int64_t qcap = grow * batch_samples + 2 * 64 * W + 64;
-  const int64_t scap = 3 * (grow * batch_samples + 64 * W);
+  int64_t qcap = grow * batch_samples + 2 * 64 * W + 64;
+  int64_t scap = 3 * (grow * batch_samples + 64 * W);
   if (qcap >= ((int64_t)1 << 32) || scap >= ((int64_t)1 << 32)) return fail(PRT_E_INVALID, "batch too large");
   // the second shadow queue of the alternating schedule (NEE scenes only)
-  const bool two_sq = PRT_OVERLAP_SHADOW == 3 && s->P.nee && s->P.n_lights > 0;
+  bool two_sq = PRT_OVERLAP_SHADOW == 3 && s->P.nee && s->P.n_lights > 0;
   if (w.batch_samples >= batch_samples && w.queue_cap >= qcap && w.shadow_cap >= scap && w.h_counts &&
       (!two_accum || w.accum2.p) && (!two_sq || w.sq2[0].p))
     return PRT_OK;
-  w.release();
+  // The device's workspace serves every scene: it only grows (to the largest
+  // sizes any call needed, keeping the optional buffers once they exist), so
+  // scenes of different sizes in turn do not reallocate it back and forth.
+  batch_samples = std::max(batch_samples, w.batch_samples);
+  qcap = std::max(qcap, w.queue_cap);
+  scap = std::max(scap, w.shadow_cap);
+  two_sq = two_sq || w.sq2[0].p != nullptr;
+  two_accum = two_accum || w.accum2.p != nullptr;
   int rc;
   for (int qi = 0; qi < 2; ++qi)
     if ((rc = alloc_ray_queue(w.q[qi], qcap))) return rc;
@@ -616,7 +652,7 @@ extern "C++" int prt::ensure_workspace(prt_scene* s, int64_t batch_samples, bool
   if ((rc = dev_alloc(w.fetch, (size_t)FETCH_LAUNCHES * PRT_FETCH_COUNTERS * PRT_FETCH_STRIDE * sizeof(uint32_t))))
     return rc;
   if ((rc = dev_alloc(w.ctr, CTR_COUNT * sizeof(uint64_t)))) return rc;
-  HIPCHK(hipHostMalloc((void**)&w.h_counts, 16 * sizeof(uint32_t), hipHostMallocDefault));
+  if (!w.h_counts) HIPCHK(hipHostMalloc((void**)&w.h_counts, 16 * sizeof(uint32_t), hipHostMallocDefault));
   w.batch_samples = batch_samples;
   w.queue_cap = qcap;
   w.shadow_cap = scap;
@@ -674,7 +710,7 @@ static int setup_render(prt_scene* s, const prt_render_opts* o, RenderSetup& R) 
 // the batch's end waits for every shadow launch.
 static int run_batch_on(prt_scene* s, const Params& P, const Batch& B, uint64_t* accum, const Adjoint& A, Timer& tm,
                         CallTotals& tot) {
-  Workspace& w = s->ws;
+  Workspace& w = s->pool->ws;
   hipStream_t st = s->stream;
   const int overlap = A.on ? 0 : PRT_OVERLAP_SHADOW;
   hipStream_t st_sh = overlap ? s->stream2 : st;
@@ -854,7 +890,7 @@ extern "C++" int prt::run_batch(prt_scene* s, const Params& P, const Batch& B, u
 static int finish_stats(prt_scene* s, const RenderSetup& R, Timer& tm, const CallTotals& tot, prt_stats* stats,
                         uint64_t* c) {
   hipStream_t st = s->stream;
-  HIPCHK(hipMemcpyAsync(c, s->ws.ctr.p, CTR_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(c, s->pool->ws.ctr.p, CTR_COUNT * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   if (c[CTR_DROP] != 0) return fail(PRT_E_INVALID, "ray queue overflow (" + std::to_string(c[CTR_DROP]) + " rays)");
   // every queued ray must have been traced exactly once
@@ -935,6 +971,7 @@ struct ImageDumper {
 static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, float* d_var, prt_stats* stats,
                          ImageDumper* dumper = nullptr) {
   HIPCHK(hipSetDevice(s->device));
+  std::lock_guard<std::recursive_mutex> pool_lock(s->pool->m);
   // Scenes with glass stream through the batched renderer's tile queue: a
   // depth-synchronous batch of them drains through long, thin tails after
   // the glass splits, which the stream keeps filled with new tiles.  Same
@@ -993,7 +1030,7 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   if ((rc = setup_render(s, o, R))) return rc;
   if ((rc = ensure_workspace(s, R.bp * R.P.spp, false))) return rc;
   hipStream_t st = s->stream;
-  HIPCHK(launch_fill_u64((uint64_t*)s->ws.ctr.p, 0ull, CTR_COUNT, st));
+  HIPCHK(launch_fill_u64((uint64_t*)s->pool->ws.ctr.p, 0ull, CTR_COUNT, st));
   Timer tm;
   if ((rc = tm.begin(o && o->timing != 0, st, &s->timer_events))) return rc;
   CallTotals tot;
@@ -1010,8 +1047,8 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
   }
   for (int64_t pb = 0; pb < R.n_pix; pb += R.bp) {
     Batch B{pb, std::min(R.bp, R.n_pix - pb)};
-    if ((rc = run_batch(s, R.P, B, (uint64_t*)s->ws.accum.p, none, tm, tot))) return rc;
-    HIPCHK(launch_resolve(R.P, B, (const uint64_t*)s->ws.accum.p, d_rgb, d_var, f64, st));
+    if ((rc = run_batch(s, R.P, B, (uint64_t*)s->pool->ws.accum.p, none, tm, tot))) return rc;
+    HIPCHK(launch_resolve(R.P, B, (const uint64_t*)s->pool->ws.accum.p, d_rgb, d_var, f64, st));
     if ((rc = tm.mark(T_OTHER))) return rc;
     const int64_t done = pb + B.n_pixels;
     if (dumping && done < R.n_pix && dumper->due() &&
@@ -1134,6 +1171,7 @@ int prt_set_emission(prt_scene* s, int entity, const double emission[3]) {
 int prt_adjoint_device(prt_scene* s, const prt_render_opts* o, const float* d_dimage, int entity,
                        float* d_albedo_grad, int light, double* emission_grad, prt_stats* stats) {
   if (!s || !d_dimage) return fail(PRT_E_INVALID, "bad arguments");
+  std::lock_guard<std::recursive_mutex> pool_lock(s->pool->m);
   if (s->has_glass)
     return fail(PRT_E_UNSUPPORTED, "the adjoint needs one path per sample (no glass splits)");
   if (s->P.cutoff > 0.0f) return fail(PRT_E_UNSUPPORTED, "the adjoint needs luminance_cutoff <= 0");
@@ -1155,7 +1193,7 @@ int prt_adjoint_device(prt_scene* s, const prt_render_opts* o, const float* d_di
   double* g_alb = (double*)s->grad_buf.p;
   double* g_e = g_alb + 3 * nf;
   HIPCHK(launch_fill_u64((uint64_t*)g_alb, 0ull, 3 * nf + 3, st));
-  HIPCHK(launch_fill_u64((uint64_t*)s->ws.ctr.p, 0ull, CTR_COUNT, st));
+  HIPCHK(launch_fill_u64((uint64_t*)s->pool->ws.ctr.p, 0ull, CTR_COUNT, st));
   Timer tm;
   if ((rc = tm.begin(o && o->timing != 0, st, &s->timer_events))) return rc;
   CallTotals tot;
@@ -1165,16 +1203,16 @@ int prt_adjoint_device(prt_scene* s, const prt_render_opts* o, const float* d_di
   A.entity = want_alb ? entity : -1;
   A.light = light;
   A.dimage = d_dimage;
-  A.ltot = (const uint64_t*)s->ws.accum2.p;
+  A.ltot = (const uint64_t*)s->pool->ws.accum2.p;
   A.grad_alb = g_alb;
   A.grad_e = g_e;
   A.scale = R.P.sensor_response * R.P.smp;
   for (int64_t pb = 0; pb < R.n_pix; pb += R.bp) {
     Batch B{pb, std::min(R.bp, R.n_pix - pb)};
-    if ((rc = run_batch(s, R.P, B, (uint64_t*)s->ws.accum2.p, none, tm, tot))) return rc;
-    if ((rc = run_batch(s, R.P, B, (uint64_t*)s->ws.accum.p, A, tm, tot))) return rc;
-    HIPCHK(launch_count_mismatch((const uint64_t*)s->ws.accum.p, (const uint64_t*)s->ws.accum2.p,
-                                 3 * B.n_pixels * R.P.spp, (uint64_t*)s->ws.ctr.p, st));
+    if ((rc = run_batch(s, R.P, B, (uint64_t*)s->pool->ws.accum2.p, none, tm, tot))) return rc;
+    if ((rc = run_batch(s, R.P, B, (uint64_t*)s->pool->ws.accum.p, A, tm, tot))) return rc;
+    HIPCHK(launch_count_mismatch((const uint64_t*)s->pool->ws.accum.p, (const uint64_t*)s->pool->ws.accum2.p,
+                                 3 * B.n_pixels * R.P.spp, (uint64_t*)s->pool->ws.ctr.p, st));
   }
   if (want_alb) HIPCHK(launch_grad_to_f32(g_alb, d_albedo_grad, 3 * nf, st));
   double ge[3] = {0, 0, 0};

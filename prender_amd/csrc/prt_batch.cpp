@@ -169,9 +169,9 @@ StreamGeometry stream_geometry(const BatchConfig& c, bool glass) {
 
 int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, const StreamGeometry& g, int n_scenes,
                             int64_t n_tiles, int ovf_words) {
-  if (!lead->sws) lead->sws = new (std::nothrow) StreamWorkspace();
-  if (!lead->sws) return fail(PRT_E_NOMEM, "out of host memory");
-  StreamWorkspace& w = *lead->sws;
+  if (!lead->pool->sws) lead->pool->sws = new (std::nothrow) StreamWorkspace();
+  if (!lead->pool->sws) return fail(PRT_E_NOMEM, "out of host memory");
+  StreamWorkspace& w = *lead->pool->sws;
   const int64_t T = (int64_t)1 << c.tile_shift;
   const int64_t inflight = (int64_t)c.n_slots * T;
   if (g.q_cap >= ((int64_t)1 << 32) || g.sq_cap >= ((int64_t)1 << 32))
@@ -255,7 +255,7 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
   int ovf_words = 2;  // the deepest BVH of the batch sizes the shared overflow area
   for (int i = 0; i < n; ++i) ovf_words = std::max(ovf_words, (int)scenes[i]->P.ovf_words);
   if ((rc = ensure_stream_workspace(lead, c, geo, n, (int64_t)tiles.size(), ovf_words))) return rc;
-  StreamWorkspace& w = *lead->sws;
+  StreamWorkspace& w = *lead->pool->sws;
   hipStream_t s = lead->stream;
 
   std::vector<Params> params(n);
@@ -562,9 +562,9 @@ int run_persistent(prt_scene* const* scenes, int n, const BatchConfig& c, float*
   const int blocks = cu * per_cu;
   const int64_t threads = (int64_t)blocks * 256;
 
-  if (!lead->sws) lead->sws = new (std::nothrow) StreamWorkspace();
-  if (!lead->sws) return fail(PRT_E_NOMEM, "out of host memory");
-  StreamWorkspace& w = *lead->sws;
+  if (!lead->pool->sws) lead->pool->sws = new (std::nothrow) StreamWorkspace();
+  if (!lead->pool->sws) return fail(PRT_E_NOMEM, "out of host memory");
+  StreamWorkspace& w = *lead->pool->sws;
   int rc;
   const int64_t ovf_threads = (int64_t)cu * kMaxTravBlocksPerCu * 256;
   if ((int64_t)w.ovf.bytes < (int64_t)PRT_OVF_REGIONS * ovf_threads * ovf_words * (int64_t)sizeof(uint32_t) ||
@@ -760,6 +760,7 @@ int prt_render_batch_device(prt_scene* const* scenes, int n_scenes, const prt_ba
                             float* const* d_var, prt_stats* stats) {
   int rc;
   if ((rc = check_batch_args(scenes, n_scenes, d_rgb))) return rc;
+  std::lock_guard<std::recursive_mutex> pool_lock(scenes[0]->pool->m);
   BatchConfig c;
   if ((rc = resolve_config(scenes, n_scenes, opts, c))) return rc;
   if (c.persistent == 1) return run_persistent(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
@@ -776,6 +777,7 @@ int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts*
   int rc;
   if ((rc = check_batch_args(scenes, n_scenes, d_rgb))) return rc;
   if (!best) return fail(PRT_E_INVALID, "null result");
+  std::lock_guard<std::recursive_mutex> pool_lock(scenes[0]->pool->m);
   prt_batch_opts cur{};
   if (base) cur = *base;
   uint64_t total = 0;

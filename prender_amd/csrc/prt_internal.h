@@ -4,9 +4,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include <cstdint>
-#include <string>
 #include <algorithm>
+#include <cstdint>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/prt.h"
@@ -209,6 +210,24 @@ enum { T_EXTEND = 0, T_SHADE = 1, T_SHADOW = 2, T_OTHER = 3, T_EXTEND0 = 4, T_NE
 struct StreamWorkspace;
 void release_stream_workspace(StreamWorkspace* w);
 
+// Render memory of one device, shared by every scene handle on it: the ray
+// and shadow queues, accumulators and counters of the per-scene renderer and
+// the batched renderer's workspace, each sized for the largest call so far.
+// Render calls are synchronous (they return after the device is done), so
+// the pool serves one call at a time: an entry point that uses it holds its
+// lock for the whole call (recursive: entry points call each other).  It is
+// released when the last scene handle on the device is destroyed.
+struct DevicePool {
+  std::recursive_mutex m;
+  Workspace ws;
+  StreamWorkspace* sws = nullptr;
+  int scenes = 0;  // live scene handles on the device (guarded by the registry's lock)
+};
+// The pool of `device`, counting one more user (acquire) or one fewer
+// (release, which frees its memory when the count reaches zero).
+DevicePool* acquire_device_pool(int device);
+void release_device_pool(int device);
+
 // Chain and seed-path state of Metropolis renders (prt_mlt.cpp).
 struct MltWorkspace {
   DevBuf cur, prop, pix_new, pix_old, f_old, img;
@@ -262,8 +281,7 @@ struct prt_scene {
   double bvh_ms = 0;
   bool has_glass = false;
   int max_trav_blocks = 0;
-  prt::Workspace ws;
-  prt::StreamWorkspace* sws = nullptr;  // batched renders led by this scene
+  prt::DevicePool* pool = nullptr;      // the device's shared render memory
   prt::DevBuf out_rgb, out_var;         // staging for host-output renders
   prt::DevBuf d_slot_of_tri;            // input triangle -> leaf slot (per-face albedo updates)
   prt::DevBuf grad_buf;                 // adjoint accumulators (double)

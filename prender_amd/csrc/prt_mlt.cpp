@@ -75,6 +75,7 @@ static int render_mlt(prt_scene* s, const prt_mlt_opts* o, void* d_out, int as_f
   prt_mlt_opts opts{};
   if (o) opts = *o;
   HIPCHK(hipSetDevice(s->device));
+  std::lock_guard<std::recursive_mutex> pool_lock(s->pool->m);
   Params P = s->P;
   MltPlan m;
   int rc;
@@ -109,8 +110,8 @@ static int render_mlt(prt_scene* s, const prt_mlt_opts* o, void* d_out, int as_f
   P.mlt_pix = CH.pix_new;
 
   hipStream_t st = s->stream;
-  uint64_t* ctr = (uint64_t*)s->ws.ctr.p;
-  uint64_t* accum = (uint64_t*)s->ws.accum.p;
+  uint64_t* ctr = (uint64_t*)s->pool->ws.ctr.p;
+  uint64_t* accum = (uint64_t*)s->pool->ws.accum.p;
   HIPCHK(launch_fill_u64(ctr, 0ull, CTR_COUNT, st));
   HIPCHK(launch_fill_u64(CH.img, 0ull, 3 * n_pix, st));
   Timer tm;
@@ -185,6 +186,7 @@ static int render_erpt(prt_scene* s, const prt_erpt_opts* o, void* d_out, int as
   prt_erpt_opts opts{};
   if (o) opts = *o;
   HIPCHK(hipSetDevice(s->device));
+  std::lock_guard<std::recursive_mutex> pool_lock(s->pool->m);
   Params P = s->P;
   const int samples = opts.samples_override > 0 ? opts.samples_override : P.samples;
   if (samples <= 0) return fail(PRT_E_INVALID, "samples must be positive");
@@ -234,8 +236,8 @@ static int render_erpt(prt_scene* s, const prt_erpt_opts* o, void* d_out, int as
                 kC[0], kC[1], 0u, 0.0};
 
   hipStream_t st = s->stream;
-  uint64_t* ctr = (uint64_t*)s->ws.ctr.p;
-  uint64_t* accum = (uint64_t*)s->ws.accum.p;
+  uint64_t* ctr = (uint64_t*)s->pool->ws.ctr.p;
+  uint64_t* accum = (uint64_t*)s->pool->ws.accum.p;
   uint64_t* sums = (uint64_t*)w.sums.p;
   HIPCHK(launch_fill_u64(ctr, 0ull, CTR_COUNT, st));
   HIPCHK(launch_fill_u64(CH.img, 0ull, 3 * n_pix, st));
