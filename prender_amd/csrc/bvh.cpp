@@ -2,6 +2,7 @@
 #include "bvh.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -37,6 +38,11 @@ struct BNode {
 };
 
 constexpr int kBins = 32;
+// 1: 4-wide nodes by the cost-optimal collapse (DpCollapser), 0: by opening
+// the largest child until four (Collapser)
+#ifndef PRT_BVH_DP_COLLAPSE
+#define PRT_BVH_DP_COLLAPSE 1
+#endif
 // SAH costs of a binary node step and of a triangle test
 #ifndef PRT_SAH_COST_TRI
 #define PRT_SAH_COST_TRI 1.0f
@@ -186,6 +192,118 @@ struct Collapser {
   }
 };
 
+// Cost-optimal collapse of the binary tree into 4-wide nodes (dynamic
+// programming over the binary tree, in the manner of Ylitie, Karras and
+// Laine, "Efficient Incoherent Ray Traversal on GPUs Through Compressed
+// Wide BVHs", HPG 2017, for width 4): every binary node n gets the least
+// surface-area cost of its subtree as one root of a wide node's child list,
+// either a leaf (when its triangles, a contiguous range, number at most
+// PRT_MAX_LEAF) or a wide node whose up-to-4 children are the cheapest cut of
+// n's subtree.  Cost = half area x (PRT_WIDE_COST_NODE per wide-node visit,
+// PRT_WIDE_COST_TRI per triangle test).
+#ifndef PRT_WIDE_COST_NODE
+#define PRT_WIDE_COST_NODE 1.0f
+#endif
+#ifndef PRT_WIDE_COST_TRI
+#define PRT_WIDE_COST_TRI 1.0f
+#endif
+struct DpCollapser {
+  const std::vector<BNode>& bn;
+  std::vector<DevNode>& out;
+  int max_depth = 0;
+  std::vector<float> cost;                 // best cost of n as one root
+  std::vector<uint8_t> leaf;               // that best is a leaf
+  std::vector<std::array<float, 5>> dcut;  // least cost of covering n's subtree with at most j roots
+  std::vector<std::array<int8_t, 5>> kcut; // its split: 0 = n itself, k = k roots from the left child
+  std::vector<uint32_t> first, count;
+
+  DpCollapser(const std::vector<BNode>& b, std::vector<DevNode>& o)
+      : bn(b), out(o), cost(b.size()), leaf(b.size()), dcut(b.size()), kcut(b.size()), first(b.size()),
+        count(b.size()) {}
+
+  void eval(int32_t n) {
+    const BNode& b = bn[n];
+    const float a = b.box.half_area();
+    if (b.count > 0) {
+      first[n] = b.first;
+      count[n] = b.count;
+      cost[n] = a * PRT_WIDE_COST_TRI * (float)b.count;
+      leaf[n] = 1;
+    } else {
+      eval(b.left);
+      eval(b.right);
+      first[n] = first[b.left];
+      count[n] = count[b.left] + count[b.right];
+      float open = std::numeric_limits<float>::infinity();
+      for (int k = 1; k <= 3; ++k) open = std::min(open, dcut[b.left][k] + dcut[b.right][4 - k]);
+      const float wide = a * PRT_WIDE_COST_NODE + open;
+      const float as_leaf = count[n] <= (uint32_t)PRT_MAX_LEAF ? a * PRT_WIDE_COST_TRI * (float)count[n]
+                                                               : std::numeric_limits<float>::infinity();
+      leaf[n] = as_leaf <= wide;
+      cost[n] = leaf[n] ? as_leaf : wide;
+    }
+    dcut[n][0] = std::numeric_limits<float>::infinity();
+    kcut[n][0] = 0;
+    for (int j = 1; j <= 4; ++j) {
+      dcut[n][j] = cost[n];
+      kcut[n][j] = 0;
+      if (b.count > 0 || j == 1) continue;
+      for (int k = 1; k < j; ++k) {
+        const float c = dcut[b.left][k] + dcut[b.right][j - k];
+        if (c < dcut[n][j]) {
+          dcut[n][j] = c;
+          kcut[n][j] = (int8_t)k;
+        }
+      }
+    }
+  }
+  void collect(int32_t n, int j, std::vector<int32_t>& roots) const {
+    const int k = kcut[n][j];
+    if (k == 0) {
+      roots.push_back(n);
+      return;
+    }
+    collect(bn[n].left, k, roots);
+    collect(bn[n].right, j - k, roots);
+  }
+  // emits binary node n (internal) as a wide node
+  int32_t emit(int32_t n, int depth) {
+    max_depth = std::max(max_depth, depth);
+    const int32_t me = (int32_t)out.size();
+    out.emplace_back();
+    const BNode& b = bn[n];
+    int kbest = 1;
+    float best = std::numeric_limits<float>::infinity();
+    for (int k = 1; k <= 3; ++k) {
+      const float c = dcut[b.left][k] + dcut[b.right][4 - k];
+      if (c < best) { best = c; kbest = k; }
+    }
+    std::vector<int32_t> roots;
+    collect(b.left, kbest, roots);
+    collect(b.right, 4 - kbest, roots);
+    int32_t code[4] = {0, 0, 0, 0};
+    for (size_t i = 0; i < roots.size(); ++i) {
+      const int32_t r = roots[i];
+      code[i] = leaf[r] ? prt_leaf_code(first[r], count[r]) : emit(r, depth + 1);
+    }
+    DevNode& d = out[me];
+    const float inf = std::numeric_limits<float>::infinity();
+    for (int i = 0; i < 4; ++i) {
+      if (i < (int)roots.size()) {
+        const Box& x = bn[roots[i]].box;
+        d.lo_x[i] = x.lo[0]; d.lo_y[i] = x.lo[1]; d.lo_z[i] = x.lo[2];
+        d.hi_x[i] = x.hi[0]; d.hi_y[i] = x.hi[1]; d.hi_z[i] = x.hi[2];
+      } else {
+        d.lo_x[i] = d.lo_y[i] = d.lo_z[i] = inf;
+        d.hi_x[i] = d.hi_y[i] = d.hi_z[i] = inf;
+      }
+      d.child[i] = code[i];
+      d.pad[i] = 0;
+    }
+    return me;
+  }
+};
+
 // Renumbers the nodes so that those of the top PRT_TOP_LEVELS levels come
 // first, breadth first (the traversal kernels keep these in LDS); the other
 // nodes keep their depth-first order.  Internal child references (>= 0) are
@@ -276,9 +394,15 @@ bool prt_build_bvh(const BvhInput& in, BvhOutput& out, const char** err) {
     out.nodes.push_back(d);
     out.depth = 1;
   } else {
-    Collapser C{B.nodes, out.nodes};
     out.nodes.reserve(B.nodes.size() / 2 + 1);
+#if PRT_BVH_DP_COLLAPSE
+    DpCollapser C(B.nodes, out.nodes);
+    C.eval(0);
     C.emit(0, 1);
+#else
+    Collapser C{B.nodes, out.nodes};
+    C.emit(0, 1);
+#endif
     out.depth = C.max_depth;
   }
   number_top_levels(out);
