@@ -590,12 +590,26 @@ __global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq, in
 // Finished tiles (busy slots whose live count is zero after this iteration's
 // shade; launched after the shadow kernels of the earlier iterations): fold
 // every pixel's samples into its scene's image, clear the slot's
-// accumulators and return the slot to the free list.
+// accumulators and return the slot to the free list.  One thread reads the
+// slot's record for the whole block and the slot is released only after
+// every wave of the block has finished its pixels: a wave that read the
+// record after another wave had already released the slot would skip its
+// pixels, leaving them unwritten and their accumulators uncleared for the
+// slot's next tile.
 __global__ __launch_bounds__(256) void k_stream_resolve(Stream st) {
+  __shared__ SlotInfo s_si;
+  __shared__ int32_t s_live;
   const uint32_t shift = (uint32_t)st.tile_shift;
   for (uint32_t ts = blockIdx.x; ts < (uint32_t)st.n_slots; ts += gridDim.x) {
-    const SlotInfo si = st.slot_info[ts];
-    if (si.tile < 0 || st.live[ts] != 0) continue;  // free, or rays still in flight
+    if (threadIdx.x == 0) {
+      s_si = st.slot_info[ts];
+      s_live = st.live[ts];
+    }
+    __syncthreads();
+    const SlotInfo si = s_si;
+    const bool done = si.tile >= 0 && s_live == 0;  // busy, and no ray of its tile left
+    __syncthreads();  // s_si / s_live are rewritten for the next slot only after every thread read them
+    if (!done) continue;
     const Params& P = scene_params(st.scenes, si.scene);
     const TileDesc td = st.tiles[si.tile];
     float* rgb = st.out_rgb[si.scene];
@@ -613,6 +627,7 @@ __global__ __launch_bounds__(256) void k_stream_resolve(Stream st) {
       }
       for (int k = 0; k < 3 * P.spp; ++k) a[k] = 0ull;
     }
+    __syncthreads();  // every pixel of the tile is folded and cleared
     if (threadIdx.x == 0) {
       st.slot_info[ts].tile = -1;
       const uint32_t f = atomicAdd(&st.ctl->free_count, 1u);

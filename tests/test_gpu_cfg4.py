@@ -113,3 +113,30 @@ def test_cfg4_scenes_common_random_numbers(cfg4_descs):
         rg, vg, st = prt.Scene(d).render(seed=SEED)
         ro, vo, _ = OracleScene(d).render(seed=SEED)
         assert_crn(rg, vg, ro, vo, f"cfg4 scene {i}")
+
+
+def test_cfg4_repeated_tile_queue_renders(cfg4_descs):
+    """Eight renders of the batch (split form, 1024-sample tiles: a tile of an
+    8- or 16-spp scene holds 128 or 64 pixels, so two waves of the resolve
+    kernel fold one tile), each NEE scene's image against its own per-scene
+    render of the same seed.  A resolve kernel that freed a tile's slot while
+    another wave of its block had yet to read the slot record left half such
+    a tile unwritten and its sums in the slot's next tile, in about one render
+    in twenty."""
+    import torch
+    from prender_amd import prt
+    scenes = [prt.Scene(d) for d in cfg4_descs]
+    nee = [i for i, s in enumerate(scenes) if s.desc.sampling.next_event_estimation]
+    outs = [torch.empty((s.height, s.width, 3), dtype=torch.float32, device="cuda") for s in scenes]
+    ref = {i: torch.empty_like(outs[i]) for i in nee}
+    opts = dict(tile_samples=1024, inflight_tiles=4096, stack_lds=8, trav_blocks_per_cu=8, persistent=2)
+    for k in range(8):
+        seed = 200 + k
+        prt.render_batch_device(scenes, [o.data_ptr() for o in outs], seed=seed, **opts)
+        torch.cuda.synchronize()
+        for i in nee:
+            scenes[i].render_device(ref[i].data_ptr(), seed=seed)
+        torch.cuda.synchronize()
+        bad = [i for i in nee if not torch.equal(outs[i], ref[i])]
+        print(f"render {k} seed {seed}: scenes differing {bad}", flush=True)
+        assert not bad
