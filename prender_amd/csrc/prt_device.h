@@ -19,8 +19,10 @@
 #endif
 // a leaf child stores count-1 in bits 27..30 and is negative after ~
 static_assert(PRT_MAX_LEAF >= 1 && PRT_MAX_LEAF <= 16, "leaf size must fit the 4-bit count of a child code");
+// LDS traversal-stack entries of the per-scene kernels (12: five 256-thread
+// blocks of a traversal kernel fit one CU's LDS)
 #ifndef PRT_STACK_LDS
-#define PRT_STACK_LDS 16
+#define PRT_STACK_LDS 12
 #endif
 // the BVH's top levels (at most 1 + 4 + 16 nodes) are numbered first and
 // copied into LDS by every traversal block
