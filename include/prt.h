@@ -350,7 +350,8 @@ typedef struct prt_batch_opts {
     int32_t tile_samples;       /* camera samples per tile, rounded up to a power of two
                                    and to the largest spp (0 = 4096) */
     int32_t inflight_tiles;     /* tile slots (0 = 2^21 samples in flight) */
-    int32_t stack_lds;          /* LDS traversal stack entries: 8, 16 or 32 (0 = 16) */
+    int32_t stack_lds;          /* LDS traversal stack entries: 8, 12, 16 or 32 (0 = 12; 12 runs
+                                   the traversal kernels at 5 waves per SIMD) */
     int32_t trav_blocks_per_cu; /* persistent traversal blocks per CU, <= 16 (0 = 8) */
     int32_t use_graph;          /* >= 0: hipGraph replay; -1: plain launches */
     int32_t tile_stride;        /* render only every k-th tile (tuning trials; 0 = all) */

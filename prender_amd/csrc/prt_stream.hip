@@ -255,9 +255,14 @@ __global__ __launch_bounds__(256) void k_stream_gen(Stream st, RayQ q, int cur) 
   if (lane_id() == 0 && rays) atomicAdd((unsigned long long*)&st.ctl->total_prim, (unsigned long long)rays);
 }
 
+// Waves per SIMD the traversal kernels' registers are budgeted for: the
+// 12-entry LDS stack lets five blocks share a CU, so that instance is built
+// for 5 waves; the others keep the compiler's budget.
+#define PRT_STREAM_WAVES(NL) ((NL) == 12 ? 5 : 1)
+
 // Closest hit for every chunk of queue cur (slab chunks, then camera rays).
 template <int NL>
-__global__ __launch_bounds__(256) void k_stream_extend(Stream st, RayQ q, int cur, double* __restrict__ hit_t,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PRT_STREAM_WAVES(NL)))) void k_stream_extend(Stream st, RayQ q, int cur, double* __restrict__ hit_t,
                                                        int32_t* __restrict__ hit_id) {
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL, 0);
   // the top-level node copy serves a batch of one scene only
@@ -562,7 +567,8 @@ __global__ __launch_bounds__(256) void k_stream_nee(Stream st, RayQ q, int nxt, 
 // Any-hit tests of the shadow queue's chunks; unoccluded rays add their
 // contribution.
 template <int NL>
-__global__ __launch_bounds__(256) void k_stream_shadow(Stream st, ShadowQ sq, int par) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PRT_STREAM_WAVES(NL)))) void k_stream_shadow(
+    Stream st, ShadowQ sq, int par) {
   PRT_DECLARE_STACK_N(S, st.scenes[0], NL, 1);
   PRT_TOP_CACHE(S, st.scenes[0].nodes, st.n_scenes == 1 ? st.scenes[0].n_top : 0);
   const ClaimRange cr =
@@ -653,6 +659,8 @@ hipError_t launch_stream_extend(const Stream& st, const RayQ& q, int cur, double
                                 int stack_lds, int blocks, hipStream_t s) {
   if (stack_lds <= 8)
     hipLaunchKernelGGL(k_stream_extend<8>, dim3(blocks), dim3(256), 0, s, st, q, cur, hit_t, hit_id);
+  else if (stack_lds <= 12)
+    hipLaunchKernelGGL(k_stream_extend<12>, dim3(blocks), dim3(256), 0, s, st, q, cur, hit_t, hit_id);
   else if (stack_lds <= 16)
     hipLaunchKernelGGL(k_stream_extend<16>, dim3(blocks), dim3(256), 0, s, st, q, cur, hit_t, hit_id);
   else
@@ -683,6 +691,8 @@ hipError_t launch_stream_shadow(const Stream& st, const ShadowQ& sq, int par, in
   if (par != 0 && par != 1) return hipErrorInvalidValue;
   if (stack_lds <= 8)
     hipLaunchKernelGGL(k_stream_shadow<8>, dim3(blocks), dim3(256), 0, s, st, sq, par);
+  else if (stack_lds <= 12)
+    hipLaunchKernelGGL(k_stream_shadow<12>, dim3(blocks), dim3(256), 0, s, st, sq, par);
   else if (stack_lds <= 16)
     hipLaunchKernelGGL(k_stream_shadow<16>, dim3(blocks), dim3(256), 0, s, st, sq, par);
   else

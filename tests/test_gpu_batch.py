@@ -83,7 +83,7 @@ def test_tuner_returns_valid_options(batch_env):
     d = best.as_dict()
     print("tuned", d, "ns/sample", ns)
     assert ns > 0
-    assert d["stack_lds"] in (8, 16, 32) and d["trav_blocks_per_cu"] in (4, 8, 16)
+    assert d["stack_lds"] in (8, 12, 16, 32) and d["trav_blocks_per_cu"] in (4, 8, 16)
     assert d["tile_samples"] in (1024, 4096, 16384) and d["inflight_tiles"] >= 1
     assert d["persistent"] in (0, 1, 2) and d["persist_waves"] in (0, 2, 3, 4)
     # the tuned options still render identically
