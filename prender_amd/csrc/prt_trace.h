@@ -17,11 +17,13 @@ struct TravStats {
 
 // Per-lane traversal stack of (node, t) entries: the first NL in LDS (lane
 // interleaved, one 64-bit entry per level, bank-conflict free), deeper ones
-// in a per-thread global area of Params::ovf_words words (enough for the
-// scene's BVH depth).  The overflow accesses are volatile so that the
-// compiler keeps them apart from the LDS ones: merged, both would become
-// flat-address loads and stores, which wait on every outstanding memory
-// operation.
+// in a global overflow area of Params::ovf_words words per thread (enough for
+// the scene's BVH depth), thread interleaved as well: entry k of every
+// thread of a region lies together (k * ovf_threads + thread), so the lanes
+// of a wave that spill at the same depth write 512 contiguous bytes instead
+// of 64 lines.  The overflow accesses are volatile so that the compiler keeps
+// them apart from the LDS ones: merged, both would become flat-address loads
+// and stores, which wait on every outstanding memory operation.
 // four floats as a native vector (copyable out of an address-space-qualified
 // object, unlike float4)
 typedef float prt_f4 __attribute__((ext_vector_type(4)));
@@ -30,7 +32,9 @@ template <int NL>
 struct StackN {
   static constexpr int stride = 256;  // threads per block of every traversal kernel
   uint2* lds;                         // [NL][256]
-  uint2* ovf;                         // 3 * BVH depth + 1 - NL entries used at most
+  uint2* ovf;                         // this thread's first overflow entry
+  size_t ostride;                     // entries between its consecutive overflow entries
+  // (3 * BVH depth + 1 - NL overflow entries used at most)
   // the block's LDS copy of BVH nodes 0..n_top-1 (PRT_TOP_CACHE)
   const __attribute__((address_space(3))) prt_f4* top = nullptr;
   int n_top = 0;
@@ -39,7 +43,7 @@ struct StackN {
     if (sp < NL) {
       lds[sp * stride] = e;
     } else {
-      volatile uint32_t* g = reinterpret_cast<volatile uint32_t*>(ovf + (sp - NL));
+      volatile uint32_t* g = reinterpret_cast<volatile uint32_t*>(ovf + (size_t)(sp - NL) * ostride);
       g[0] = e.x;
       g[1] = e.y;
     }
@@ -49,7 +53,7 @@ struct StackN {
     if (sp < NL) {
       e = lds[sp * stride];
     } else {
-      const volatile uint32_t* g = reinterpret_cast<const volatile uint32_t*>(ovf + (sp - NL));
+      const volatile uint32_t* g = reinterpret_cast<const volatile uint32_t*>(ovf + (size_t)(sp - NL) * ostride);
       e.x = g[0];
       e.y = g[1];
     }
@@ -64,13 +68,13 @@ typedef StackN<PRT_STACK_LDS> Stack;
 // within one of PRT_OVF_REGIONS regions: closest-hit kernels use region 0,
 // shadow kernels region 1, because the two run at the same time on two
 // streams.
-#define PRT_DECLARE_STACK_N(S, P, NL, REGION)                                                                \
-  __shared__ uint2 s_stack_[(NL) * 256];                                                                     \
-  StackN<NL> S{s_stack_ + threadIdx.x,                                                                       \
-               reinterpret_cast<uint2*>(                                                                     \
-                   (P).stack_ovf + ((size_t)(REGION) * (size_t)(P).ovf_threads +                              \
-                                    (size_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % (P).ovf_threads)) * \
-                                       (size_t)(P).ovf_words)}
+#define PRT_DECLARE_STACK_N(S, P, NL, REGION)                                                             \
+  __shared__ uint2 s_stack_[(NL) * 256];                                                                  \
+  StackN<NL> S{s_stack_ + threadIdx.x,                                                                    \
+               reinterpret_cast<uint2*>((P).stack_ovf +                                                   \
+                                        (size_t)(REGION) * (size_t)(P).ovf_threads * (size_t)(P).ovf_words) + \
+                   (size_t)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % (P).ovf_threads),          \
+               (size_t)(P).ovf_threads}
 #define PRT_DECLARE_STACK(S, P, REGION) PRT_DECLARE_STACK_N(S, P, PRT_STACK_LDS, REGION)
 
 // Copies BVH nodes 0..NTOP-1 (the top levels, numbered first by the builder;
