@@ -520,6 +520,12 @@ int run_stream(prt_scene* const* scenes, int n, const BatchConfig& c, float* con
 #ifndef PRT_PERSIST_CLAIM
 #define PRT_PERSIST_CLAIM 4
 #endif
+// 32: the 2-wave persistent kernel keeps 32 stack entries in LDS (two
+// 66-KB blocks per CU; fewer spills: cfg4's persistent half 215 -> 212 ms);
+// 0: the batch's stack_lds decides between 8 and 16
+#ifndef PRT_PERSIST_STACK
+#define PRT_PERSIST_STACK 32
+#endif
 constexpr uint64_t kMaxPersistSamples = (uint64_t)1 << 28;
 
 // One batched render as a single persistent launch (prt_persist.hip), then
@@ -559,7 +565,9 @@ int run_persistent(prt_scene* const* scenes, int n, const BatchConfig& c, float*
   // picks the register budget (2 = the compiler's own, about 230 VGPRs;
   // 3 and 4 spill part of the state), capped by what fits
   const int waves = std::min(4, std::max(2, c.persist_blocks_per_cu));
-  const int stack = c.stack_lds <= 8 ? 8 : 16;
+  // LDS stack of the persistent kernel: PRT_PERSIST_STACK when set, else from
+  // the batch's stack_lds (8, or 16 for 12..32)
+  const int stack = PRT_PERSIST_STACK == 32 && waves <= 2 ? 32 : (c.stack_lds <= 8 ? 8 : 16);
   int occ = 0;
   HIPCHK(persist_blocks_per_cu(stack, waves, &occ));
   const int cu = std::max(1, lead->cu_count);

@@ -269,9 +269,14 @@ static hipError_t with_persist_kernel(int stack_lds, int waves, F&& f) {
     if (waves == 3) return f(k_persist<8, 3>);
     return f(k_persist<8, 4>);
   }
-  if (waves <= 2) return f(k_persist<16, 2>);
-  if (waves == 3) return f(k_persist<16, 3>);
-  return f(k_persist<16, 4>);
+  if (stack_lds <= 16) {
+    if (waves <= 2) return f(k_persist<16, 2>);
+    if (waves == 3) return f(k_persist<16, 3>);
+    return f(k_persist<16, 4>);
+  }
+  // 32 entries (64 KB of LDS per block) only at the 2-wave budget, where two
+  // blocks per CU fit
+  return f(k_persist<32, 2>);
 }
 
 hipError_t launch_persist(const Persist& p, int stack_lds, int waves, int blocks, hipStream_t s) {

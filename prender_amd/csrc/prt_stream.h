@@ -169,8 +169,8 @@ struct Persist {
   int32_t pad_;
 };
 
-// stack_lds: 8 or 16 LDS stack entries (32 does not fit beside the kernel's
-// registers); waves: waves per SIMD its registers are budgeted for (2..4).
+// stack_lds: 8, 16 or (at 2 waves, where two 66-KB blocks fit a CU) 32 LDS
+// stack entries; waves: waves per SIMD its registers are budgeted for (2..4).
 hipError_t launch_persist(const Persist& p, int stack_lds, int waves, int blocks, hipStream_t s);
 // Resident 256-thread blocks per CU of that kernel instance.
 hipError_t persist_blocks_per_cu(int stack_lds, int waves, int* blocks);
