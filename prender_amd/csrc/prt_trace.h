@@ -97,8 +97,22 @@ __device__ __forceinline__ float fup(double t) {
   return (double)f < t ? __uint_as_float(__float_as_uint(f) + 1u) : f;
 }
 
+// Byte offsets within a 128-byte node record of the row holding each axis'
+// entry planes, packed x | y << 8 | z << 16: the lo row (x 0, y 16, z 32)
+// where the ray's inverse direction on that axis is >= 0, else the hi row
+// (x 48, y 64, z 80).  The exit planes are in the other row, at the entry
+// offset XOR 48, 80 or 112.  ix, iy, iz are never zero (|1 / d| >= 1 for a
+// unit direction, and an infinite one is clamped to +-1e30).
+__device__ __forceinline__ uint32_t entry_rows(float ix, float iy, float iz) {
+  return (ix < 0.0f ? 48u : 0u) | ((iy < 0.0f ? 64u : 16u) << 8) | ((iz < 0.0f ? 80u : 32u) << 16);
+}
+
 // Slab test of the 4 children of node `cur` against the ray's float slabs:
 // entry distances sorted ascending with their child references (+inf: miss).
+// Each axis reads its entry and exit planes from the rows entry_rows picks:
+// fma(c, i, -oi) is monotonic in the plane coordinate c and every child has
+// lo <= hi (an empty slot lo = hi = +inf), so the entry plane's distance is
+// exactly the smaller of the two, without a min / max per axis.
 // The sort is branch-free: each distance (>= 0, sign cleared so -0 sorts as
 // 0) becomes a 32-bit key whose bit pattern orders like the float, with the
 // child slot in its two low bits, and a min/max network sorts the keys.  The
@@ -106,35 +120,41 @@ __device__ __forceinline__ float fup(double t) {
 // against it stays conservative; +inf stays +inf.
 // Nodes of the top levels come from the block's LDS copy.
 template <class Stk>
-__device__ __forceinline__ void node_children(const Stk& S, const DevNode* nodes, int32_t cur, float ix, float iy,
-                                              float iz, float oix, float oiy, float oiz, float tb, float tn[4],
-                                              int32_t cc[4]) {
-  float4 nd[7];
+__device__ __forceinline__ void node_children(const Stk& S, const DevNode* nodes, int32_t cur, uint32_t rows,
+                                              float ix, float iy, float iz, float oix, float oiy, float oiz, float tb,
+                                              float tn[4], int32_t cc[4]) {
+  const uint32_t nx = rows & 0xFFu, ny = (rows >> 8) & 0xFFu, nz = rows >> 16;
+  const uint32_t fx = nx ^ 48u, fy = ny ^ 80u, fz = nz ^ 112u;
+  float4 nd[7];  // entry x, y, z rows, exit x, y, z rows, children
   if (cur < S.n_top) {
-    const __attribute__((address_space(3))) prt_f4* t = S.top + (size_t)cur * 8;
+    typedef const __attribute__((address_space(3))) char lds_c;
+    typedef const __attribute__((address_space(3))) prt_f4 lds_f4;
+    lds_c* t = reinterpret_cast<lds_c*>(S.top + (size_t)cur * 8);
+    const uint32_t off[7] = {nx, ny, nz, fx, fy, fz, 96u};
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-      const prt_f4 v = t[k];
+      const prt_f4 v = *reinterpret_cast<lds_f4*>(t + off[k]);
       nd[k] = make_float4(v.x, v.y, v.z, v.w);
     }
   } else {
-    const float4* g = reinterpret_cast<const float4*>(nodes + cur);
+    const char* g = reinterpret_cast<const char*>(nodes + cur);
+    const uint32_t off[7] = {nx, ny, nz, fx, fy, fz, 96u};
 #pragma unroll
-    for (int k = 0; k < 7; ++k) nd[k] = g[k];
+    for (int k = 0; k < 7; ++k) nd[k] = *reinterpret_cast<const float4*>(g + off[k]);
   }
-  const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+  const float4 ex = nd[0], ey = nd[1], ez = nd[2], xx = nd[3], xy = nd[4], xz = nd[5];
   const int4 ch = make_int4(__float_as_int(nd[6].x), __float_as_int(nd[6].y), __float_as_int(nd[6].z),
                             __float_as_int(nd[6].w));
-  const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, lya[4] = {ly.x, ly.y, ly.z, ly.w}, lza[4] = {lz.x, lz.y, lz.z, lz.w};
-  const float hxa[4] = {hx.x, hx.y, hx.z, hx.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+  const float exa[4] = {ex.x, ex.y, ex.z, ex.w}, eya[4] = {ey.x, ey.y, ey.z, ey.w}, eza[4] = {ez.x, ez.y, ez.z, ez.w};
+  const float xxa[4] = {xx.x, xx.y, xx.z, xx.w}, xya[4] = {xy.x, xy.y, xy.z, xy.w}, xza[4] = {xz.x, xz.y, xz.z, xz.w};
   uint32_t key[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const float x0 = __builtin_fmaf(lxa[k], ix, -oix), x1 = __builtin_fmaf(hxa[k], ix, -oix);
-    const float y0 = __builtin_fmaf(lya[k], iy, -oiy), y1 = __builtin_fmaf(hya[k], iy, -oiy);
-    const float z0 = __builtin_fmaf(lza[k], iz, -oiz), z1 = __builtin_fmaf(hza[k], iz, -oiz);
-    const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-    const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tb));
+    const float x0 = __builtin_fmaf(exa[k], ix, -oix), x1 = __builtin_fmaf(xxa[k], ix, -oix);
+    const float y0 = __builtin_fmaf(eya[k], iy, -oiy), y1 = __builtin_fmaf(xya[k], iy, -oiy);
+    const float z0 = __builtin_fmaf(eza[k], iz, -oiz), z1 = __builtin_fmaf(xza[k], iz, -oiz);
+    const float tnear = fmaxf(fmaxf(x0, y0), fmaxf(z0, 0.0f));
+    const float tfar = fminf(fminf(x1, y1), fminf(z1, tb));
     const uint32_t tk = (tnear <= tfar) ? (__float_as_uint(tnear) & 0x7FFFFFFCu) : 0x7F800000u;
     key[k] = tk | (uint32_t)k;
   }
@@ -201,6 +221,7 @@ __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double&
   if (__builtin_isinf(iy)) iy = __builtin_copysignf(1.0e30f, iy);
   if (__builtin_isinf(iz)) iz = __builtin_copysignf(1.0e30f, iz);
   const float oix = ox * ix, oiy = oy * iy, oiz = oz * iz;
+  const uint32_t rows = entry_rows(ix, iy, iz);
   float tb = t_best < 1.0e30 ? fup(t_best) : __builtin_inff();
   bool found = false;
   int sp = 0;
@@ -223,7 +244,7 @@ __device__ __forceinline__ bool traverse_ww(const Params& P, d3 o, d3 d, double&
     while ((uint32_t)cur < (uint32_t)PRT_TRAV_DONE) {
       float tn[4];
       int32_t cc[4];
-      node_children(S, P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
+      node_children(S, P.nodes, cur, rows, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
       ts.nodes++;
       push_children(S, sp, tn, cc);
       cur = (tn[0] != __builtin_inff()) ? cc[0] : pop_next();
@@ -390,6 +411,7 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
   const ClaimRange cr = claim_range(fetch, n_chunks);
   d3 o = mk(0.0, 0.0, 0.0), d = o;
   float ix = 0.0f, iy = 0.0f, iz = 0.0f, oix = 0.0f, oiy = 0.0f, oiz = 0.0f, tb = 0.0f;
+  uint32_t rows = 0;
   double t_best = 0.0;
   int32_t id_best = -1, e_best = 0x7FFFFFFF, cur = PRT_TRAV_DONE, lf = 0;
   int sp = 0;
@@ -473,6 +495,7 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
             oix = (float)o.x * ix;
             oiy = (float)o.y * iy;
             oiz = (float)o.z * iz;
+            rows = entry_rows(ix, iy, iz);
             tb = t_best < 1.0e30 ? fup(t_best) : __builtin_inff();
             cur = 0;
           }
@@ -487,7 +510,7 @@ __device__ __forceinline__ void traverse_pool(const Params& P, uint32_t n_chunks
     while ((uint32_t)cur < (uint32_t)PRT_TRAV_DONE) {
       float tn[4];
       int32_t cc[4];
-      node_children(S, P.nodes, cur, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
+      node_children(S, P.nodes, cur, rows, ix, iy, iz, oix, oiy, oiz, tb, tn, cc);
       ts.nodes++;
       push_children(S, sp, tn, cc);
       cur = (tn[0] != __builtin_inff()) ? cc[0] : pop_next();
