@@ -174,9 +174,10 @@ def main():
             "ms_per_step": sum_("ms_traversal") / args.steps,
             "achieved": (sum_("bytes_extend") + sum_("bytes_shadow")) / max(1e-12, sum_("ms_traversal") * 1e-3) / 1e9,
             "unit": "GB/s", "peak": HBM_PEAK_GBS,
-            "note": "k_shadow runs on a second stream alongside the next depth's k_extend, so each kernel's "
-                    "own launch time includes the time it shares the chip; this is both kernels' algorithmic "
-                    "bytes over the time during which either runs"},
+            "note": ("the shadow kernel runs on a second stream (depth-synchronous engine: beside the next "
+                     "depth's shade and NEE; tile queue: beside the next iteration's closest-hit kernel), so "
+                     "its launch time includes the time it shares the chip; this is both traversal kernels' "
+                     "algorithmic bytes over the time during which either runs")},
         "device": info["arch"],
         "scene_build_s": build_s,
     }
