@@ -520,11 +520,15 @@ __device__ __forceinline__ double luminance(double r, double g, double b) {
 
 // Folds the spp samples of one pixel (a[3*k..3*k+2], fixed point): v_k = L_k *
 // smp, luminance clamp (render.h:1375-1384), image += v_k, *= sensor_response
-// (render.h:1405-1409), and the variance of the pixel mean.
+// (render.h:1405-1409), and the variance of the pixel mean.  The sums run in
+// sample order; the accumulators are read 8 samples at a time, so that a
+// pixel's thread waits for one memory round trip per 8 samples instead of one
+// per sample.
 __device__ __forceinline__ void fold_pixel(const Params& P, const uint64_t* a, double rgb[3], double var[3]) {
   double s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
-  for (int k = 0; k < P.spp; ++k) {
-    double v0 = accum_get(a + 3 * k) * P.smp, v1 = accum_get(a + 3 * k + 1) * P.smp, v2 = accum_get(a + 3 * k + 2) * P.smp;
+  auto add = [&](uint64_t w0, uint64_t w1, uint64_t w2) {
+    double v0 = ((double)w0 * PRT_FIX_INV) * P.smp, v1 = ((double)w1 * PRT_FIX_INV) * P.smp,
+           v2 = ((double)w2 * PRT_FIX_INV) * P.smp;
     if (P.cutoff > 0.0f) {
       const float lv = (float)luminance(v0, v1, v2);
       if (lv > P.cutoff) {
@@ -534,7 +538,17 @@ __device__ __forceinline__ void fold_pixel(const Params& P, const uint64_t* a, d
     }
     s1[0] += v0; s1[1] += v1; s1[2] += v2;
     s2[0] += v0 * v0; s2[1] += v1 * v1; s2[2] += v2 * v2;
+  };
+  constexpr int kB = 8;
+  int k = 0;
+  for (; k + kB <= P.spp; k += kB) {
+    uint64_t w[3 * kB];
+#pragma unroll
+    for (int j = 0; j < 3 * kB; ++j) w[j] = a[3 * k + j];
+#pragma unroll
+    for (int j = 0; j < kB; ++j) add(w[3 * j], w[3 * j + 1], w[3 * j + 2]);
   }
+  for (; k < P.spp; ++k) add(a[3 * k], a[3 * k + 1], a[3 * k + 2]);
   const double sr = P.sensor_response;
   const double nn = (double)P.spp;
   for (int c = 0; c < 3; ++c) {
