@@ -556,3 +556,54 @@ __device__ __forceinline__ void fold_pixel(const Params& P, const uint64_t* a, d
     var[c] = nn > 1 ? fmax(0.0, (nn * s2[c] - s1[c] * s1[c]) / (nn - 1)) * sr * sr : 0.0;
   }
 }
+
+// One channel of fold_pixel: the same values in the same order for channel
+// ch (0..2), so that three threads can fold one pixel.  With a luminance
+// cutoff every channel's clamp needs all three, otherwise only channel ch
+// is read.
+__device__ __forceinline__ void fold_channel(const Params& P, const uint64_t* a, int ch, double& rgb, double& var) {
+  double s1 = 0, s2 = 0;
+  constexpr int kB = 16;
+  int k = 0;
+  if (P.cutoff > 0.0f) {
+    auto add = [&](uint64_t w0, uint64_t w1, uint64_t w2) {
+      double v0 = ((double)w0 * PRT_FIX_INV) * P.smp, v1 = ((double)w1 * PRT_FIX_INV) * P.smp,
+             v2 = ((double)w2 * PRT_FIX_INV) * P.smp;
+      const float lv = (float)luminance(v0, v1, v2);
+      if (lv > P.cutoff) {
+        const float f = P.cutoff / lv;
+        v0 *= f; v1 *= f; v2 *= f;
+      }
+      const double v = ch == 0 ? v0 : (ch == 1 ? v1 : v2);
+      s1 += v;
+      s2 += v * v;
+    };
+    for (; k + kB / 2 <= P.spp; k += kB / 2) {
+      uint64_t w[3 * (kB / 2)];
+#pragma unroll
+      for (int j = 0; j < 3 * (kB / 2); ++j) w[j] = a[3 * k + j];
+#pragma unroll
+      for (int j = 0; j < kB / 2; ++j) add(w[3 * j], w[3 * j + 1], w[3 * j + 2]);
+    }
+    for (; k < P.spp; ++k) add(a[3 * k], a[3 * k + 1], a[3 * k + 2]);
+  } else {
+    auto add = [&](uint64_t w) {
+      const double v = ((double)w * PRT_FIX_INV) * P.smp;
+      s1 += v;
+      s2 += v * v;
+    };
+    const uint64_t* ac = a + ch;
+    for (; k + kB <= P.spp; k += kB) {
+      uint64_t w[kB];
+#pragma unroll
+      for (int j = 0; j < kB; ++j) w[j] = ac[3 * (k + j)];
+#pragma unroll
+      for (int j = 0; j < kB; ++j) add(w[j]);
+    }
+    for (; k < P.spp; ++k) add(ac[3 * k]);
+  }
+  const double sr = P.sensor_response;
+  const double nn = (double)P.spp;
+  rgb = s1 * sr;
+  var = nn > 1 ? fmax(0.0, (nn * s2 - s1 * s1) / (nn - 1)) * sr * sr : 0.0;
+}

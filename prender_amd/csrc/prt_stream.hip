@@ -620,20 +620,21 @@ __global__ __launch_bounds__(256) void k_stream_resolve(Stream st) {
     const TileDesc td = st.tiles[si.tile];
     float* rgb = st.out_rgb[si.scene];
     float* var = st.out_var[si.scene];
-    for (int p = threadIdx.x; p < td.n_pixels; p += blockDim.x) {
+    // three threads per pixel, one per channel
+    uint64_t* tile_acc = st.accum + 3 * ((size_t)ts << shift);
+    for (int j = threadIdx.x; j < 3 * td.n_pixels; j += blockDim.x) {
+      const int p = j / 3, ch = j - 3 * p;
       const int64_t pix = td.pixel_base + p;
       const int x = (int)(pix % P.W), y = (int)(pix / P.W);
       const size_t ii = (size_t)(P.H - y - 1) * P.W + x;  // render.h:1234
-      uint64_t* a = st.accum + 3 * (((size_t)ts << shift) + (size_t)p * P.spp);
-      double c[3], v[3];
-      fold_pixel(P, a, c, v);
-      for (int k = 0; k < 3; ++k) {
-        rgb[3 * ii + k] = (float)c[k];
-        if (var) var[3 * ii + k] = (float)v[k];
-      }
-      for (int k = 0; k < 3 * P.spp; ++k) a[k] = 0ull;
+      double c, v;
+      fold_channel(P, tile_acc + 3 * (size_t)p * P.spp, ch, c, v);
+      rgb[3 * ii + ch] = (float)c;
+      if (var) var[3 * ii + ch] = (float)v;
     }
-    __syncthreads();  // every pixel of the tile is folded and cleared
+    __syncthreads();  // every pixel of the tile is folded
+    for (int j = threadIdx.x; j < 3 * td.n_pixels * P.spp; j += blockDim.x) tile_acc[j] = 0ull;
+    __syncthreads();  // and cleared
     if (threadIdx.x == 0) {
       st.slot_info[ts].tile = -1;
       const uint32_t f = atomicAdd(&st.ctl->free_count, 1u);
