@@ -398,7 +398,6 @@ def bench_batch(args, torch, G, prt, info):
         progress("batch tuner")
         opts, tune_ns = prt.tune_batch(scenes, ptrs)
     tune_s = time.time() - t0
-    opts.use_graph = 0
     for i in range(args.warmup):
         opts.seed = 1000 + i
         prt.render_batch_device(scenes, ptrs, opts=opts)

@@ -342,8 +342,8 @@ int prt_render_erpt_device(prt_scene* scene, const prt_erpt_opts* opts, float* d
  * (main.cpp:173-200).  All scenes of a batch go through one shared work queue
  * of pixel tiles (a tile = consecutive pixels of one scene with all their
  * samples); a fixed pool of tile slots keeps rays of many tiles, of whatever
- * scenes, in flight, and two iterations of the wavefront are replayed as one
- * captured hipGraph until the queue is drained.  Each scene's image is
+ * scenes, in flight, and the wavefront iterates, two iterations per host
+ * step, until the queue is drained.  Each scene's image is
  * bit-identical to prt_render_device of that scene with the same seed. */
 typedef struct prt_batch_opts {
     uint64_t seed;
@@ -353,7 +353,8 @@ typedef struct prt_batch_opts {
     int32_t stack_lds;          /* LDS traversal stack entries: 8, 12, 16 or 32 (0 = 12; 12 runs
                                    the traversal kernels at 5 waves per SIMD) */
     int32_t trav_blocks_per_cu; /* persistent traversal blocks per CU, <= 16 (0 = 8) */
-    int32_t use_graph;          /* >= 0: hipGraph replay; -1: plain launches */
+    int32_t use_graph;          /* 1: the iterations replayed as a captured hipGraph; 0 (or -1):
+                                   plain launches, the faster form measured on MI355X */
     int32_t tile_stride;        /* render only every k-th tile (tuning trials; 0 = all) */
     int32_t max_depth;          /* safety cap on path vertices (0 = 128) */
     int32_t timing;             /* 1 = per-kernel device timing into prt_stats (plain launches only) */

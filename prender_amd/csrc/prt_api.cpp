@@ -1010,7 +1010,7 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
     b.seed = o ? o->seed : 0;
     b.max_depth = o ? o->max_depth : 0;
     b.timing = o ? o->timing : 0;
-    b.use_graph = -1;            // plain launches (timed renders need them)
+    b.use_graph = -1;            // plain launches (faster than graph replay; timed renders need them)
     // samples in flight: 2^22 for large frames (cfg3's 50 M: 398 against 352
     // M samples/s at 2^21), a whole frame of up to 2^21, else 2^20.  A scene
     // keeps its workspace (about 3 KB per sample in flight with glass) for

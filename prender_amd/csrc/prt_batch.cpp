@@ -4,9 +4,10 @@
 //
 // The device side is prt_stream.hip; every iteration is refill, gen, extend,
 // shade, nee, shadow, resolve on the lead scene's stream (nee and shadow on
-// its second stream when PRT_BATCH_OVERLAP is set), two iterations per
-// captured hipGraph.  The host only replays the graph and, once per group of
-// replays, reads the control block to learn whether the work queue is drained.
+// its second stream when PRT_BATCH_OVERLAP is set), launched two iterations
+// at a time directly or (use_graph = 1) as one captured hipGraph.  Once per
+// group of four such pairs the host reads the control block to learn whether
+// the work queue is drained.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -83,7 +84,7 @@ constexpr int64_t kDefaultInflightSamples = (int64_t)1 << 21;
 constexpr int kDefaultStackLds = PRT_BATCH_STACK_LDS;
 constexpr int kDefaultTravBlocksPerCu = 8;
 constexpr int kMaxTravBlocksPerCu = 16;
-constexpr int kReplaysPerGroup = 4;  // graph replays (2 iterations each) between status reads
+constexpr int kReplaysPerGroup = 4;  // pairs of iterations (or graph replays) between status reads
 constexpr int64_t kMaxIterations = 2000000;
 
 struct BatchConfig {
@@ -91,7 +92,7 @@ struct BatchConfig {
   int n_slots = 512;
   int stack_lds = kDefaultStackLds;
   int trav_blocks_per_cu = kDefaultTravBlocksPerCu;
-  bool use_graph = true;
+  bool use_graph = false;
   int tile_stride = 1;
   int max_vertices = 128;
   bool timing = false;
@@ -124,7 +125,7 @@ int resolve_config(prt_scene* const* scenes, int n, const prt_batch_opts* o, Bat
     return fail(PRT_E_INVALID, "stack_lds must be 8, 12, 16 or 32");
   c.trav_blocks_per_cu = opts.trav_blocks_per_cu > 0 ? opts.trav_blocks_per_cu : kDefaultTravBlocksPerCu;
   if (c.trav_blocks_per_cu > kMaxTravBlocksPerCu) return fail(PRT_E_INVALID, "trav_blocks_per_cu above 16");
-  c.use_graph = opts.use_graph >= 0;  // -1 = plain launches
+  c.use_graph = opts.use_graph > 0;  // 0 / -1 = plain launches
   c.timing = opts.timing != 0 && !c.use_graph;
   c.tile_stride = std::max(1, opts.tile_stride);
   c.max_vertices = opts.max_depth > 0 ? std::min(opts.max_depth, 255) : 128;
@@ -850,6 +851,13 @@ int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts*
     if (v == cur.trav_blocks_per_cu) continue;
     prt_batch_opts o = cur;
     o.trav_blocks_per_cu = v;
+    double ns;
+    if ((rc = trial(o, inflight, ns))) return rc;
+    if (ns < best_ns) { best_ns = ns; cur = o; }
+  }
+  {  // the other launch form: graph replay or plain launches
+    prt_batch_opts o = cur;
+    o.use_graph = cur.use_graph > 0 ? 0 : 1;
     double ns;
     if ((rc = trial(o, inflight, ns))) return rc;
     if (ns < best_ns) { best_ns = ns; cur = o; }

@@ -332,7 +332,7 @@ class Scene:
 
 # ---- batched scenes
 def batch_opts(seed: int = 0, tile_samples: int = 0, inflight_tiles: int = 0, stack_lds: int = 0,
-               trav_blocks_per_cu: int = 0, use_graph: bool = True, tile_stride: int = 0,
+               trav_blocks_per_cu: int = 0, use_graph: bool = False, tile_stride: int = 0,
                max_depth: int = 0, timing: bool = False, persistent: int = 0,
                persist_waves: int = 0) -> PrtBatchOpts:
     o = PrtBatchOpts()
@@ -341,7 +341,7 @@ def batch_opts(seed: int = 0, tile_samples: int = 0, inflight_tiles: int = 0, st
     o.inflight_tiles = int(inflight_tiles)
     o.stack_lds = int(stack_lds)
     o.trav_blocks_per_cu = int(trav_blocks_per_cu)
-    o.use_graph = 0 if use_graph else -1
+    o.use_graph = 1 if use_graph else 0
     o.tile_stride = int(tile_stride)
     o.max_depth = int(max_depth)
     o.timing = 1 if timing else 0

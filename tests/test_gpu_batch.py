@@ -39,7 +39,7 @@ def _run_batch(torch, prt, scenes, **kw):
 
 @pytest.mark.parametrize("kw", [
     {},
-    {"use_graph": False},
+    {"use_graph": True},
     {"tile_samples": 256, "inflight_tiles": 7},
     {"tile_samples": 1024, "inflight_tiles": 3, "stack_lds": 8, "trav_blocks_per_cu": 4},
     {"tile_samples": 16384, "stack_lds": 32, "trav_blocks_per_cu": 16},
