@@ -349,7 +349,9 @@ typedef struct prt_batch_opts {
     uint64_t seed;
     int32_t tile_samples;       /* camera samples per tile, rounded up to a power of two
                                    and to the largest spp (0 = 4096) */
-    int32_t inflight_tiles;     /* tile slots (0 = 2^21 samples in flight) */
+    int32_t inflight_tiles;     /* tile slots (0 = automatic: the most samples in flight, up to 2^24
+                                   and up to the batch's camera samples, whose workspace fits 3/8
+                                   of the device memory; more in flight is faster) */
     int32_t stack_lds;          /* LDS traversal stack entries: 8, 12, 16 or 32 (0 = 12; 12 runs
                                    the traversal kernels at 5 waves per SIMD) */
     int32_t trav_blocks_per_cu; /* persistent traversal blocks per CU, <= 16 (0 = 8) */
@@ -379,15 +381,15 @@ typedef struct prt_batch_opts {
 int prt_render_batch_device(prt_scene* const* scenes, int n_scenes,
                             const prt_batch_opts* opts, float* const* d_rgb,
                             float* const* d_var, prt_stats* stats);
-/* Measures candidate in-flight sizes (2^20..2^22 samples), tile sizes
- * (1024..16384 samples), LDS stack depths and traversal grids by coordinate
+/* Measures candidate tile sizes (1024..16384 samples), LDS stack depths,
+ * traversal grids and the launch form (plain or graph replay) by coordinate
  * descent on a subsample of the batch's tiles (every k-th tile, k from
  * base->tile_stride or chosen for ~16M samples per trial), then the
  * persistent engine and the split form, each at 2, 3 and 4 waves per SIMD, on
  * the whole batch (when it fits and base->tile_stride asks for no
  * subsample), and writes the fastest
- * options to *best (tile_stride copied from base).  The trial renders write
- * the output buffers. */
+ * options to *best (tile_stride copied from base; inflight_tiles from base,
+ * 0 = the automatic size).  The trial renders write the output buffers. */
 int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts* base,
                    float* const* d_rgb, float* const* d_var, prt_batch_opts* best,
                    double* best_ns_per_sample);

@@ -122,8 +122,21 @@ void Workspace::release() {
   batch_samples = queue_cap = shadow_cap = 0;
 }
 
+static const size_t kRayColBytes[RAYQ_COLS] = {8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+static const size_t kShadowColBytes[SHADOWQ_COLS] = {8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4};
+size_t ray_entry_bytes() {
+  size_t b = 0;
+  for (size_t v : kRayColBytes) b += v;
+  return b;
+}
+size_t shadow_entry_bytes() {
+  size_t b = 0;
+  for (size_t v : kShadowColBytes) b += v;
+  return b;
+}
+
 int alloc_ray_queue(DevBuf* cols, int64_t cap) {
-  static const size_t dsz[RAYQ_COLS] = {8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4};
+  const size_t* dsz = kRayColBytes;
   for (int k = 0; k < RAYQ_COLS; ++k) {
     const int rc = dev_alloc(cols[k], (size_t)cap * dsz[k]);
     if (rc) return rc;
@@ -132,7 +145,7 @@ int alloc_ray_queue(DevBuf* cols, int64_t cap) {
 }
 
 int alloc_shadow_queue(DevBuf* cols, int64_t cap) {
-  static const size_t ssz[SHADOWQ_COLS] = {8, 8, 8, 8, 8, 8, 8, 4, 4, 4, 4, 4};
+  const size_t* ssz = kShadowColBytes;
   for (int k = 0; k < SHADOWQ_COLS; ++k) {
     const int rc = dev_alloc(cols[k], (size_t)cap * ssz[k]);
     if (rc) return rc;
@@ -1011,15 +1024,12 @@ static int render_device(prt_scene* s, const prt_render_opts* o, float* d_rgb, f
     b.max_depth = o ? o->max_depth : 0;
     b.timing = o ? o->timing : 0;
     b.use_graph = -1;            // plain launches (faster than graph replay; timed renders need them)
-    // samples in flight: 2^22 for large frames (cfg3's 50 M: 398 against 352
-    // M samples/s at 2^21), a whole frame of up to 2^21, else 2^20.  A scene
-    // keeps its workspace (about 3 KB per sample in flight with glass) for
-    // later calls, and a caller may hold many scenes.
+    // samples in flight: a whole frame of up to 2^21, else the batched
+    // renderer's automatic size (the most, up to 2^24, whose workspace fits
+    // its share of the device memory)
     const int64_t frame = (int64_t)s->P.W * s->P.H * s->P.spp;
-    const int64_t inflight = frame >= ((int64_t)1 << 25) ? ((int64_t)1 << 22)
-                           : frame <= ((int64_t)1 << 21) ? frame : ((int64_t)1 << 20);
     b.tile_samples = 4096;
-    b.inflight_tiles = (int32_t)std::max<int64_t>(1, (inflight + 4095) / 4096);
+    b.inflight_tiles = frame <= ((int64_t)1 << 21) ? (int32_t)std::max<int64_t>(1, (frame + 4095) / 4096) : 0;
     prt_scene* const one[1] = {s};
     float* const rgb[1] = {d_rgb};
     float* const var[1] = {d_var};

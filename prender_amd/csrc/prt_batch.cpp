@@ -75,7 +75,12 @@ namespace {
 
 // Defaults used where prt_batch_opts leaves a field at 0.
 constexpr int kDefaultTileSamples = 4096;
-constexpr int64_t kDefaultInflightSamples = (int64_t)1 << 21;
+// Automatic samples in flight (inflight_tiles = 0): the most, up to 2^24 and
+// up to the batch's camera samples, whose tile-queue workspace fits 3/8 of
+// the device memory.  More keep the traversal kernels' grids full for longer
+// and need fewer iterations: cfg3 549, 630 and 682 M samples/s at 2^22, 2^23
+// and 2^24 (about 97 GB of workspace with glass), cfg4 351, 363, 369.
+constexpr int64_t kMaxAutoInflight = (int64_t)1 << 24;
 // 12 LDS stack entries by default: that instance of the traversal kernels
 // runs 5 waves per SIMD (cfg3 478 -> 504 M samples/s against 16)
 #ifndef PRT_BATCH_STACK_LDS
@@ -107,6 +112,8 @@ int ceil_log2(int64_t v) {
   return s;
 }
 
+int64_t auto_inflight(prt_scene* const* scenes, int n, const BatchConfig& c);
+
 int resolve_config(prt_scene* const* scenes, int n, const prt_batch_opts* o, BatchConfig& c) {
   prt_batch_opts opts{};
   if (o) opts = *o;
@@ -117,8 +124,7 @@ int resolve_config(prt_scene* const* scenes, int n, const prt_batch_opts* o, Bat
   c.tile_shift = std::max(std::max(ceil_log2(ts), ceil_log2(max_spp)), 6);
   if (c.tile_shift > 24) return fail(PRT_E_INVALID, "tile_samples above 2^24");
   const int64_t T = (int64_t)1 << c.tile_shift;
-  c.n_slots = opts.inflight_tiles > 0 ? opts.inflight_tiles
-                                      : (int)std::max<int64_t>(1, kDefaultInflightSamples / T);
+  c.n_slots = opts.inflight_tiles > 0 ? opts.inflight_tiles : (int)std::max<int64_t>(1, auto_inflight(scenes, n, c) / T);
   if ((int64_t)c.n_slots * T > ((int64_t)1 << 28)) return fail(PRT_E_INVALID, "more than 2^28 samples in flight");
   c.stack_lds = opts.stack_lds > 0 ? opts.stack_lds : kDefaultStackLds;
   if (c.stack_lds != 8 && c.stack_lds != 12 && c.stack_lds != 16 && c.stack_lds != 32)
@@ -171,6 +177,36 @@ StreamGeometry stream_geometry(const BatchConfig& c, bool glass) {
   g.max_desc = W * (g.slab_ray / 64 + g.max_runs) + g.prim_cap / 64;
   g.max_sh_desc = W * (g.slab_sh / 64 + g.max_runs);
   return g;
+}
+
+// Device bytes of the tile queue's workspace that grow with the samples in
+// flight: two ray queues and the hits, the shadow queue, the accumulators
+// and the chunk lists.
+int64_t stream_workspace_bytes(const BatchConfig& c, bool glass) {
+  const StreamGeometry g = stream_geometry(c, glass);
+  const int64_t inflight = (int64_t)c.n_slots << c.tile_shift;
+  return g.q_cap * (int64_t)(2 * ray_entry_bytes() + 12) + g.sq_cap * (int64_t)shadow_entry_bytes() +
+         inflight * 24 + (2 * g.max_desc + g.max_sh_desc) * (int64_t)sizeof(ChunkDesc);
+}
+
+int64_t auto_inflight(prt_scene* const* scenes, int n, const BatchConfig& c) {
+  uint64_t total = 0;
+  bool glass = false;
+  for (int i = 0; i < n; ++i) {
+    total += (uint64_t)scenes[i]->P.W * scenes[i]->P.H * scenes[i]->P.spp;
+    glass |= scenes[i]->has_glass;
+  }
+  size_t dev_total = 0;
+  if (hipDeviceTotalMem(&dev_total, scenes[0]->device) != hipSuccess) dev_total = 0;
+  const int64_t budget = dev_total ? (int64_t)(dev_total / 8 * 3) : ((int64_t)1 << 33);
+  const int64_t T = (int64_t)1 << c.tile_shift;
+  for (int64_t v = kMaxAutoInflight; v > T; v >>= 1) {
+    if ((uint64_t)v > total) continue;
+    BatchConfig t = c;
+    t.n_slots = (int)(v / T);
+    if (stream_workspace_bytes(t, glass) <= budget) return v;
+  }
+  return T;
 }
 
 int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, const StreamGeometry& g, int n_scenes,
@@ -803,12 +839,11 @@ int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts*
   if (cur.tile_samples <= 0) cur.tile_samples = kDefaultTileSamples;
   if (cur.stack_lds <= 0) cur.stack_lds = kDefaultStackLds;
   if (cur.trav_blocks_per_cu <= 0) cur.trav_blocks_per_cu = kDefaultTravBlocksPerCu;
-  auto inflight_of = [](const prt_batch_opts& o) {
-    return o.inflight_tiles > 0 ? (int64_t)o.inflight_tiles * o.tile_samples : kDefaultInflightSamples;
-  };
-  int64_t inflight = inflight_of(cur);
+  // samples in flight: the caller's, else the automatic size (the most that
+  // fits the workspace budget, which measured fastest at every size tried)
+  const int64_t inflight = cur.inflight_tiles > 0 ? (int64_t)cur.inflight_tiles * cur.tile_samples : 0;
   auto trial = [&](prt_batch_opts o, int64_t infl, double& ns) -> int {
-    o.inflight_tiles = (int32_t)std::max<int64_t>(1, infl / o.tile_samples);
+    o.inflight_tiles = infl > 0 ? (int32_t)std::max<int64_t>(1, infl / o.tile_samples) : 0;
     BatchConfig c;
     int r = resolve_config(scenes, n_scenes, &o, c);
     if (r) return r;
@@ -821,16 +856,9 @@ int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts*
   double best_ns = 0;
   if ((rc = trial(cur, inflight, best_ns))) return rc;  // also allocates the workspace
   if ((rc = trial(cur, inflight, best_ns))) return rc;
-  const int64_t inflights[] = {(int64_t)1 << 20, (int64_t)1 << 21, (int64_t)1 << 22};
   const int tiles[] = {1024, 4096, 16384};
   const int stacks[] = {8, 12, 16, 32};
   const int blocks[] = {4, 8, 16};
-  for (int64_t v : inflights) {
-    if (v == inflight) continue;
-    double ns;
-    if ((rc = trial(cur, v, ns))) return rc;
-    if (ns < best_ns) { best_ns = ns; inflight = v; }
-  }
   for (int v : tiles) {
     if (v == cur.tile_samples) continue;
     prt_batch_opts o = cur;
@@ -862,7 +890,7 @@ int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts*
     if ((rc = trial(o, inflight, ns))) return rc;
     if (ns < best_ns) { best_ns = ns; cur = o; }
   }
-  cur.inflight_tiles = (int32_t)std::max<int64_t>(1, inflight / cur.tile_samples);
+  cur.inflight_tiles = inflight > 0 ? (int32_t)std::max<int64_t>(1, inflight / cur.tile_samples) : 0;
   cur.tile_stride = base ? base->tile_stride : 0;
   // the persistent engine (every sample, one launch) at each register
   // budget, when the batch fits it and no subsample was asked for

@@ -66,6 +66,8 @@ int dev_upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
 // Ray / shadow queues as 17 / 12 separately allocated SoA columns.
 enum { RAYQ_COLS = 17, SHADOWQ_COLS = 12 };
 int alloc_ray_queue(DevBuf* cols, int64_t cap);
+size_t ray_entry_bytes();     // device bytes per ray-queue entry
+size_t shadow_entry_bytes();  // device bytes per shadow-queue entry
 int alloc_shadow_queue(DevBuf* cols, int64_t cap);
 RayQ ray_queue(DevBuf* b, int64_t cap);
 ShadowQ shadow_queue(DevBuf* b, int64_t cap);

@@ -84,11 +84,13 @@ def test_tuner_returns_valid_options(batch_env):
     print("tuned", d, "ns/sample", ns)
     assert ns > 0
     assert d["stack_lds"] in (8, 12, 16, 32) and d["trav_blocks_per_cu"] in (4, 8, 16)
-    assert d["tile_samples"] in (1024, 4096, 16384) and d["inflight_tiles"] >= 1
+    # inflight_tiles 0: the automatic in-flight size (the tuner keeps it)
+    assert d["tile_samples"] in (1024, 4096, 16384) and d["inflight_tiles"] == 0
     assert d["persistent"] in (0, 1, 2) and d["persist_waves"] in (0, 2, 3, 4)
+    assert d["use_graph"] in (0, 1)
     # the tuned options still render identically
     o2, v2, _ = _run_batch(torch, prt, scenes, **{k: d[k] for k in ("tile_samples", "inflight_tiles", "stack_lds",
                                                                       "trav_blocks_per_cu", "persistent",
-                                                                      "persist_waves")})
+                                                                      "persist_waves")}, use_graph=bool(d["use_graph"]))
     for i, o in enumerate(o2):
         assert torch.equal(o, ref[i][0])
