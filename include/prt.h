@@ -360,7 +360,7 @@ typedef struct prt_batch_opts {
     int32_t tile_stride;        /* render only every k-th tile (tuning trials; 0 = all) */
     int32_t max_depth;          /* safety cap on path vertices (0 = 128) */
     int32_t timing;             /* 1 = per-kernel device timing into prt_stats (plain launches only) */
-    int32_t persistent;         /* 0 = the replayed tile queue above.
+    int32_t persistent;         /* 0 = the tile queue above.
                                    1 = one persistent launch renders every camera sample of the batch:
                                    waves claim 64-sample chunks from one shared counter and follow each
                                    sample's whole path (shading, NEE, shadow rays, glass branches) with

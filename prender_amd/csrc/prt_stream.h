@@ -41,9 +41,9 @@
 // or earlier.  Shadow rays therefore stay out of the live count, and the
 // shadow claim counters have one region per queue parity (refill of
 // iteration i clears parity i's region, whose previous user, shadow i-2,
-// has finished).  Two iterations (one per queue parity) are captured as one
-// hipGraph and replayed until refill reports that the work queue is drained
-// and no tile is in flight.
+// has finished).  Two iterations (one per queue parity) are launched at a
+// time (directly, or replayed as one captured hipGraph) until refill reports
+// that the work queue is drained and no tile is in flight.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -100,7 +100,8 @@ struct ChunkDesc {
   uint32_t pad_;
 };
 
-// Control block (device memory; the host reads it after each graph replay).
+// Control block (device memory; the host reads it once per group of
+// iteration pairs).
 struct StreamCtl {
   uint32_t next_tile;        // first tile of the work queue not yet assigned
   uint32_t free_count;       // entries of free_list
