@@ -70,6 +70,7 @@ int dev_alloc(DevBuf& b, size_t bytes) {
   hipError_t e = hipMalloc(&b.p, bytes);
   if (e != hipSuccess) {
     b.p = nullptr;
+    (void)hipGetLastError();  // not left for the next launch's error check
     return fail(PRT_E_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes: " + hipGetErrorString(e));
   }
   b.bytes = bytes;
@@ -648,6 +649,10 @@ extern "C++" int prt::ensure_workspace(prt_scene* s, int64_t batch_samples, bool
   scap = std::max(scap, w.shadow_cap);
   two_sq = two_sq || w.sq2[0].p != nullptr;
   two_accum = two_accum || w.accum2.p != nullptr;
+  // recorded sizes only once every buffer holds them: a failed allocation
+  // has released its buffer, and the next call must not take the old sizes
+  // for granted
+  w.batch_samples = w.queue_cap = w.shadow_cap = 0;
   int rc;
   for (int qi = 0; qi < 2; ++qi)
     if ((rc = alloc_ray_queue(w.q[qi], qcap))) return rc;

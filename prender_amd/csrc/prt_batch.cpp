@@ -220,6 +220,7 @@ int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, const StreamG
     return fail(PRT_E_INVALID, "queue capacity above 2^32");
   int rc;
   if (w.q_cap < g.q_cap) {
+    w.q_cap = 0;  // until every column holds g.q_cap entries (a failed allocation released its buffer)
     for (int qi = 0; qi < 2; ++qi)
       if ((rc = alloc_ray_queue(w.q[qi], g.q_cap))) return rc;
     if ((rc = dev_alloc(w.hit_t, (size_t)g.q_cap * 8))) return rc;
@@ -227,6 +228,7 @@ int ensure_stream_workspace(prt_scene* lead, const BatchConfig& c, const StreamG
     w.q_cap = g.q_cap;
   }
   if (w.sq_cap < g.sq_cap) {
+    w.sq_cap = 0;
     if ((rc = alloc_shadow_queue(w.sq, g.sq_cap))) return rc;
     w.sq_cap = g.sq_cap;
   }
@@ -814,8 +816,17 @@ int prt_render_batch_device(prt_scene* const* scenes, int n_scenes, const prt_ba
   BatchConfig c;
   if ((rc = resolve_config(scenes, n_scenes, opts, c))) return rc;
   if (c.persistent == 1) return run_persistent(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
-  if (c.persistent == 2) return run_split(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
-  return run_stream(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
+  // the automatic in-flight size is halved (down to 2^20 samples) when its
+  // workspace cannot be allocated, e.g. beside the caller's own allocations
+  const bool auto_size = !opts || opts->inflight_tiles <= 0;
+  while (true) {
+    rc = c.persistent == 2 ? run_split(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr)
+                           : run_stream(scenes, n_scenes, c, d_rgb, d_var, stats, nullptr);
+    if (rc != PRT_E_NOMEM || !auto_size || c.n_slots <= 1 ||
+        ((int64_t)c.n_slots << c.tile_shift) <= ((int64_t)1 << 20))
+      return rc;
+    c.n_slots /= 2;
+  }
 }
 
 // Coordinate descent over the in-flight sample count, tile size, LDS stack
