@@ -829,10 +829,11 @@ int prt_render_batch_device(prt_scene* const* scenes, int n_scenes, const prt_ba
   }
 }
 
-// Coordinate descent over the in-flight sample count, tile size, LDS stack
-// depth and traversal grid, each trial rendering every k-th tile so that a
-// trial costs about 1/k of the whole batch; ranked by device time per camera
-// sample.
+// Coordinate descent over tile size, LDS stack depth, traversal grid and
+// launch form (the samples in flight stay the caller's or the automatic
+// size), each trial rendering every k-th tile so that a trial costs about
+// 1/k of the whole batch; ranked by device time per camera sample.  Then
+// the persistent and split forms on the whole batch.
 int prt_tune_batch(prt_scene* const* scenes, int n_scenes, const prt_batch_opts* base, float* const* d_rgb,
                    float* const* d_var, prt_batch_opts* best, double* best_ns_per_sample) {
   int rc;
